@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Benchmark: incoherent closest-hit Mrays/s over 1M triangles on one MI355X.
+
+Workload (one "step" = one launch of the closest-hit kernel over the whole batch):
+  scene  1,048,576-triangle soup, centroids U[0,1]^3, log-normal edges (median 2e-3, sigma 0.5),
+         random orientations (seed 1), one SAH BVH2 built by the host builder;
+  rays   16,777,216 incoherent rays: origins U[0,1]^3 (seed 2), directions uniform on the sphere
+         (seed 3), tMax U[0.1, 2] (seed 4); inputs resident in HBM before timing starts.
+Prints ONE JSON line. ``--gpus N`` with N > 1 is not supported (one GPU per box).
+
+roofline.achieved = algorithmic bytes of the launch / average launch time, with the per-ray
+figure B = 28 (ray + tMax) + 16 (hit record) + 64 x interior visits + 40 x triangle tests,
+the visit and test counts taken from the counting variant of the same kernel over the same rays
+(checked against the CPU oracle on the baseline sample). cpu_baseline = the CPU oracle (a port of
+the reference's mojo_cpu_traverse_batch) on the host cores, on a bounded sample of the same rays.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "incoherent Mrays/s @1M tris"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--tris", type=int, default=1 << 20)
+    ap.add_argument("--rays", type=int, default=1 << 24)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    return ap.parse_args()
+
+
+def main() -> int:
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "the traversal shards by rays with a replicated scene, which this build does not launch",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    from gonzales_amd import abi, scenes
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 2
+    lib = abi.load()
+    if not lib.gz_gpu_available():
+        print("no gfx950 device", file=sys.stderr)
+        return 2
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+
+    t0 = time.perf_counter()
+    bh = scenes.soup_hierarchy(args.tris, 1)
+    t_build = time.perf_counter() - t0
+    log(f"scene {args.tris} tris, {len(bh.nodes)} nodes, host build {t_build:.2f}s")
+    t0 = time.perf_counter()
+    bh.upload_to_gpu()
+    t_upload = time.perf_counter() - t0
+    info = bh.info()
+    log(f"upload {t_upload:.2f}s, device bytes {info.deviceBytes / 2**20:.1f} MiB, depth {info.maxDepth}")
+
+    rays, tmax = scenes.incoherent_rays(args.rays)
+    d_rays = torch.from_numpy(rays.view(np.uint8)).to(dev)
+    d_tmax = torch.from_numpy(tmax).to(dev)
+    n = args.rays
+    d_hits = torch.empty(n * 16, dtype=torch.uint8, device=dev)
+    d_stats = torch.empty(n * 8, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+
+    # Work counts for the algorithmic bytes (untimed counting pass).
+    bh.closest_device(d_rays, d_tmax, d_hits, d_stats)
+    bh.check_traversal()
+    st = d_stats.view(torch.int32).view(n, 2).to(torch.int64)
+    sum_interior = int(st[:, 0].sum())
+    sum_tris = int(st[:, 1].sum())
+    algo_bytes = 44 * n + 64 * sum_interior + 40 * sum_tris
+    hit_rate = float((d_hits.view(torch.int32).view(n, 4)[:, 0] >= 0).float().mean())
+    log(f"interior/ray {sum_interior / n:.1f}, tris/ray {sum_tris / n:.2f}, bytes/ray {algo_bytes / n:.0f}, "
+        f"hit rate {hit_rate:.3f}")
+
+    for _ in range(args.warmup):
+        bh.closest_device(d_rays, d_tmax, d_hits)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        bh.closest_device(d_rays, d_tmax, d_hits)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    bh.check_traversal()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    mrays = n / (ms * 1e-3) / 1e6
+    achieved = algo_bytes / (ms * 1e-3) / 1e9
+    log(f"closest: {ms:.3f} ms/launch, {mrays:.1f} Mrays/s (wall {n * args.steps / wall / 1e6:.1f})")
+
+    extra = {"host_build_s": round(t_build, 3), "upload_s": round(t_upload, 3),
+             "bytes_per_ray": round(algo_bytes / n, 1), "interior_per_ray": round(sum_interior / n, 2),
+             "tris_per_ray": round(sum_tris / n, 3), "hit_rate": round(hit_rate, 4),
+             "wall_mrays_per_s": round(n * args.steps / wall / 1e6, 2)}
+
+    def timed(fn, reps=5):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    if not args.no_extra:
+        d_occ = torch.empty(n, dtype=torch.uint8, device=dev)
+        ms_any = timed(lambda: bh.any_device(d_rays, d_tmax, d_occ))
+        extra["incoherent_anyhit_mrays_per_s"] = round(n / (ms_any * 1e-3) / 1e6, 2)
+        crays, ctmax = scenes.coherent_rays(4096)
+        d_crays = torch.from_numpy(crays.view(np.uint8)).to(dev)
+        d_ctmax = torch.from_numpy(ctmax).to(dev)
+        nc = len(crays)
+        d_chits = torch.empty(nc * 16, dtype=torch.uint8, device=dev)
+        ms_c = timed(lambda: bh.closest_device(d_crays, d_ctmax, d_chits))
+        ms_ca = timed(lambda: bh.any_device(d_crays, d_ctmax, d_occ))
+        extra["coherent_closest_mrays_per_s"] = round(nc / (ms_c * 1e-3) / 1e6, 2)
+        extra["coherent_anyhit_mrays_per_s"] = round(nc / (ms_ca * 1e-3) / 1e6, 2)
+        # End to end through host buffers (H2D + trace + Intersection_C expansion + D2H), 1M rays.
+        m = 1 << 20
+        t1 = time.perf_counter()
+        bh.intersect_batch_gpu(rays[:m], tmax[:m])
+        t2 = time.perf_counter()
+        extra["host_buffer_path_mrays_per_s"] = round(m / (t2 - t1) / 1e6, 2)
+        bh.check_traversal()
+        log(f"any-hit {extra['incoherent_anyhit_mrays_per_s']} Mrays/s, coherent "
+            f"{extra['coherent_closest_mrays_per_s']} / {extra['coherent_anyhit_mrays_per_s']}")
+
+    cpu = None
+    if not args.no_cpu:
+        import ctypes
+
+        import oracle
+
+        desc, meshes_c = bh.descriptor()
+        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        cores = max(1, min(cores, 16))
+        # calibrate, then size the sample to about --cpu-seconds of work
+        calib = 1 << 14
+        t1 = time.perf_counter()
+        oracle.traverse(ctypes.addressof(desc), len(bh.nodes), rays[:calib], tmax[:calib], abi.INTERSECTION, cores)
+        rate = calib / (time.perf_counter() - t1)
+        m = int(min(n, max(calib, rate * args.cpu_seconds)))
+        m -= m % 4096 if m > 4096 else 0
+        t1 = time.perf_counter()
+        ref, ref_st = oracle.traverse(ctypes.addressof(desc), len(bh.nodes), rays[:m], tmax[:m], abi.INTERSECTION,
+                                      cores, with_stats=True)
+        dt = time.perf_counter() - t1
+        cpu_rate = m / dt / 1e6
+        # parity of the timed GPU kernel on the same sample (bit-exact t, u, v and work counts)
+        hits = d_hits.view(torch.int32).view(n, 4)[:m].cpu().numpy().view(abi.GZ_HIT).reshape(-1)
+        gst = d_stats.view(torch.int32).view(n, 2)[:m].cpu().numpy().view(abi.TRACE_STATS).reshape(-1)
+        ok = bool(np.array_equal(hits["t"].view(np.uint32), ref["tHit"].view(np.uint32))
+                  and np.array_equal(hits["prim"] >= 0, ref["hit"].astype(bool))
+                  and gst.tobytes() == ref_st.tobytes())
+        extra["cpu_sample_parity_bit_exact"] = ok
+        cpu = {"value": round(cpu_rate, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
+               "sample": f"first {m} of the {n} incoherent rays, same scene and BVH2, "
+                         f"{cores} threads over 4096-ray chunks, {dt:.1f} s"}
+        log(f"cpu oracle {cpu_rate:.3f} Mrays/s on {cores} threads ({m} rays, {dt:.1f}s), parity {ok}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"closest-hit traversal, {args.tris} triangle soup, one SAH BVH2, "
+                               f"{n} incoherent rays (origins in the unit cube, directions uniform on the sphere, "
+                               "tMax U[0.1,2])",
+                   "tris": args.tris, "rays": n, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+    bh.free()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

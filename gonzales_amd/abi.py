@@ -1,0 +1,145 @@
+"""ctypes binding of ``include/gonzales_amd.h`` (the library ``libgonzales_amd.so``).
+
+numpy structured dtypes mirror the C structs byte for byte so host arrays can be handed to the
+C-ABI without copies; torch tensors are passed by ``data_ptr()`` for the ``*_device`` entry
+points. Loading fails loudly if the library has not been built: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libgonzales_amd.so"
+
+# Byte layouts of mojo_kernel.h:4-58 (the reference's C structs).
+PRIM_ID = np.dtype({"names": ["id1", "id2", "materialIndex", "type"],
+                    "formats": ["<i8", "<i8", "<i8", "i1"], "offsets": [0, 8, 16, 24], "itemsize": 32})
+MATERIAL = np.dtype({"names": ["type", "albedo", "emission"],
+                     "formats": ["i1", ("<f4", 3), ("<f4", 3)], "offsets": [0, 4, 16], "itemsize": 28})
+RAY = np.dtype({"names": ["orgX", "orgY", "orgZ", "dirX", "dirY", "dirZ"],
+                "formats": ["<f4"] * 6, "offsets": [0, 4, 8, 12, 16, 20], "itemsize": 24})
+INTERSECTION = np.dtype({"names": ["primId", "tHit", "u", "v", "hit"],
+                         "formats": [PRIM_ID, "<f4", "<f4", "<f4", "i1"],
+                         "offsets": [0, 32, 36, 40, 44], "itemsize": 48})
+PATH_STATE = np.dtype({"names": ["ray", "throughput", "estimate", "albedo", "pcgState", "pcgInc", "active"],
+                       "formats": [RAY, ("<f4", 3), ("<f4", 3), ("<f4", 3), "<u8", "<u8", "i1"],
+                       "offsets": [0, 24, 36, 48, 64, 72, 80], "itemsize": 88})
+BVH2_NODE = np.dtype({"names": ["boundsMinX", "boundsMinY", "boundsMinZ", "boundsMaxX", "boundsMaxY",
+                                "boundsMaxZ", "offset", "count"],
+                      "formats": ["<f4"] * 6 + ["<i4", "<i4"],
+                      "offsets": [0, 4, 8, 12, 16, 20, 24, 28], "itemsize": 32})
+GZ_HIT = np.dtype({"names": ["prim", "t", "u", "v"], "formats": ["<i4", "<f4", "<f4", "<f4"],
+                   "offsets": [0, 4, 8, 12], "itemsize": 16})
+TRACE_STATS = np.dtype({"names": ["interior", "tris"], "formats": ["<u4", "<u4"], "offsets": [0, 4],
+                        "itemsize": 8})
+
+GZ_OK = 0
+GZ_E_INVALID = -1
+GZ_E_NO_DEVICE = -2
+GZ_E_HIP = -3
+GZ_E_OVERFLOW = -4
+GZ_E_ALLOC = -5
+GZ_STACK_CAPACITY = 64
+
+
+class TriangleMeshC(ctypes.Structure):
+    _fields_ = [("points", ctypes.c_void_p), ("faceIndices", ctypes.c_void_p), ("vertexIndices", ctypes.c_void_p)]
+
+
+class SceneDescriptor2C(ctypes.Structure):
+    _fields_ = [("bvh2Nodes", ctypes.c_void_p), ("primIds", ctypes.c_void_p), ("meshes", ctypes.c_void_p),
+                ("meshCount", ctypes.c_int64)]
+
+
+class GzSceneInfo(ctypes.Structure):
+    _fields_ = [("nodeCount", ctypes.c_int64), ("interiorCount", ctypes.c_int64), ("primCount", ctypes.c_int64),
+                ("triRecordCount", ctypes.c_int64), ("meshCount", ctypes.c_int64), ("materialCount", ctypes.c_int64),
+                ("deviceBytes", ctypes.c_int64), ("maxDepth", ctypes.c_int32), ("rootIsLeaf", ctypes.c_int32)]
+
+
+class GzZSobolParams(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_int64), ("log2SamplesPerPixel", ctypes.c_int32), ("nBase4Digits", ctypes.c_int32)]
+
+
+class GzError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{message} (code {code})")
+        self.code = code
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "gz_gpu_available": (ctypes.c_bool, []),
+    "gz_gpu_upload_scene": (_P, [_P, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _I64]),
+    "gz_gpu_traverse_batch": (None, [_P, _P, _P, _I64, _P]),
+    "gz_gpu_shade_batch": (None, [_P, _P, _I64, _P]),
+    "gz_gpu_free_scene": (None, [_P]),
+    "gz_last_error": (ctypes.c_char_p, []),
+    "gz_gpu_scene_info": (ctypes.c_int, [_P, _P]),
+    "gz_gpu_occluded_batch": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
+    "gz_gpu_traverse_closest_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
+    "gz_gpu_traverse_any_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
+    "gz_gpu_traverse_closest_stats_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P]),
+    "gz_gpu_traverse_any_stats_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P]),
+    "gz_gpu_expand_hits_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
+    "gz_gpu_check_traversal": (ctypes.c_int, [_P, _P]),
+    "gz_gpu_shade_device": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
+    "gz_zsobol_params": (ctypes.c_int, [_I64, _I64, _I64, _I64, _P]),
+    "gz_zsobol_samples_device": (ctypes.c_int, [_P, _P, _P, _P, _I64, _I32, _P, _I32, _P, _I32, _P]),
+    "gz_zsobol_tile_device": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _I32, _P, _I32, _P]),
+    "gz_sobol_matrices": (ctypes.c_int, [_P]),
+    "gz_bvh2_build": (_I64, [_P, _I64, _P, _P]),
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads libgonzales_amd.so (built in-tree). Raises if it is missing: no fallback exists."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m gonzales_amd.build` "
+                           "(the HIP library is required, there is no CPU fallback)")
+    lib = ctypes.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | ctypes.RTLD_GLOBAL)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().gz_last_error().decode("utf-8", "replace")
+
+
+def check(code: int, what: str = "") -> None:
+    if code != GZ_OK:
+        raise GzError(code, f"{what}: {last_error()}" if what else last_error())
+
+
+def ptr(a) -> int:
+    """Address of a numpy array (host) or a torch tensor (device or host)."""
+    if a is None:
+        return 0
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return int(a.data_ptr())
+
+
+def stream_ptr(stream=None) -> int:
+    """hipStream_t of a torch stream (default: torch's current stream on the current device)."""
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)

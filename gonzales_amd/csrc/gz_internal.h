@@ -1,0 +1,93 @@
+// Internal declarations shared by the library's translation units (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+
+#include "gonzales_amd.h"
+
+namespace gz {
+
+// Child-pair record of one interior BVH2 node (64 bytes, one half of a 128-byte cache line).
+// Holds both children's boxes, which the reference reads from two separate 32-byte nodes per
+// interior visit (kernel.mojo:314-327), and both children's descriptors:
+//   desc >= 0 : the child is interior, desc is its record index;
+//   desc <  0 : the child is a leaf, ~desc is the index of its first triangle record.
+struct alignas(16) NodePair {
+        float4 a;  // left.minX, left.minY, left.minZ, left.maxX
+        float4 b;  // left.maxY, left.maxZ, right.minX, right.minY
+        float4 c;  // right.minZ, right.maxX, right.maxY, right.maxZ
+        int4 d;    // left desc, right desc, 0, 0
+};
+static_assert(sizeof(NodePair) == 64, "NodePair");
+
+// One primitive of a leaf, pre-gathered (48 bytes): the three vertices the reference fetches
+// through PrimId -> TriangleMesh_C -> vertexIndices -> points (kernel.mojo:266-300).
+//   p0.w : (first record of a leaf only) number of records in the leaf, as int bits
+//   p1.w : index into primIds of this primitive (the reference's hitIndex), as int bits
+//   p2.w : flags as int bits; bit 0 = skip (the reference `continue`s: id2 == -1 or an
+//          unknown type, kernel.mojo:273-277)
+struct alignas(16) TriRec {
+        float4 p0;
+        float4 p1;
+        float4 p2;
+};
+static_assert(sizeof(TriRec) == 48, "TriRec");
+
+enum : int { kTriSkip = 1 };
+
+// Device-resident mesh view for the shading kernels: TriangleMesh_C's pointers plus the element
+// counts passed at upload, so a kernel can reject an index the reference would read past.
+struct DevMesh {
+        const float *points;
+        const int64_t *faceIndices;
+        const int64_t *vertexIndices;
+        int64_t pointFloats;
+        int64_t vertexIndexCount;
+};
+
+struct Scene {
+        std::mutex mu;
+        int device = 0;
+        hipStream_t stream = nullptr;  // used by the synchronous host-buffer entry points
+
+        int64_t nodeCount = 0, interiorCount = 0, primCount = 0, triCount = 0;
+        int64_t meshCount = 0, materialCount = 0;
+        int32_t maxDepth = 0;
+        int32_t rootDesc = 0;
+        bool empty = true;  // no BVH nodes: every query misses
+
+        NodePair *dNodes = nullptr;
+        TriRec *dTris = nullptr;
+        PrimId_C *dPrims = nullptr;
+        DevMesh *dMeshes = nullptr;
+        Material_C *dMaterials = nullptr;
+        int *dErr = nullptr;  // bit 0: stack overflow seen
+        void **meshBuffers = nullptr;  // per-mesh device arrays to free
+        int64_t meshBufferCount = 0;
+        int64_t deviceBytes = 0;
+
+        // Grow-only staging for the synchronous entry points.
+        void *dStage = nullptr;
+        size_t stageBytes = 0;
+};
+
+void set_error(const char *fmt, ...);
+void clear_error();
+bool hip_ok(hipError_t e, const char *what);
+
+// Makes the scene's device current; returns false (error set) on failure.
+bool enter_device(Scene *s);
+// Ensures the staging buffer holds at least `bytes`.
+bool ensure_stage(Scene *s, size_t bytes);
+
+// Kernel launchers (gz_traverse.hip).
+int launch_trace(Scene *s, const Ray_C *d_rays, const float *d_tMax, int64_t n, GzHit *d_hits,
+                 uint8_t *d_occ, GzTraceStats *d_stats, bool anyHit, hipStream_t stream);
+int launch_expand(Scene *s, const GzHit *d_hits, const float *d_tMax, int64_t n, Intersection_C *d_out,
+                  hipStream_t stream);
+
+}  // namespace gz

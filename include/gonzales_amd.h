@@ -1,0 +1,308 @@
+/*
+ * gonzales_amd.h — C-ABI of the MI355X (gfx950) hot path of the Gonzales path tracer.
+ *
+ * Drop-in boundary. The structs are byte-identical to the reference's
+ * Sources/mojoKernel/include/mojo_kernel.h:4-58, so the Swift caller
+ * (Accelerators/BoundingHierarchy.swift, Core/Tile.swift) keeps its marshalling code and only
+ * swaps the imported module and the `mojo_gpu_*` symbol prefix for `gz_gpu_*`.
+ *
+ * Conventions of this library:
+ *   - every entry point is `extern "C"`, takes plain pointers and sizes, no HIP or torch types;
+ *     a stream is passed as `void *` (a hipStream_t, NULL = the legacy default stream);
+ *   - `*_device` entry points take device pointers and are asynchronous on the given stream;
+ *     every other entry point takes host pointers and is synchronous (results ready on return),
+ *     as the reference's are (kernel.mojo:608-673, 788-846);
+ *   - the `gz_gpu_*` functions that mirror `mojo_gpu_*` keep their `void`/pointer returns; on
+ *     failure they leave outputs untouched (kernel.mojo:672-673) and record a message that
+ *     gz_last_error() returns; new entry points return GZ_OK (0) or a negative GZ_E_* code;
+ *   - a scene handle may be used from several host threads at once: each call serialises on an
+ *     internal per-handle lock (the reference shares one DeviceContext across tile tasks,
+ *     TileRenderer.swift:77-107).
+ */
+#ifndef GONZALES_AMD_H
+#define GONZALES_AMD_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------------------- */
+/* Structs shared with the reference (mojo_kernel.h:4-58).                                   */
+/* ---------------------------------------------------------------------------------------- */
+
+/* mojo_kernel.h:4-9. type: 0 triangle (id1 mesh, id2 = 3*triangle), 1 geometric primitive,
+ * 2 transformed primitive, 3 area light; for 1..3 id2 = mesh<<32 | triangle, or -1 for a
+ * non-triangle shape (BoundingHierarchy.swift:21-58, 479-484). 32 bytes. */
+struct PrimId_C {
+        int64_t id1;
+        int64_t id2;
+        int64_t materialIndex;
+        int8_t type;
+};
+
+/* mojo_kernel.h:11-15. type: 0 none, 1 diffuse, 2 area light. 28 bytes. */
+struct Material_C {
+        int8_t type;
+        float albedo[3];
+        float emission[3];
+};
+
+/* mojo_kernel.h:17-21. points: 4 floats per vertex (x, y, z, 1); indices are int64. */
+struct TriangleMesh_C {
+        const float *points;
+        const int64_t *faceIndices;
+        const int64_t *vertexIndices;
+};
+
+/* mojo_kernel.h:23-26. 24 bytes. */
+struct Ray_C {
+        float orgX, orgY, orgZ;
+        float dirX, dirY, dirZ;
+};
+
+/* mojo_kernel.h:28-34. 48 bytes; a miss carries PrimId(-1, -1, 0, 0), tHit = tMax. */
+struct Intersection_C {
+        struct PrimId_C primId;
+        float tHit;
+        float u;
+        float v;
+        int8_t hit;
+};
+
+/* mojo_kernel.h:36-44. 88 bytes. */
+struct PathState_C {
+        struct Ray_C ray;
+        float throughput[3];
+        float estimate[3];
+        float albedo[3];
+        uint64_t pcgState;
+        uint64_t pcgInc;
+        int8_t active;
+};
+
+/* mojo_kernel.h:46-51. Depth-first BVH2: left child = index + 1, right child = offset;
+ * count == 0 marks an interior node, count > 0 a leaf over primIds[offset, offset+count). */
+struct BVH2Node {
+        float boundsMinX, boundsMinY, boundsMinZ;
+        float boundsMaxX, boundsMaxY, boundsMaxZ;
+        int32_t offset;
+        int32_t count;
+};
+
+/* mojo_kernel.h:53-58 (the C definition, 32 bytes; the Mojo side declares two more fields,
+ * kernel.mojo:208-215, which no C caller fills). */
+struct SceneDescriptor2_C {
+        const struct BVH2Node *bvh2Nodes;
+        const struct PrimId_C *primIds;
+        const struct TriangleMesh_C *meshes;
+        int64_t meshCount;
+};
+
+/* ---------------------------------------------------------------------------------------- */
+/* Types introduced by this library.                                                         */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Compact closest-hit record written by the device-resident traversal (16 bytes).
+ * prim is the index into the scene's primIds array (the reference's `hitIndex`,
+ * kernel.mojo:306) or -1 on a miss; on a miss t = tMax and u = v = 0 (kernel.mojo:360-364). */
+struct GzHit {
+        int32_t prim;
+        float t;
+        float u;
+        float v;
+};
+
+/* Per-ray traversal work counters (used for the algorithmic-bytes figure):
+ * interior = interior nodes whose two child boxes were tested, tris = triangle tests run. */
+struct GzTraceStats {
+        uint32_t interior;
+        uint32_t tris;
+};
+
+/* Scene summary filled by gz_gpu_scene_info. */
+struct GzSceneInfo {
+        int64_t nodeCount;      /* BVH2 nodes passed at upload */
+        int64_t interiorCount;  /* child-pair records on the device */
+        int64_t primCount;      /* primIds */
+        int64_t triRecordCount; /* 48-byte triangle records on the device (leaf order) */
+        int64_t meshCount;
+        int64_t materialCount;
+        int64_t deviceBytes;    /* device memory held by the handle */
+        int32_t maxDepth;       /* deepest root-to-leaf path (edges) */
+        int32_t rootIsLeaf;
+};
+
+enum {
+        GZ_OK = 0,
+        GZ_E_INVALID = -1,    /* bad argument or malformed scene */
+        GZ_E_NO_DEVICE = -2,  /* no usable gfx950 device */
+        GZ_E_HIP = -3,        /* a HIP runtime call failed */
+        GZ_E_OVERFLOW = -4,   /* a traversal needed more than GZ_STACK_CAPACITY stack entries */
+        GZ_E_ALLOC = -5       /* device or host allocation failed */
+};
+
+/* Traversal stack capacity per ray, the reference's InlineArray[Int, 64] (kernel.mojo:250).
+ * The reference does not check it; this library stops the ray and reports GZ_E_OVERFLOW. */
+#define GZ_STACK_CAPACITY 64
+
+/* ---------------------------------------------------------------------------------------- */
+/* Entry points that replace the reference's GPU half of mojo_kernel.h.                       */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Replaces mojo_gpu_available (mojo_kernel.h:65; kernel.mojo:407-410). True when a gfx950
+ * device is visible and the kernels in this library load on it. */
+bool gz_gpu_available(void);
+
+/* Replaces mojo_gpu_upload_scene (mojo_kernel.h:70-75; kernel.mojo:430-587). Same arguments:
+ * per-mesh point counts are in float elements (4 per vertex), index counts in int64 elements
+ * (BoundingHierarchy.swift:160-172). Copies and re-lays out the scene into device memory
+ * (64-byte child-pair node records, 48-byte triangle records in leaf order); keeps no host
+ * pointer. Returns NULL on failure (kernel.mojo:583-585) with gz_last_error() set. */
+void *gz_gpu_upload_scene(const struct BVH2Node *bvh2Nodes, int64_t bvh2NodesCount,
+                          const struct PrimId_C *primIds, int64_t primIdsCount,
+                          const struct TriangleMesh_C *meshes, int64_t meshCount,
+                          const int64_t *meshPointsCounts, const int64_t *meshFaceIndicesCounts,
+                          const int64_t *meshVertexIndicesCounts, const struct Material_C *materials,
+                          int64_t materialCount);
+
+/* Replaces mojo_gpu_traverse_batch (mojo_kernel.h:77-78; kernel.mojo:608-673): closest hit of
+ * `count` host rays with per-ray tMax, results into host `results`. NULL handle or count == 0
+ * is a no-op (kernel.mojo:616-622). */
+void gz_gpu_traverse_batch(void *handle, const struct Ray_C *rays, const float *tMaxValues,
+                           int64_t count, struct Intersection_C *results);
+
+/* Replaces mojo_gpu_shade_batch (mojo_kernel.h:80-81; kernel.mojo:788-846): the diffuse
+ * wavefront shade of the reference's Tile.render loop (kernel.mojo:676-773), in place. */
+void gz_gpu_shade_batch(void *handle, struct PathState_C *paths, int64_t count,
+                        const struct Intersection_C *intersections);
+
+/* Replaces mojo_gpu_free_scene (mojo_kernel.h:91; kernel.mojo:848-854). NULL is a no-op. */
+void gz_gpu_free_scene(void *handle);
+
+/* ---------------------------------------------------------------------------------------- */
+/* New entry points (status-returning).                                                        */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char *gz_last_error(void);
+
+/* Fills *info for a handle returned by gz_gpu_upload_scene. */
+int gz_gpu_scene_info(void *handle, struct GzSceneInfo *info);
+
+/* Occlusion of `count` host rays: occluded[i] = 1 iff the closest-hit traversal of ray i with
+ * tMaxValues[i] reports a hit, which is what BoundingHierarchy.intersect(scene:ray:tHit:) ->
+ * Bool answers (BoundingHierarchy.swift:348-384) and Visibility.occluded consumes
+ * (Core/Visibility.swift:5-24). Runs the any-hit kernel, which stops at the first accepted
+ * triangle. Synchronous, host buffers. */
+int gz_gpu_occluded_batch(void *handle, const struct Ray_C *rays, const float *tMaxValues,
+                          int64_t count, uint8_t *occluded);
+
+/* Device-resident closest hit: d_rays, d_tMax, d_hits are device pointers; asynchronous on
+ * `stream`. The batch form behind mojo_gpu_traverse_batch, without the host round trip. */
+int gz_gpu_traverse_closest_device(void *handle, const struct Ray_C *d_rays, const float *d_tMax,
+                                   int64_t count, struct GzHit *d_hits, void *stream);
+
+/* Device-resident any-hit (occlusion): d_occluded[i] in {0, 1}. Asynchronous on `stream`. */
+int gz_gpu_traverse_any_device(void *handle, const struct Ray_C *d_rays, const float *d_tMax,
+                               int64_t count, uint8_t *d_occluded, void *stream);
+
+/* Device-resident closest hit that also counts per-ray work (d_stats may be NULL for hits only).
+ * Same results as gz_gpu_traverse_closest_device; slower, used for the byte accounting. */
+int gz_gpu_traverse_closest_stats_device(void *handle, const struct Ray_C *d_rays,
+                                         const float *d_tMax, int64_t count, struct GzHit *d_hits,
+                                         struct GzTraceStats *d_stats, void *stream);
+
+/* Device-resident any-hit with per-ray work counters (counted up to the early exit). */
+int gz_gpu_traverse_any_stats_device(void *handle, const struct Ray_C *d_rays, const float *d_tMax,
+                                     int64_t count, uint8_t *d_occluded,
+                                     struct GzTraceStats *d_stats, void *stream);
+
+/* Expands compact hits into the reference's 48-byte Intersection_C records (primIds gathered on
+ * the device; misses become PrimId(-1, -1, 0, 0), tHit = tMax). d_tMax supplies tMax for misses.
+ * Asynchronous on `stream`. */
+int gz_gpu_expand_hits_device(void *handle, const struct GzHit *d_hits, const float *d_tMax,
+                              int64_t count, struct Intersection_C *d_results, void *stream);
+
+/* Waits for `stream` and returns GZ_E_OVERFLOW if any traversal since the last call stopped on
+ * a full stack (the flag is then cleared), else GZ_OK. */
+int gz_gpu_check_traversal(void *handle, void *stream);
+
+/* Device-resident diffuse shade: the kernel behind gz_gpu_shade_batch without the host round
+ * trip (kernel.mojo:676-786). d_paths is updated in place. Asynchronous on `stream`. */
+int gz_gpu_shade_device(void *handle, struct PathState_C *d_paths, int64_t count,
+                        const struct Intersection_C *d_isects, void *stream);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Z-Sobol sampler (replaces the Swift ZSobolSampler, Sampler/ZSobolSampler.swift).            */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Sampler constants of ZSobolSampler.init (ZSobolSampler.swift:114-122). */
+struct GzZSobolParams {
+        int64_t seed;                /* the reference's .pbrt path hard-codes 1234 (Random.swift:66) */
+        int32_t log2SamplesPerPixel; /* trailing zero bits of spp (non-powers of 2 truncate) */
+        int32_t nBase4Digits;        /* log2(nextPow2(max(w, h))) + (log2spp + 1) / 2 */
+};
+
+/* Host: fills *out as ZSobolSampler(samplesPerPixel:fullResolution:seed:) does. */
+int gz_zsobol_params(int64_t samplesPerPixel, int64_t resX, int64_t resY, int64_t seed,
+                     struct GzZSobolParams *out);
+
+/* For each i < count: startPixelSample(pixel (d_px[i], d_py[i]), index d_sampleIndex[i], dim
+ * startDim) (ZSobolSampler.swift:130-136), then the calls of `pattern` (host array of nCalls
+ * entries, each 1, 2 or 3 = get1D / get2D / get3D, ZSobolSampler.swift:203-280), values in call
+ * order. layout 0: d_out[i * stride + k]; layout 1: d_out[k * count + i]; stride = sum(pattern).
+ * nCalls <= 64. Asynchronous on `stream`. */
+int gz_zsobol_samples_device(const struct GzZSobolParams *params, const int32_t *d_px, const int32_t *d_py,
+                             const int64_t *d_sampleIndex, int64_t count, int32_t startDim,
+                             const int8_t *pattern, int32_t nCalls, float *d_out, int32_t layout,
+                             void *stream);
+
+/* Same for every sample of a pixel tile in the order of Tile.render (Core/Tile.swift:28-29):
+ * sample i is pixel (x0 + (i / spp) % width, y0 + (i / spp) / width), index i % spp. */
+int gz_zsobol_tile_device(const struct GzZSobolParams *params, int32_t x0, int32_t y0, int32_t width,
+                          int32_t height, int32_t spp, int32_t startDim, const int8_t *pattern,
+                          int32_t nCalls, float *d_out, int32_t layout, void *stream);
+
+/* Host: the 1024 x 52 Sobol generator matrices (column-per-bit uint32) the sampler uses, as the
+ * reference's SobolGenerator emits them (SobolGenerator/main.swift:117-185). */
+int gz_sobol_matrices(uint32_t *out);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Host-side helper (no device work): the SAH BVH2 builder the Swift host runs before upload.  */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Restates BoundingHierarchyBuilder (Accelerators/BoundingHierarchyBuilder.swift:66-339):
+ * binned SAH, 12 buckets on the centroid bounds' longest axis, leaves of up to 4 primitives,
+ * Hoare partition by bucket, depth-first flatten (left = i + 1, right = offset).
+ * primBounds: n x 6 floats (minX, minY, minZ, maxX, maxY, maxZ) per primitive.
+ * nodesOut: capacity 2n - 1 (n >= 1). primOrderOut: n entries; the primitive stored at BVH
+ * position k is primOrderOut[k] (BoundingHierarchyBuilder.swift:102-108).
+ * Returns the node count (0 for n == 0), or a negative GZ_E_* code. */
+int64_t gz_bvh2_build(const float *primBounds, int64_t n, struct BVH2Node *nodesOut,
+                      int64_t *primOrderOut);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#ifdef __cplusplus
+static_assert(sizeof(struct PrimId_C) == 32, "PrimId_C layout");
+static_assert(sizeof(struct Material_C) == 28, "Material_C layout");
+static_assert(sizeof(struct TriangleMesh_C) == 24, "TriangleMesh_C layout");
+static_assert(sizeof(struct Ray_C) == 24, "Ray_C layout");
+static_assert(sizeof(struct Intersection_C) == 48, "Intersection_C layout");
+static_assert(offsetof(struct Intersection_C, tHit) == 32, "Intersection_C.tHit");
+static_assert(offsetof(struct Intersection_C, hit) == 44, "Intersection_C.hit");
+static_assert(sizeof(struct PathState_C) == 88, "PathState_C layout");
+static_assert(offsetof(struct PathState_C, pcgState) == 64, "PathState_C.pcgState");
+static_assert(offsetof(struct PathState_C, active) == 80, "PathState_C.active");
+static_assert(sizeof(struct BVH2Node) == 32, "BVH2Node layout");
+static_assert(sizeof(struct SceneDescriptor2_C) == 32, "SceneDescriptor2_C layout");
+static_assert(sizeof(struct GzHit) == 16, "GzHit layout");
+#endif
+
+#endif /* GONZALES_AMD_H */

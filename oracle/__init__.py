@@ -1,0 +1,160 @@
+"""ORACLE — test infrastructure only.
+
+ctypes front end of ``oracle/liboracle.so``, the CPU restatement of the reference's hot path
+(Sources/mojoKernel/kernel.mojo traversal + shade, Sampler/ZSobolSampler.swift). Imported only by
+tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg, always as the checker; the
+product (``gonzales_amd``) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+JOE_KUO = HERE.parent / "gonzales_amd" / "data" / "new-joe-kuo-6.1023"
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+
+_SIGS = {
+    "orc_intersect_triangle": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_float, _P, _P, _P]),
+    "orc_intersect_aabb": (ctypes.c_int, [_P, _P, _P, ctypes.c_float, _P]),
+    "orc_traverse_one": (ctypes.c_int, [_P, _I64, _P, ctypes.c_float, _P, _P]),
+    "orc_traverse_batch": (ctypes.c_int, [_P, _I64, _P, _P, _I64, _P, _P, ctypes.c_int]),
+    "orc_occluded_batch": (ctypes.c_int, [_P, _I64, _P, _P, _I64, _P, _P, ctypes.c_int]),
+    "orc_brute_closest": (None, [_P, _I64, _P, _P, _I64, _P, _P]),
+    "orc_reverse_bits32": (ctypes.c_uint32, [ctypes.c_uint32]),
+    "orc_owen": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
+    "orc_encode_morton2": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32]),
+    "orc_mix_bits": (ctypes.c_uint32, [ctypes.c_uint64]),
+    "orc_next_pow2": (_I64, [_I64]),
+    "orc_sobol_matrices": (ctypes.c_int, [ctypes.c_char_p, _P]),
+    "orc_sobol_sample": (ctypes.c_float, [_I64, ctypes.c_int, ctypes.c_uint32, _P]),
+    "orc_zsobol_init": (None, [_P, _I64, _I64, _I64, _I64]),
+    "orc_zsobol_start": (None, [_P, _I32, _I32, _I64, _I32]),
+    "orc_zsobol_sample_index": (ctypes.c_uint64, [_P]),
+    "orc_zsobol_get1d": (ctypes.c_float, [_P, _P]),
+    "orc_zsobol_get2d": (None, [_P, _P, _P]),
+    "orc_zsobol_get3d": (None, [_P, _P, _P]),
+    "orc_zsobol_batch": (None, [_I64, _I64, _I64, _I64, _P, _P, _P, _P, _I64, _I32, _P, _I32, _P]),
+    "orc_pcg32": (None, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _P]),
+    "orc_shade_batch": (None, [_P, _I64, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError(f"{LIB} is not built (make -C oracle)")
+        l = ctypes.CDLL(str(LIB))
+        for name, (res, args) in _SIGS.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def _p(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+STATS = np.dtype({"names": ["interior", "tris"], "formats": ["<u4", "<u4"], "offsets": [0, 4], "itemsize": 8})
+
+
+class ZSobolState(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_int64), ("log2spp", ctypes.c_int32), ("nBase4Digits", ctypes.c_int32),
+                ("morton", ctypes.c_uint64), ("dim", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+# ---- traversal -------------------------------------------------------------------------------
+
+def intersect_triangle(org, d, p0, p1, p2, t_max):
+    a = [np.ascontiguousarray(x, np.float32) for x in (org, d, p0, p1, p2)]
+    t, u, v = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    hit = lib().orc_intersect_triangle(*[_p(x) for x in a], float(t_max), ctypes.byref(t), ctypes.byref(u),
+                                       ctypes.byref(v))
+    return bool(hit), t.value, u.value, v.value
+
+
+def intersect_aabb(bounds, org, d, t_max):
+    a = [np.ascontiguousarray(x, np.float32) for x in (bounds, org, d)]
+    tn = ctypes.c_float()
+    hit = lib().orc_intersect_aabb(*[_p(x) for x in a], float(t_max), ctypes.byref(tn))
+    return bool(hit), tn.value
+
+
+def traverse(desc_ptr: int, node_count: int, rays: np.ndarray, t_max: np.ndarray, isect_dtype, nthreads: int = 1,
+             with_stats: bool = False):
+    """Closest hit for every ray (mojo_cpu_traverse_batch). Returns (Intersection array, stats|None)."""
+    n = len(rays)
+    out = np.zeros(n, isect_dtype)
+    st = np.zeros(n, STATS) if with_stats else None
+    rc = lib().orc_traverse_batch(desc_ptr, node_count, _p(rays), _p(np.ascontiguousarray(t_max, np.float32)), n,
+                                  _p(out), _p(st) if st is not None else None, nthreads)
+    if rc != 0:
+        raise RuntimeError("oracle traversal stack overflow")
+    return out, st
+
+
+def occluded(desc_ptr: int, node_count: int, rays: np.ndarray, t_max: np.ndarray, nthreads: int = 1,
+             with_stats: bool = False):
+    n = len(rays)
+    out = np.zeros(n, np.uint8)
+    st = np.zeros(n, STATS) if with_stats else None
+    rc = lib().orc_occluded_batch(desc_ptr, node_count, _p(rays), _p(np.ascontiguousarray(t_max, np.float32)), n,
+                                  _p(out), _p(st) if st is not None else None, nthreads)
+    if rc != 0:
+        raise RuntimeError("oracle traversal stack overflow")
+    return out.astype(bool), st
+
+
+def brute_closest(desc_ptr: int, prim_count: int, rays: np.ndarray, t_max: np.ndarray):
+    n = len(rays)
+    hi = np.zeros(n, np.int64)
+    t = np.zeros(n, np.float32)
+    lib().orc_brute_closest(desc_ptr, prim_count, _p(rays), _p(np.ascontiguousarray(t_max, np.float32)), n, _p(hi),
+                            _p(t))
+    return hi, t
+
+
+# ---- Z-Sobol ---------------------------------------------------------------------------------
+
+def sobol_matrices() -> np.ndarray:
+    out = np.zeros(1024 * 52, np.uint32)
+    if lib().orc_sobol_matrices(str(JOE_KUO).encode(), _p(out)) != 0:
+        raise RuntimeError(f"cannot read {JOE_KUO}")
+    return out
+
+
+def zsobol_batch(spp, res_x, res_y, seed, px, py, sample_index, pattern, start_dim=0, table=None):
+    table = sobol_matrices() if table is None else table
+    px = np.ascontiguousarray(px, np.int32)
+    py = np.ascontiguousarray(py, np.int32)
+    si = np.ascontiguousarray(sample_index, np.int64)
+    pat = np.ascontiguousarray(pattern, np.int8)
+    stride = int(pat.sum())
+    out = np.zeros((len(px), stride), np.float32)
+    lib().orc_zsobol_batch(spp, res_x, res_y, seed, _p(table), _p(px), _p(py), _p(si), len(px), start_dim, _p(pat),
+                           len(pat), _p(out))
+    return out
+
+
+# ---- shade (mode A) --------------------------------------------------------------------------
+
+def pcg32(initstate: int, initseq: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    lib().orc_pcg32(initstate, initseq, n, _p(out))
+    return out
+
+
+def shade(paths: np.ndarray, isects: np.ndarray, meshes_c_addr: int, materials: np.ndarray) -> None:
+    """mojo_cpu_shade_batch: updates `paths` (PathState_C records) in place."""
+    lib().orc_shade_batch(_p(paths), len(paths), _p(isects), meshes_c_addr, _p(materials))
