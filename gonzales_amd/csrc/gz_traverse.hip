@@ -201,14 +201,14 @@ __global__ __launch_bounds__(kBlock) void k_miss(const float *__restrict__ tMaxI
 
 // kernel.mojo:360-364: Intersection_C from a compact hit.
 __global__ __launch_bounds__(kBlock) void k_expand(const GzHit *__restrict__ hits, const float *__restrict__ tMaxIn,
-                                                   const PrimId_C *__restrict__ prims, int64_t n,
-                                                   Intersection_C *__restrict__ out) {
+                                                   const PrimId_C *__restrict__ prims, int64_t primCount,
+                                                   int64_t n, Intersection_C *__restrict__ out) {
         const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) return;
         const GzHit h = hits[i];
         Intersection_C r;
         memset(&r, 0, sizeof r);
-        if (h.prim >= 0) {
+        if (h.prim >= 0 && h.prim < primCount) {
                 r.primId = prims[h.prim];
                 r.tHit = h.t;
                 r.u = h.u;
@@ -256,7 +256,7 @@ int launch_trace(Scene *s, const Ray_C *d_rays, const float *d_tMax, int64_t n, 
 int launch_expand(Scene *s, const GzHit *d_hits, const float *d_tMax, int64_t n, Intersection_C *d_out,
                   hipStream_t stream) {
         if (n <= 0) return GZ_OK;
-        hipLaunchKernelGGL(k_expand, dim3(grid_for(n)), dim3(kBlock), 0, stream, d_hits, d_tMax, s->dPrims, n, d_out);
+        hipLaunchKernelGGL(k_expand, dim3(grid_for(n)), dim3(kBlock), 0, stream, d_hits, d_tMax, s->dPrims, s->primCount, n, d_out);
         return hip_ok(hipGetLastError(), "expand launch") ? GZ_OK : GZ_E_HIP;
 }
 
