@@ -70,4 +70,15 @@ def f32_bits(a: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(a, np.float32).view(np.uint32)
 
 
+def assert_isect_equal(a: np.ndarray, b: np.ndarray) -> None:
+    """Intersection_C records equal field by field (floats bitwise; struct padding ignored, since
+    numpy does not keep padding bytes through np.load)."""
+    assert len(a) == len(b)
+    for f in ("id1", "id2", "materialIndex", "type"):
+        assert np.array_equal(a["primId"][f], b["primId"][f]), f
+    for f in ("tHit", "u", "v"):
+        assert np.array_equal(f32_bits(a[f]), f32_bits(b[f])), f
+    assert np.array_equal(a["hit"], b["hit"])
+
+
 __all__ = ["random_soup_hierarchy", "random_rays", "OracleView", "f32_bits", "scenes"]

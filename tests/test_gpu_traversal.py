@@ -12,7 +12,7 @@ import pytest
 
 from gonzales_amd import abi, scenes
 from gonzales_amd.accelerator import BoundingHierarchy, Scene, TriangleMesh
-from helpers import OracleView, random_rays, random_soup_hierarchy
+from helpers import OracleView, assert_isect_equal, random_rays, random_soup_hierarchy
 
 pytestmark = pytest.mark.gpu
 
@@ -72,7 +72,7 @@ def test_golden_intersection_records(gpu, name):
     bh = _upload_fixture(data, name)
     rays, tmax = data[name + "_rays"], data[name + "_tmax"]
     got = bh.intersect_batch_gpu(rays, tmax)  # gz_gpu_traverse_batch (host buffers)
-    assert got.tobytes() == data[name + "_isect"].tobytes()
+    assert_isect_equal(got, data[name + "_isect"])
     assert np.array_equal(bh.occluded(rays, tmax), data[name + "_occluded"].astype(bool))
     _, st = _device_closest(bh, rays, tmax, stats=True)
     assert st.tobytes() == data[name + "_stats"].tobytes()
