@@ -61,6 +61,11 @@ class GzSceneInfo(ctypes.Structure):
                 ("deviceBytes", ctypes.c_int64), ("maxDepth", ctypes.c_int32), ("rootIsLeaf", ctypes.c_int32)]
 
 
+class GzTraceConfig(ctypes.Structure):
+    _fields_ = [("persistent", ctypes.c_int32), ("ldsStack", ctypes.c_int32), ("refillLanes", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
 class GzZSobolParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_int64), ("log2SamplesPerPixel", ctypes.c_int32), ("nBase4Digits", ctypes.c_int32)]
 
@@ -91,6 +96,8 @@ SIGNATURES = {
     "gz_gpu_traverse_any_stats_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P]),
     "gz_gpu_expand_hits_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
     "gz_gpu_check_traversal": (ctypes.c_int, [_P, _P]),
+    "gz_gpu_set_trace_config": (ctypes.c_int, [_P, _P]),
+    "gz_gpu_get_trace_config": (ctypes.c_int, [_P, _P]),
     "gz_gpu_shade_device": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
     "gz_zsobol_params": (ctypes.c_int, [_I64, _I64, _I64, _I64, _P]),
     "gz_zsobol_samples_device": (ctypes.c_int, [_P, _P, _P, _P, _I64, _I32, _P, _I32, _P, _I32, _P]),

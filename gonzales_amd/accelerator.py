@@ -275,6 +275,15 @@ class BoundingHierarchy:
                                                       abi.ptr(occ_t), abi.ptr(stats_t), abi.stream_ptr(stream))
         abi.check(rc, "any_device")
 
+    def set_trace_config(self, persistent: int = 1, lds_stack: int = 16, refill_lanes: int = 16) -> None:
+        cfg = abi.GzTraceConfig(persistent, lds_stack, refill_lanes, 0)
+        abi.check(abi.load().gz_gpu_set_trace_config(self.handle, ctypes.byref(cfg)), "gz_gpu_set_trace_config")
+
+    def trace_config(self) -> abi.GzTraceConfig:
+        cfg = abi.GzTraceConfig()
+        abi.check(abi.load().gz_gpu_get_trace_config(self.handle, ctypes.byref(cfg)), "gz_gpu_get_trace_config")
+        return cfg
+
     def check_traversal(self, stream=None) -> None:
         abi.check(abi.load().gz_gpu_check_traversal(self.handle, abi.stream_ptr(stream)), "gz_gpu_check_traversal")
 

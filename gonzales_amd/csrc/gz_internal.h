@@ -73,7 +73,16 @@ struct Scene {
         // Grow-only staging for the synchronous entry points.
         void *dStage = nullptr;
         size_t stageBytes = 0;
+
+        // Traversal launch configuration and the persistent kernel's ray counters (a ring, so
+        // launches queued on different streams never share a counter).
+        GzTraceConfig traceConfig{1, 16, 16, 0};
+        int cuCount = 256;
+        unsigned long long *dCounters = nullptr;
+        uint64_t launchSeq = 0;
 };
+
+constexpr int kCounterRing = 1024;
 
 void set_error(const char *fmt, ...);
 void clear_error();

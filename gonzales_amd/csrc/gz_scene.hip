@@ -65,7 +65,7 @@ static void free_scene(Scene *s) {
         int cur = -1;
         if (hipGetDevice(&cur) == hipSuccess && cur != s->device) (void)hipSetDevice(s->device);
         if (s->stream) (void)hipStreamSynchronize(s->stream);
-        void *ptrs[] = {s->dNodes, s->dTris, s->dPrims, s->dMeshes, s->dMaterials, s->dErr, s->dStage};
+        void *ptrs[] = {s->dNodes, s->dTris, s->dPrims, s->dMeshes, s->dMaterials, s->dErr, s->dStage, s->dCounters};
         for (void *p : ptrs)
                 if (p) (void)hipFree(p);
         for (int64_t i = 0; i < s->meshBufferCount; ++i)
@@ -284,7 +284,12 @@ extern "C" void *gz_gpu_upload_scene(const BVH2Node *nodes, int64_t nNodes, cons
                   upload(&s->dPrims, prims, nPrims, s, "upload primIds") &&
                   upload(&s->dMaterials, materials, nMaterials, s, "upload materials") &&
                   hip_ok(hipMalloc((void **)&s->dErr, sizeof(int)), "hipMalloc(err)") &&
-                  hip_ok(hipMemset(s->dErr, 0, sizeof(int)), "hipMemset(err)");
+                  hip_ok(hipMemset(s->dErr, 0, sizeof(int)), "hipMemset(err)") &&
+                  hip_ok(hipMalloc((void **)&s->dCounters, sizeof(unsigned long long) * kCounterRing),
+                         "hipMalloc(counters)") &&
+                  hip_ok(hipDeviceGetAttribute(&s->cuCount, hipDeviceAttributeMultiprocessorCount, s->device),
+                         "hipDeviceGetAttribute");
+        s->deviceBytes += (int64_t)(sizeof(unsigned long long) * kCounterRing + sizeof(int));
         if (ok && nMeshes > 0) {
                 // Per-mesh arrays for the shading kernels (kernel.mojo:488-531 uploads the same).
                 std::vector<DevMesh> dm((size_t)nMeshes);
@@ -361,5 +366,36 @@ extern "C" int gz_gpu_check_traversal(void *handle, void *stream) {
                 set_error("traversal stack overflow: a ray needed more than %d entries", GZ_STACK_CAPACITY);
                 return GZ_E_OVERFLOW;
         }
+        return GZ_OK;
+}
+
+extern "C" int gz_gpu_set_trace_config(void *handle, const GzTraceConfig *config) {
+        clear_error();
+        if (!handle || !config) {
+                set_error("gz_gpu_set_trace_config: null argument");
+                return GZ_E_INVALID;
+        }
+        if (config->persistent < 0 || config->persistent > 8 ||
+            (config->ldsStack != 0 && config->ldsStack != 8 && config->ldsStack != 16) || config->refillLanes < 1 ||
+            config->refillLanes > 64) {
+                set_error("gz_gpu_set_trace_config: persistent 0..8, ldsStack 0/8/16, refillLanes 1..64");
+                return GZ_E_INVALID;
+        }
+        Scene *s = static_cast<Scene *>(handle);
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->traceConfig = *config;
+        s->traceConfig.reserved = 0;
+        return GZ_OK;
+}
+
+extern "C" int gz_gpu_get_trace_config(void *handle, GzTraceConfig *config) {
+        clear_error();
+        if (!handle || !config) {
+                set_error("gz_gpu_get_trace_config: null argument");
+                return GZ_E_INVALID;
+        }
+        Scene *s = static_cast<Scene *>(handle);
+        std::lock_guard<std::mutex> lock(s->mu);
+        *config = s->traceConfig;
         return GZ_OK;
 }

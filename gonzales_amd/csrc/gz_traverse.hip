@@ -16,6 +16,8 @@
 // MI355X layout: one lane per ray; the interior visit reads one 64-byte child-pair record
 // instead of two 32-byte nodes, a leaf reads 48-byte pre-gathered triangle records instead of
 // PrimId -> mesh -> index -> point chains (five dependent loads per vertex in the reference).
+#include <algorithm>
+
 #include "gz_internal.h"
 
 #pragma clang fp contract(off)
@@ -80,24 +82,26 @@ __device__ __forceinline__ bool tri_test(const RayState &r, float4 a, float4 b, 
         return true;
 }
 
-template <bool kAny, bool kStats>
-__global__ __launch_bounds__(kBlock) void k_trace(const NodePair *__restrict__ nodes, const TriRec *__restrict__ tris,
-                                                  int rootDesc, const Ray_C *__restrict__ rays,
-                                                  const float *__restrict__ tMaxIn, int64_t n, GzHit *__restrict__ hits,
-                                                  uint8_t *__restrict__ occ, GzTraceStats *__restrict__ stats,
-                                                  int *__restrict__ err) {
-        const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-        if (i >= n) return;
-        const Ray_C ray = rays[i];
-        const float tMax = tMaxIn[i];
+// Per-lane state of one ray in flight.
+struct Lane {
         RayState r;
+        float tMax, tBest, bu, bv;
+        int hitPrim, cur, sp;
+        uint32_t nInterior, nTris;
+};
+
+__device__ __forceinline__ void lane_start(Lane &L, const Ray_C *__restrict__ rays, const float *__restrict__ tMaxIn,
+                                           int64_t i, int rootDesc) {
+        const Ray_C ray = rays[i];
+        L.tMax = tMaxIn[i];
+        RayState &r = L.r;
         r.ox = ray.orgX;
         r.oy = ray.orgY;
         r.oz = ray.orgZ;
         r.dx = ray.dirX;
         r.dy = ray.dirY;
         r.dz = ray.dirZ;
-        r.rdx = 1.0f / r.dx;
+        r.rdx = 1.0f / r.dx;  // kernel.mojo:233-243
         r.rdy = 1.0f / r.dy;
         r.rdz = 1.0f / r.dz;
         r.ordx = r.ox * r.rdx;
@@ -106,88 +110,180 @@ __global__ __launch_bounds__(kBlock) void k_trace(const NodePair *__restrict__ n
         r.nx = r.rdx >= 0.0f;
         r.ny = r.rdy >= 0.0f;
         r.nz = r.rdz >= 0.0f;
+        L.hitPrim = -1;
+        L.tBest = L.tMax;
+        L.bu = 0.0f;
+        L.bv = 0.0f;
+        L.cur = rootDesc;
+        L.sp = 0;
+        L.nInterior = 0;
+        L.nTris = 0;
+}
 
-        int hitPrim = -1;
-        float tBest = tMax, bu = 0.0f, bv = 0.0f;
-        uint32_t nInterior = 0, nTris = 0;
-        int stack[GZ_STACK_CAPACITY];
-        int sp = 0;
-        int cur = rootDesc;
-        bool overflow = false;
-        for (;;) {
-                if (cur < 0) {
-                        const TriRec *leaf = tris + ~cur;
-                        const int count = __float_as_int(leaf[0].p0.w);
-                        bool done = false;
-                        for (int j = 0; j < count; ++j) {
-                                const float4 a = leaf[j].p0;
-                                const float4 b = leaf[j].p1;
-                                const float4 c = leaf[j].p2;
-                                if (__float_as_int(c.w) & kTriSkip) continue;
-                                if (kStats) ++nTris;
-                                float t, u, v;
-                                if (tri_test(r, a, b, c, tBest, t, u, v)) {
-                                        tBest = t;
-                                        bu = u;
-                                        bv = v;
-                                        hitPrim = __float_as_int(b.w);
-                                        if (kAny) {
-                                                done = true;
-                                                break;
-                                        }
-                                }
-                        }
-                        if (done || sp == 0) break;
-                        cur = stack[--sp];
-                } else {
-                        if (kStats) ++nInterior;
-                        const NodePair np = nodes[cur];
-                        float tl, tr;
-                        const bool lh = slab(np.a.x, np.a.y, np.a.z, np.a.w, np.b.x, np.b.y, r, tBest, tl);
-                        const bool rh = slab(np.b.z, np.b.w, np.c.x, np.c.y, np.c.z, np.c.w, r, tBest, tr);
-                        if (lh && rh) {
-                                if (sp == GZ_STACK_CAPACITY) {
-                                        overflow = true;
-                                        break;
-                                }
-                                if (tl <= tr) {
-                                        cur = np.d.x;
-                                        stack[sp++] = np.d.y;
-                                } else {
-                                        cur = np.d.y;
-                                        stack[sp++] = np.d.x;
-                                }
-                        } else if (lh) {
-                                cur = np.d.x;
-                        } else if (rh) {
-                                cur = np.d.y;
-                        } else {
-                                if (sp == 0) break;
-                                cur = stack[--sp];
+// Traversal stack of GZ_STACK_CAPACITY entries: the first kLds live in LDS (one column per
+// lane, conflict-free), the rest in per-lane private memory.
+template <int kLds>
+struct Stack {
+        int *lds;  // kLds x kBlock, column threadIdx.x
+        int priv[GZ_STACK_CAPACITY - kLds > 0 ? GZ_STACK_CAPACITY - kLds : 1];
+        __device__ __forceinline__ void put(int sp, int v) {
+                if (kLds > 0 && sp < kLds)
+                        lds[sp * kBlock + threadIdx.x] = v;
+                else
+                        priv[sp - kLds] = v;
+        }
+        __device__ __forceinline__ int get(int sp) const {
+                return (kLds > 0 && sp < kLds) ? lds[sp * kBlock + threadIdx.x] : priv[sp - kLds];
+        }
+};
+
+enum : int { kStepGo = 0, kStepDone = 1, kStepOverflow = 2 };
+
+// One step of traverse_bvh2_core's loop (kernel.mojo:256-358): a whole leaf, or one interior
+// node with its two child boxes.
+template <bool kAny, bool kStats, int kLds>
+__device__ __forceinline__ int step(Lane &L, Stack<kLds> &S, const NodePair *__restrict__ nodes,
+                                    const TriRec *__restrict__ tris) {
+        if (L.cur < 0) {
+                const TriRec *leaf = tris + ~L.cur;
+                const int count = __float_as_int(leaf[0].p0.w);
+                for (int j = 0; j < count; ++j) {
+                        const float4 a = leaf[j].p0;
+                        const float4 b = leaf[j].p1;
+                        const float4 c = leaf[j].p2;
+                        if (__float_as_int(c.w) & kTriSkip) continue;
+                        if (kStats) ++L.nTris;
+                        float t, u, v;
+                        if (tri_test(L.r, a, b, c, L.tBest, t, u, v)) {
+                                L.tBest = t;
+                                L.bu = u;
+                                L.bv = v;
+                                L.hitPrim = __float_as_int(b.w);
+                                if (kAny) return kStepDone;
                         }
                 }
+                if (L.sp == 0) return kStepDone;
+                L.cur = S.get(--L.sp);
+                return kStepGo;
         }
+        if (kStats) ++L.nInterior;
+        const NodePair np = nodes[L.cur];
+        float tl, tr;
+        const bool lh = slab(np.a.x, np.a.y, np.a.z, np.a.w, np.b.x, np.b.y, L.r, L.tBest, tl);
+        const bool rh = slab(np.b.z, np.b.w, np.c.x, np.c.y, np.c.z, np.c.w, L.r, L.tBest, tr);
+        if (lh && rh) {
+                if (L.sp == GZ_STACK_CAPACITY) return kStepOverflow;
+                const bool leftFirst = tl <= tr;
+                L.cur = leftFirst ? np.d.x : np.d.y;
+                S.put(L.sp++, leftFirst ? np.d.y : np.d.x);
+        } else if (lh) {
+                L.cur = np.d.x;
+        } else if (rh) {
+                L.cur = np.d.y;
+        } else {
+                if (L.sp == 0) return kStepDone;
+                L.cur = S.get(--L.sp);
+        }
+        return kStepGo;
+}
+
+template <bool kAny, bool kStats>
+__device__ __forceinline__ void lane_finish(const Lane &L, int64_t i, bool overflow, GzHit *__restrict__ hits,
+                                            uint8_t *__restrict__ occ, GzTraceStats *__restrict__ stats,
+                                            int *__restrict__ err) {
+        int prim = L.hitPrim;
         if (overflow) {
                 atomicOr(err, 1);
-                hitPrim = -1;
+                prim = -1;
         }
         if (kAny) {
-                occ[i] = hitPrim >= 0 ? 1 : 0;
+                occ[i] = prim >= 0 ? 1 : 0;
         } else {
                 GzHit h;
-                h.prim = hitPrim;
-                h.t = hitPrim >= 0 ? tBest : tMax;
-                h.u = hitPrim >= 0 ? bu : 0.0f;
-                h.v = hitPrim >= 0 ? bv : 0.0f;
+                h.prim = prim;
+                h.t = prim >= 0 ? L.tBest : L.tMax;  // kernel.mojo:360-364
+                h.u = prim >= 0 ? L.bu : 0.0f;
+                h.v = prim >= 0 ? L.bv : 0.0f;
                 hits[i] = h;
         }
         if (kStats) {
                 GzTraceStats st;
-                st.interior = nInterior;
-                st.tris = nTris;
+                st.interior = L.nInterior;
+                st.tris = L.nTris;
                 stats[i] = st;
         }
 }
+
+// One lane per ray, one ray per lane.
+template <bool kAny, bool kStats, int kLds>
+__global__ __launch_bounds__(kBlock) void k_trace(const NodePair *__restrict__ nodes, const TriRec *__restrict__ tris,
+                                                  int rootDesc, const Ray_C *__restrict__ rays,
+                                                  const float *__restrict__ tMaxIn, int64_t n, GzHit *__restrict__ hits,
+                                                  uint8_t *__restrict__ occ, GzTraceStats *__restrict__ stats,
+                                                  int *__restrict__ err) {
+        __shared__ int ldsStack[kLds > 0 ? kLds * kBlock : 1];
+        const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        if (i >= n) return;
+        Lane L;
+        lane_start(L, rays, tMaxIn, i, rootDesc);
+        Stack<kLds> S;
+        S.lds = ldsStack;
+        int st;
+        do {
+                st = step<kAny, kStats, kLds>(L, S, nodes, tris);
+        } while (st == kStepGo);
+        lane_finish<kAny, kStats>(L, i, st == kStepOverflow, hits, occ, stats, err);
+}
+
+// Persistent lanes with ray refill: a wave keeps its lanes busy by fetching a new ray for every
+// lane whose ray has finished (one atomic per wave per refill, ranks by prefix popcount), so the
+// long tail of deep rays no longer idles the other 63 lanes of a wave. Every wave leaves the loop
+// once the ray counter is exhausted and its lanes are idle.
+template <bool kAny, bool kStats, int kLds>
+__global__ __launch_bounds__(kBlock) void k_trace_persistent(
+        const NodePair *__restrict__ nodes, const TriRec *__restrict__ tris, int rootDesc,
+        const Ray_C *__restrict__ rays, const float *__restrict__ tMaxIn, int64_t n, GzHit *__restrict__ hits,
+        uint8_t *__restrict__ occ, GzTraceStats *__restrict__ stats, int *__restrict__ err,
+        unsigned long long *__restrict__ counter, int refill) {
+        __shared__ int ldsStack[kLds > 0 ? kLds * kBlock : 1];
+        Stack<kLds> S;
+        S.lds = ldsStack;
+        Lane L;
+        int64_t ray = -1;
+        bool exhausted = false;  // wave-uniform
+        const int laneId = (int)(threadIdx.x & 63);
+        const uint64_t below = laneId == 0 ? 0ull : (~0ull >> (64 - laneId));
+        for (;;) {
+                if (!exhausted) {
+                        const uint64_t idle = __ballot(ray < 0);
+                        const int nIdle = __popcll(idle);
+                        if (nIdle >= refill || nIdle == 64) {
+                                const int leader = __ffsll((unsigned long long)idle) - 1;
+                                unsigned long long base = 0;
+                                if (laneId == leader) base = atomicAdd(counter, (unsigned long long)nIdle);
+                                base = __shfl(base, leader);
+                                if (base + (unsigned long long)nIdle >= (unsigned long long)n) exhausted = true;
+                                if (ray < 0) {
+                                        const unsigned long long mine = base + (unsigned long long)__popcll(idle & below);
+                                        if (mine < (unsigned long long)n) {
+                                                ray = (int64_t)mine;
+                                                lane_start(L, rays, tMaxIn, ray, rootDesc);
+                                        }
+                                }
+                        }
+                }
+                if (ray >= 0) {
+                        const int st = step<kAny, kStats, kLds>(L, S, nodes, tris);
+                        if (st != kStepGo) {
+                                lane_finish<kAny, kStats>(L, ray, st == kStepOverflow, hits, occ, stats, err);
+                                ray = -1;
+                        }
+                }
+                if (exhausted && __ballot(ray >= 0) == 0ull) break;
+        }
+}
+
+__global__ void k_reset_counter(unsigned long long *counter) { *counter = 0ull; }
 
 // Empty scene (no BVH2 nodes): every ray misses (the guard of BoundingHierarchy.swift:353,392).
 __global__ __launch_bounds__(kBlock) void k_miss(const float *__restrict__ tMaxIn, int64_t n, GzHit *__restrict__ hits,
@@ -224,6 +320,39 @@ __global__ __launch_bounds__(kBlock) void k_expand(const GzHit *__restrict__ hit
 
 static inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+template <bool kAny, bool kStats, int kLds>
+static void launch_variant(Scene *s, const Ray_C *d_rays, const float *d_tMax, int64_t n, GzHit *d_hits,
+                           uint8_t *d_occ, GzTraceStats *d_stats, hipStream_t stream) {
+        const GzTraceConfig &c = s->traceConfig;
+        if (c.persistent) {
+                auto kern = k_trace_persistent<kAny, kStats, kLds>;
+                int perCU = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, kBlock, 0) != hipSuccess || perCU < 1)
+                        perCU = 1;
+                const int64_t want = (int64_t)s->cuCount * perCU * (c.persistent > 1 ? c.persistent : 1);
+                const unsigned g = (unsigned)std::min<int64_t>(want, grid_for(n));
+                unsigned long long *ctr = s->dCounters + (s->launchSeq++ % kCounterRing);
+                hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, stream, ctr);
+                hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, stream, s->dNodes, s->dTris, s->rootDesc, d_rays,
+                                   d_tMax, n, d_hits, d_occ, d_stats, s->dErr, ctr,
+                                   c.refillLanes > 0 && c.refillLanes <= 64 ? c.refillLanes : 16);
+        } else {
+                hipLaunchKernelGGL((k_trace<kAny, kStats, kLds>), dim3(grid_for(n)), dim3(kBlock), 0, stream, s->dNodes,
+                                   s->dTris, s->rootDesc, d_rays, d_tMax, n, d_hits, d_occ, d_stats, s->dErr);
+        }
+}
+
+template <bool kAny, bool kStats>
+static void launch_lds(Scene *s, const Ray_C *d_rays, const float *d_tMax, int64_t n, GzHit *d_hits, uint8_t *d_occ,
+                       GzTraceStats *d_stats, hipStream_t stream) {
+        if (s->traceConfig.ldsStack >= 16)
+                launch_variant<kAny, kStats, 16>(s, d_rays, d_tMax, n, d_hits, d_occ, d_stats, stream);
+        else if (s->traceConfig.ldsStack >= 8)
+                launch_variant<kAny, kStats, 8>(s, d_rays, d_tMax, n, d_hits, d_occ, d_stats, stream);
+        else
+                launch_variant<kAny, kStats, 0>(s, d_rays, d_tMax, n, d_hits, d_occ, d_stats, stream);
+}
+
 int launch_trace(Scene *s, const Ray_C *d_rays, const float *d_tMax, int64_t n, GzHit *d_hits, uint8_t *d_occ,
                  GzTraceStats *d_stats, bool anyHit, hipStream_t stream) {
         if (n <= 0) return GZ_OK;
@@ -231,24 +360,19 @@ int launch_trace(Scene *s, const Ray_C *d_rays, const float *d_tMax, int64_t n, 
                 set_error("traversal batch of %lld rays is too large", (long long)n);
                 return GZ_E_INVALID;
         }
-        const unsigned g = grid_for(n);
         if (s->empty) {
-                hipLaunchKernelGGL(k_miss, dim3(g), dim3(kBlock), 0, stream, d_tMax, n, anyHit ? nullptr : d_hits,
-                                   anyHit ? d_occ : nullptr, d_stats);
+                hipLaunchKernelGGL(k_miss, dim3(grid_for(n)), dim3(kBlock), 0, stream, d_tMax, n,
+                                   anyHit ? nullptr : d_hits, anyHit ? d_occ : nullptr, d_stats);
         } else if (anyHit) {
                 if (d_stats)
-                        hipLaunchKernelGGL((k_trace<true, true>), dim3(g), dim3(kBlock), 0, stream, s->dNodes, s->dTris,
-                                           s->rootDesc, d_rays, d_tMax, n, nullptr, d_occ, d_stats, s->dErr);
+                        launch_lds<true, true>(s, d_rays, d_tMax, n, nullptr, d_occ, d_stats, stream);
                 else
-                        hipLaunchKernelGGL((k_trace<true, false>), dim3(g), dim3(kBlock), 0, stream, s->dNodes, s->dTris,
-                                           s->rootDesc, d_rays, d_tMax, n, nullptr, d_occ, nullptr, s->dErr);
+                        launch_lds<true, false>(s, d_rays, d_tMax, n, nullptr, d_occ, nullptr, stream);
         } else {
                 if (d_stats)
-                        hipLaunchKernelGGL((k_trace<false, true>), dim3(g), dim3(kBlock), 0, stream, s->dNodes, s->dTris,
-                                           s->rootDesc, d_rays, d_tMax, n, d_hits, nullptr, d_stats, s->dErr);
+                        launch_lds<false, true>(s, d_rays, d_tMax, n, d_hits, nullptr, d_stats, stream);
                 else
-                        hipLaunchKernelGGL((k_trace<false, false>), dim3(g), dim3(kBlock), 0, stream, s->dNodes,
-                                           s->dTris, s->rootDesc, d_rays, d_tMax, n, d_hits, nullptr, nullptr, s->dErr);
+                        launch_lds<false, false>(s, d_rays, d_tMax, n, d_hits, nullptr, nullptr, stream);
         }
         return hip_ok(hipGetLastError(), "traversal launch") ? GZ_OK : GZ_E_HIP;
 }

@@ -227,6 +227,22 @@ int gz_gpu_traverse_any_stats_device(void *handle, const struct Ray_C *d_rays, c
 int gz_gpu_expand_hits_device(void *handle, const struct GzHit *d_hits, const float *d_tMax,
                               int64_t count, struct Intersection_C *d_results, void *stream);
 
+/* Launch configuration of the traversal kernels (no reference counterpart: the reference launches
+ * one thread per ray in blocks of 256, kernel.mojo:649-662). Results do not depend on it.
+ *   persistent  : 0 = one lane per ray; k >= 1 = persistent lanes with ray refill, k x the
+ *                 resident blocks per CU;
+ *   ldsStack    : stack entries kept in LDS per lane (0, 8 or 16; the rest in private memory);
+ *   refillLanes : idle lanes (1..64) that trigger a refill of a wave's lanes. */
+struct GzTraceConfig {
+        int32_t persistent;
+        int32_t ldsStack;
+        int32_t refillLanes;
+        int32_t reserved;
+};
+
+int gz_gpu_set_trace_config(void *handle, const struct GzTraceConfig *config);
+int gz_gpu_get_trace_config(void *handle, struct GzTraceConfig *config);
+
 /* Waits for `stream` and returns GZ_E_OVERFLOW if any traversal since the last call stopped on
  * a full stack (the flag is then cleared), else GZ_OK. */
 int gz_gpu_check_traversal(void *handle, void *stream);
