@@ -375,16 +375,16 @@ extern "C" int gz_gpu_set_trace_config(void *handle, const GzTraceConfig *config
                 set_error("gz_gpu_set_trace_config: null argument");
                 return GZ_E_INVALID;
         }
-        if (config->persistent < 0 || config->persistent > 8 ||
+        if (config->persistent < 0 || config->persistent > 2 ||
             (config->ldsStack != 0 && config->ldsStack != 8 && config->ldsStack != 16) || config->refillLanes < 1 ||
-            config->refillLanes > 64) {
-                set_error("gz_gpu_set_trace_config: persistent 0..8, ldsStack 0/8/16, refillLanes 1..64");
+            config->refillLanes > 64 || config->gridScale < 0 || config->gridScale > 16) {
+                set_error("gz_gpu_set_trace_config: persistent 0..2, ldsStack 0/8/16, refillLanes 1..64, "
+                          "gridScale 0..16");
                 return GZ_E_INVALID;
         }
         Scene *s = static_cast<Scene *>(handle);
         std::lock_guard<std::mutex> lock(s->mu);
         s->traceConfig = *config;
-        s->traceConfig.reserved = 0;
         return GZ_OK;
 }
 

@@ -283,6 +283,34 @@ __global__ __launch_bounds__(kBlock) void k_trace_persistent(
         }
 }
 
+// Strided persistent lanes: global lane g traces rays g, g + G, g + 2G, ... (G = lanes in the
+// grid) back to back, starting the next ray the step after the previous one ends. Summing many
+// rays per lane evens out the per-ray work, with no atomics; consecutive lanes still read
+// consecutive rays. Each lane leaves after its last ray, so every wave exits.
+template <bool kAny, bool kStats, int kLds>
+__global__ __launch_bounds__(kBlock) void k_trace_strided(
+        const NodePair *__restrict__ nodes, const TriRec *__restrict__ tris, int rootDesc,
+        const Ray_C *__restrict__ rays, const float *__restrict__ tMaxIn, int64_t n, GzHit *__restrict__ hits,
+        uint8_t *__restrict__ occ, GzTraceStats *__restrict__ stats, int *__restrict__ err) {
+        __shared__ int ldsStack[kLds > 0 ? kLds * kBlock : 1];
+        Stack<kLds> S;
+        S.lds = ldsStack;
+        const int64_t G = (int64_t)gridDim.x * kBlock;
+        int64_t ray = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        if (ray >= n) return;
+        Lane L;
+        lane_start(L, rays, tMaxIn, ray, rootDesc);
+        for (;;) {
+                const int st = step<kAny, kStats, kLds>(L, S, nodes, tris);
+                if (st != kStepGo) {
+                        lane_finish<kAny, kStats>(L, ray, st == kStepOverflow, hits, occ, stats, err);
+                        ray += G;
+                        if (ray >= n) break;
+                        lane_start(L, rays, tMaxIn, ray, rootDesc);
+                }
+        }
+}
+
 __global__ void k_reset_counter(unsigned long long *counter) { *counter = 0ull; }
 
 // Empty scene (no BVH2 nodes): every ray misses (the guard of BoundingHierarchy.swift:353,392).
@@ -324,12 +352,21 @@ template <bool kAny, bool kStats, int kLds>
 static void launch_variant(Scene *s, const Ray_C *d_rays, const float *d_tMax, int64_t n, GzHit *d_hits,
                            uint8_t *d_occ, GzTraceStats *d_stats, hipStream_t stream) {
         const GzTraceConfig &c = s->traceConfig;
-        if (c.persistent) {
+        if (c.persistent == 2) {
+                auto kern = k_trace_strided<kAny, kStats, kLds>;
+                int perCU = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, kBlock, 0) != hipSuccess || perCU < 1)
+                        perCU = 1;
+                const int64_t want = (int64_t)s->cuCount * perCU * (c.gridScale > 0 ? c.gridScale : 1);
+                const unsigned g = (unsigned)std::min<int64_t>(want, grid_for(n));
+                hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, stream, s->dNodes, s->dTris, s->rootDesc, d_rays,
+                                   d_tMax, n, d_hits, d_occ, d_stats, s->dErr);
+        } else if (c.persistent == 1) {
                 auto kern = k_trace_persistent<kAny, kStats, kLds>;
                 int perCU = 0;
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, kBlock, 0) != hipSuccess || perCU < 1)
                         perCU = 1;
-                const int64_t want = (int64_t)s->cuCount * perCU * (c.persistent > 1 ? c.persistent : 1);
+                const int64_t want = (int64_t)s->cuCount * perCU * (c.gridScale > 0 ? c.gridScale : 1);
                 const unsigned g = (unsigned)std::min<int64_t>(want, grid_for(n));
                 unsigned long long *ctr = s->dCounters + (s->launchSeq++ % kCounterRing);
                 hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, stream, ctr);

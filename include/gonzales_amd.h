@@ -229,15 +229,16 @@ int gz_gpu_expand_hits_device(void *handle, const struct GzHit *d_hits, const fl
 
 /* Launch configuration of the traversal kernels (no reference counterpart: the reference launches
  * one thread per ray in blocks of 256, kernel.mojo:649-662). Results do not depend on it.
- *   persistent  : 0 = one lane per ray; k >= 1 = persistent lanes with ray refill, k x the
- *                 resident blocks per CU;
+ *   persistent  : 0 = one lane per ray; 1 = persistent lanes refilled from an atomic ray
+ *                 counter; 2 = persistent lanes that trace a fixed stride of rays;
  *   ldsStack    : stack entries kept in LDS per lane (0, 8 or 16; the rest in private memory);
- *   refillLanes : idle lanes (1..64) that trigger a refill of a wave's lanes. */
+ *   refillLanes : idle lanes (1..64) that trigger a refill of a wave's lanes (mode 1);
+ *   gridScale   : persistent grid = gridScale x the blocks resident per CU x CUs (0 = 1). */
 struct GzTraceConfig {
         int32_t persistent;
         int32_t ldsStack;
         int32_t refillLanes;
-        int32_t reserved;
+        int32_t gridScale;
 };
 
 int gz_gpu_set_trace_config(void *handle, const struct GzTraceConfig *config);
