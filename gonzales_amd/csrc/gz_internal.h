@@ -76,7 +76,7 @@ struct Scene {
 
         // Traversal launch configuration and the persistent kernel's ray counters (a ring, so
         // launches queued on different streams never share a counter).
-        GzTraceConfig traceConfig{1, 16, 48, 1};
+        GzTraceConfig traceConfig{1, 16, 56, 1};
         int cuCount = 256;
         unsigned long long *dCounters = nullptr;
         uint64_t launchSeq = 0;
