@@ -63,7 +63,7 @@ class GzSceneInfo(ctypes.Structure):
 
 class GzTraceConfig(ctypes.Structure):
     _fields_ = [("persistent", ctypes.c_int32), ("ldsStack", ctypes.c_int32), ("refillLanes", ctypes.c_int32),
-                ("gridScale", ctypes.c_int32)]
+                ("gridScale", ctypes.c_int32), ("chunk", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
 class GzZSobolParams(ctypes.Structure):

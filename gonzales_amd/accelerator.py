@@ -275,9 +275,9 @@ class BoundingHierarchy:
                                                       abi.ptr(occ_t), abi.ptr(stats_t), abi.stream_ptr(stream))
         abi.check(rc, "any_device")
 
-    def set_trace_config(self, persistent: int = 1, lds_stack: int = 16, refill_lanes: int = 56,
-                         grid_scale: int = 1) -> None:
-        cfg = abi.GzTraceConfig(persistent, lds_stack, refill_lanes, grid_scale)
+    def set_trace_config(self, persistent: int = 1, lds_stack: int = 16, refill_lanes: int = 16,
+                         grid_scale: int = 1, chunk: int = 128) -> None:
+        cfg = abi.GzTraceConfig(persistent, lds_stack, refill_lanes, grid_scale, chunk)
         abi.check(abi.load().gz_gpu_set_trace_config(self.handle, ctypes.byref(cfg)), "gz_gpu_set_trace_config")
 
     def trace_config(self) -> abi.GzTraceConfig:

@@ -203,6 +203,18 @@ extern "C" void *gz_gpu_upload_scene(const BVH2Node *nodes, int64_t nNodes, cons
                                 }
                         }
                         s->interiorCount = (int64_t)order.size();
+                        // Every box the kernel tests is a child of a reachable interior node.
+                        bool safe = true;
+                        for (size_t k = 0; k < order.size() && safe; ++k) {
+                                const BVH2Node *ch[2] = {&nodes[order[k] + 1], &nodes[nodes[order[k]].offset]};
+                                for (const BVH2Node *c : ch) {
+                                        const float v[6] = {c->boundsMinX, c->boundsMinY, c->boundsMinZ,
+                                                            c->boundsMaxX, c->boundsMaxY, c->boundsMaxZ};
+                                        for (float x : v)
+                                                if (!(x >= -0x1p40f && x <= 0x1p40f)) safe = false;
+                                }
+                        }
+                        s->boundsSafe = safe;
                         s->triCount = triNext;
                         s->rootDesc = recOf[0];
                         pairs.resize(order.size());
@@ -377,9 +389,10 @@ extern "C" int gz_gpu_set_trace_config(void *handle, const GzTraceConfig *config
         }
         if (config->persistent < 0 || config->persistent > 2 ||
             (config->ldsStack != 0 && config->ldsStack != 8 && config->ldsStack != 16) || config->refillLanes < 1 ||
-            config->refillLanes > 64 || config->gridScale < 0 || config->gridScale > 16) {
+            config->refillLanes > 64 || config->gridScale < 0 || config->gridScale > 16 || config->chunk < 1 ||
+            config->chunk > 4096) {
                 set_error("gz_gpu_set_trace_config: persistent 0..2, ldsStack 0/8/16, refillLanes 1..64, "
-                          "gridScale 0..16");
+                          "gridScale 0..16, chunk 1..4096");
                 return GZ_E_INVALID;
         }
         Scene *s = static_cast<Scene *>(handle);

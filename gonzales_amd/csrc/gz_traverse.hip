@@ -30,11 +30,14 @@ struct RayState {
         float ox, oy, oz, dx, dy, dz;
         float rdx, rdy, rdz, ordx, ordy, ordz;
         bool nx, ny, nz;
+        bool safe;  // no slab value can be NaN: the min/max form equals the select form
 };
 
-// kernel.mojo:171-206
-__device__ __forceinline__ bool slab(float mnx, float mny, float mnz, float mxx, float mxy, float mxz, const RayState &r,
-                                     float tMax, float &tNearOut) {
+// kernel.mojo:171-206, written as the reference's select chains: `a > b ? a : b` keeps the
+// second operand when either is NaN, which decides the result for rays with a zero direction
+// component (0 * inf) and must be reproduced exactly.
+__device__ __forceinline__ bool slab_exact(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
+                                           const RayState &r, float tMax, float &tNearOut) {
         const float nearX = r.nx ? mnx : mxx, farX = r.nx ? mxx : mnx;
         const float nearY = r.ny ? mny : mxy, farY = r.ny ? mxy : mny;
         const float nearZ = r.nz ? mnz : mxz, farZ = r.nz ? mxz : mnz;
@@ -53,6 +56,32 @@ __device__ __forceinline__ bool slab(float mnx, float mny, float mnz, float mxx,
         tFar = tFar * 1.0000003f;
         tNearOut = tNear;
         return tNear <= tFar;
+}
+
+// Same test for rays whose slab values are all finite (RayState::safe): with no NaN operand the
+// select chains are max/min (up to the sign of a zero, which no later comparison can see), so
+// they map to the three-operand v_max3/v_min3 instructions. A NaN tMax is ignored by both forms.
+__device__ __forceinline__ bool slab_fast(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
+                                          const RayState &r, float tMax, float &tNearOut) {
+        const float nearX = r.nx ? mnx : mxx, farX = r.nx ? mxx : mnx;
+        const float nearY = r.ny ? mny : mxy, farY = r.ny ? mxy : mny;
+        const float nearZ = r.nz ? mnz : mxz, farZ = r.nz ? mxz : mnz;
+        const float tNearX = nearX * r.rdx - r.ordx;
+        const float tNearY = nearY * r.rdy - r.ordy;
+        const float tNearZ = nearZ * r.rdz - r.ordz;
+        const float tFarX = farX * r.rdx - r.ordx;
+        const float tFarY = farY * r.rdy - r.ordy;
+        const float tFarZ = farZ * r.rdz - r.ordz;
+        const float tNear = fmaxf(fmaxf(fmaxf(tNearX, tNearY), tNearZ), 0.0f);
+        const float tFar = fminf(fminf(fminf(tFarX, tFarY), tFarZ), tMax) * 1.0000003f;
+        tNearOut = tNear;
+        return tNear <= tFar;
+}
+
+__device__ __forceinline__ bool slab(float mnx, float mny, float mnz, float mxx, float mxy, float mxz, const RayState &r,
+                                     float tMax, float &tNearOut) {
+        if (r.safe) return slab_fast(mnx, mny, mnz, mxx, mxy, mxz, r, tMax, tNearOut);
+        return slab_exact(mnx, mny, mnz, mxx, mxy, mxz, r, tMax, tNearOut);
 }
 
 // kernel.mojo:122-156 (cross :84-90 as a.yzx*b.zxy - a.zxy*b.yzx, dot :92-95 left to right).
@@ -91,7 +120,7 @@ struct Lane {
 };
 
 __device__ __forceinline__ void lane_start(Lane &L, const Ray_C *__restrict__ rays, const float *__restrict__ tMaxIn,
-                                           int64_t i, int rootDesc) {
+                                           int64_t i, int rootDesc, bool sceneSafe) {
         const Ray_C ray = rays[i];
         L.tMax = tMaxIn[i];
         RayState &r = L.r;
@@ -110,6 +139,12 @@ __device__ __forceinline__ void lane_start(Lane &L, const Ray_C *__restrict__ ra
         r.nx = r.rdx >= 0.0f;
         r.ny = r.rdy >= 0.0f;
         r.nz = r.rdz >= 0.0f;
+        // 2^-40 <= |d| <= 2^40 and |o| <= 2^40 on every axis, with box coordinates <= 2^40
+        // (checked at upload), bound every slab product and difference by 2^81: no inf, no NaN.
+        const float lo = 0x1p-40f, hi = 0x1p40f;
+        const float ax = fabsf(r.dx), ay = fabsf(r.dy), az = fabsf(r.dz);
+        r.safe = sceneSafe && ax >= lo && ay >= lo && az >= lo && ax <= hi && ay <= hi && az <= hi &&
+                 fabsf(r.ox) <= hi && fabsf(r.oy) <= hi && fabsf(r.oz) <= hi;
         L.hitPrim = -1;
         L.tBest = L.tMax;
         L.bu = 0.0f;
@@ -120,20 +155,28 @@ __device__ __forceinline__ void lane_start(Lane &L, const Ray_C *__restrict__ ra
         L.nTris = 0;
 }
 
-// Traversal stack of GZ_STACK_CAPACITY entries: the first kLds live in LDS (one column per
-// lane, conflict-free), the rest in per-lane private memory.
+// The first entries of every lane's stack, [entry][lane of the block]: a lane's pushes and pops
+// hit its own column, so a wave's 64 lanes touch 64 consecutive words (no bank conflicts).
+constexpr int kLdsMax = 16;
+__shared__ int g_stackLds[kLdsMax * kBlock];
+
+// Traversal stack of GZ_STACK_CAPACITY entries: the first kLds live in LDS, the rest in
+// per-lane private memory.
 template <int kLds>
 struct Stack {
-        int *lds;  // kLds x kBlock, column threadIdx.x
         int priv[GZ_STACK_CAPACITY - kLds > 0 ? GZ_STACK_CAPACITY - kLds : 1];
         __device__ __forceinline__ void put(int sp, int v) {
                 if (kLds > 0 && sp < kLds)
-                        lds[sp * kBlock + threadIdx.x] = v;
+                        g_stackLds[sp * kBlock + threadIdx.x] = v;
                 else
                         priv[sp - kLds] = v;
         }
         __device__ __forceinline__ int get(int sp) const {
-                return (kLds > 0 && sp < kLds) ? lds[sp * kBlock + threadIdx.x] : priv[sp - kLds];
+                // The volatile LDS read keeps the two loads apart: merged, they become one generic
+                // (flat) load that pays the latency of both memories on every pop.
+                typedef __attribute__((address_space(3))) volatile int LdsInt;
+                if (kLds > 0 && sp < kLds) return *(LdsInt *)&g_stackLds[sp * kBlock + threadIdx.x];
+                return priv[sp - kLds];
         }
 };
 
@@ -217,17 +260,15 @@ __device__ __forceinline__ void lane_finish(const Lane &L, int64_t i, bool overf
 // One lane per ray, one ray per lane.
 template <bool kAny, bool kStats, int kLds>
 __global__ __launch_bounds__(kBlock) void k_trace(const NodePair *__restrict__ nodes, const TriRec *__restrict__ tris,
-                                                  int rootDesc, const Ray_C *__restrict__ rays,
+                                                  int rootDesc, bool sceneSafe, const Ray_C *__restrict__ rays,
                                                   const float *__restrict__ tMaxIn, int64_t n, GzHit *__restrict__ hits,
                                                   uint8_t *__restrict__ occ, GzTraceStats *__restrict__ stats,
                                                   int *__restrict__ err) {
-        __shared__ int ldsStack[kLds > 0 ? kLds * kBlock : 1];
         const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) return;
         Lane L;
-        lane_start(L, rays, tMaxIn, i, rootDesc);
+        lane_start(L, rays, tMaxIn, i, rootDesc, sceneSafe);
         Stack<kLds> S;
-        S.lds = ldsStack;
         int st;
         do {
                 st = step<kAny, kStats, kLds>(L, S, nodes, tris);
@@ -235,40 +276,63 @@ __global__ __launch_bounds__(kBlock) void k_trace(const NodePair *__restrict__ n
         lane_finish<kAny, kStats>(L, i, st == kStepOverflow, hits, occ, stats, err);
 }
 
-// Persistent lanes with ray refill: a wave keeps its lanes busy by fetching a new ray for every
-// lane whose ray has finished (one atomic per wave per refill, ranks by prefix popcount), so the
-// long tail of deep rays no longer idles the other 63 lanes of a wave. Every wave leaves the loop
-// once the ray counter is exhausted and its lanes are idle.
+// Persistent lanes with ray refill. A wave keeps its lanes busy: once `refill` lanes are idle,
+// each idle lane takes the next ray index from the wave's reserve, a range of `chunk`
+// consecutive rays grabbed from the global counter with one atomic (so the counter sees one
+// atomic per `chunk` rays, not one per refill). The long tail of deep rays no longer idles the
+// other lanes of a wave. A wave leaves once the counter is exhausted and its lanes are idle.
 template <bool kAny, bool kStats, int kLds>
 __global__ __launch_bounds__(kBlock) void k_trace_persistent(
-        const NodePair *__restrict__ nodes, const TriRec *__restrict__ tris, int rootDesc,
+        const NodePair *__restrict__ nodes, const TriRec *__restrict__ tris, int rootDesc, bool sceneSafe,
         const Ray_C *__restrict__ rays, const float *__restrict__ tMaxIn, int64_t n, GzHit *__restrict__ hits,
         uint8_t *__restrict__ occ, GzTraceStats *__restrict__ stats, int *__restrict__ err,
-        unsigned long long *__restrict__ counter, int refill) {
-        __shared__ int ldsStack[kLds > 0 ? kLds * kBlock : 1];
+        unsigned long long *__restrict__ counter, int refill, int chunk) {
         Stack<kLds> S;
-        S.lds = ldsStack;
         Lane L;
         int64_t ray = -1;
         bool exhausted = false;  // wave-uniform
+        unsigned long long resBase = 0, resCount = 0;  // wave-uniform reserve [resBase, resBase + resCount)
         const int laneId = (int)(threadIdx.x & 63);
         const uint64_t below = laneId == 0 ? 0ull : (~0ull >> (64 - laneId));
+        const unsigned long long un = (unsigned long long)n;
         for (;;) {
                 if (!exhausted) {
                         const uint64_t idle = __ballot(ray < 0);
                         const int nIdle = __popcll(idle);
                         if (nIdle >= refill || nIdle == 64) {
-                                const int leader = __ffsll((unsigned long long)idle) - 1;
-                                unsigned long long base = 0;
-                                if (laneId == leader) base = atomicAdd(counter, (unsigned long long)nIdle);
-                                base = __shfl(base, leader);
-                                if (base + (unsigned long long)nIdle >= (unsigned long long)n) exhausted = true;
-                                if (ray < 0) {
-                                        const unsigned long long mine = base + (unsigned long long)__popcll(idle & below);
-                                        if (mine < (unsigned long long)n) {
-                                                ray = (int64_t)mine;
-                                                lane_start(L, rays, tMaxIn, ray, rootDesc);
+                                const unsigned long long rank = (unsigned long long)__popcll(idle & below);
+                                unsigned long long fresh = 0;
+                                const unsigned long long need = (unsigned long long)nIdle;
+                                if (resCount < need) {
+                                        // Top up with a new chunk (at least enough for every idle lane).
+                                        const unsigned long long take =
+                                                need - resCount > (unsigned long long)chunk ? need - resCount
+                                                                                             : (unsigned long long)chunk;
+                                        const int leader = __ffsll((unsigned long long)idle) - 1;
+                                        if (laneId == leader) fresh = atomicAdd(counter, take);
+                                        fresh = __shfl(fresh, leader);
+                                        if (ray < 0) {
+                                                const unsigned long long mine =
+                                                        rank < resCount ? resBase + rank : fresh + (rank - resCount);
+                                                if (mine < un) {
+                                                        ray = (int64_t)mine;
+                                                        lane_start(L, rays, tMaxIn, ray, rootDesc, sceneSafe);
+                                                }
                                         }
+                                        const unsigned long long used = need - resCount;
+                                        resBase = fresh + used;
+                                        resCount = take - used;
+                                        if (fresh + take >= un) exhausted = true;
+                                } else {
+                                        if (ray < 0) {
+                                                const unsigned long long mine = resBase + rank;
+                                                if (mine < un) {
+                                                        ray = (int64_t)mine;
+                                                        lane_start(L, rays, tMaxIn, ray, rootDesc, sceneSafe);
+                                                }
+                                        }
+                                        resBase += need;
+                                        resCount -= need;
                                 }
                         }
                 }
@@ -289,24 +353,22 @@ __global__ __launch_bounds__(kBlock) void k_trace_persistent(
 // consecutive rays. Each lane leaves after its last ray, so every wave exits.
 template <bool kAny, bool kStats, int kLds>
 __global__ __launch_bounds__(kBlock) void k_trace_strided(
-        const NodePair *__restrict__ nodes, const TriRec *__restrict__ tris, int rootDesc,
+        const NodePair *__restrict__ nodes, const TriRec *__restrict__ tris, int rootDesc, bool sceneSafe,
         const Ray_C *__restrict__ rays, const float *__restrict__ tMaxIn, int64_t n, GzHit *__restrict__ hits,
         uint8_t *__restrict__ occ, GzTraceStats *__restrict__ stats, int *__restrict__ err) {
-        __shared__ int ldsStack[kLds > 0 ? kLds * kBlock : 1];
         Stack<kLds> S;
-        S.lds = ldsStack;
         const int64_t G = (int64_t)gridDim.x * kBlock;
         int64_t ray = (int64_t)blockIdx.x * kBlock + threadIdx.x;
         if (ray >= n) return;
         Lane L;
-        lane_start(L, rays, tMaxIn, ray, rootDesc);
+        lane_start(L, rays, tMaxIn, ray, rootDesc, sceneSafe);
         for (;;) {
                 const int st = step<kAny, kStats, kLds>(L, S, nodes, tris);
                 if (st != kStepGo) {
                         lane_finish<kAny, kStats>(L, ray, st == kStepOverflow, hits, occ, stats, err);
                         ray += G;
                         if (ray >= n) break;
-                        lane_start(L, rays, tMaxIn, ray, rootDesc);
+                        lane_start(L, rays, tMaxIn, ray, rootDesc, sceneSafe);
                 }
         }
 }
@@ -359,8 +421,8 @@ static void launch_variant(Scene *s, const Ray_C *d_rays, const float *d_tMax, i
                         perCU = 1;
                 const int64_t want = (int64_t)s->cuCount * perCU * (c.gridScale > 0 ? c.gridScale : 1);
                 const unsigned g = (unsigned)std::min<int64_t>(want, grid_for(n));
-                hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, stream, s->dNodes, s->dTris, s->rootDesc, d_rays,
-                                   d_tMax, n, d_hits, d_occ, d_stats, s->dErr);
+                hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, stream, s->dNodes, s->dTris, s->rootDesc,
+                                   s->boundsSafe, d_rays, d_tMax, n, d_hits, d_occ, d_stats, s->dErr);
         } else if (c.persistent == 1) {
                 auto kern = k_trace_persistent<kAny, kStats, kLds>;
                 int perCU = 0;
@@ -370,12 +432,14 @@ static void launch_variant(Scene *s, const Ray_C *d_rays, const float *d_tMax, i
                 const unsigned g = (unsigned)std::min<int64_t>(want, grid_for(n));
                 unsigned long long *ctr = s->dCounters + (s->launchSeq++ % kCounterRing);
                 hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, stream, ctr);
-                hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, stream, s->dNodes, s->dTris, s->rootDesc, d_rays,
-                                   d_tMax, n, d_hits, d_occ, d_stats, s->dErr, ctr,
-                                   c.refillLanes > 0 && c.refillLanes <= 64 ? c.refillLanes : 16);
+                hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, stream, s->dNodes, s->dTris, s->rootDesc,
+                                   s->boundsSafe, d_rays, d_tMax, n, d_hits, d_occ, d_stats, s->dErr, ctr,
+                                   c.refillLanes > 0 && c.refillLanes <= 64 ? c.refillLanes : 16,
+                                   c.chunk >= 1 && c.chunk <= 4096 ? c.chunk : 64);
         } else {
                 hipLaunchKernelGGL((k_trace<kAny, kStats, kLds>), dim3(grid_for(n)), dim3(kBlock), 0, stream, s->dNodes,
-                                   s->dTris, s->rootDesc, d_rays, d_tMax, n, d_hits, d_occ, d_stats, s->dErr);
+                                   s->dTris, s->rootDesc, s->boundsSafe, d_rays, d_tMax, n, d_hits, d_occ, d_stats,
+                                   s->dErr);
         }
 }
 

@@ -233,12 +233,15 @@ int gz_gpu_expand_hits_device(void *handle, const struct GzHit *d_hits, const fl
  *                 counter; 2 = persistent lanes that trace a fixed stride of rays;
  *   ldsStack    : stack entries kept in LDS per lane (0, 8 or 16; the rest in private memory);
  *   refillLanes : idle lanes (1..64) that trigger a refill of a wave's lanes (mode 1);
- *   gridScale   : persistent grid = gridScale x the blocks resident per CU x CUs (0 = 1). */
+ *   gridScale   : persistent grid = gridScale x the blocks resident per CU x CUs (0 = 1);
+ *   chunk       : rays a wave takes from the global counter per atomic (mode 1, 1..4096). */
 struct GzTraceConfig {
         int32_t persistent;
         int32_t ldsStack;
         int32_t refillLanes;
         int32_t gridScale;
+        int32_t chunk;
+        int32_t reserved[3];
 };
 
 int gz_gpu_set_trace_config(void *handle, const struct GzTraceConfig *config);
