@@ -290,8 +290,10 @@ __global__ __launch_bounds__(kBlock) void k_trace_persistent(
         Stack<kLds> S;
         Lane L;
         int64_t ray = -1;
-        bool exhausted = false;  // wave-uniform
-        unsigned long long resBase = 0, resCount = 0;  // wave-uniform reserve [resBase, resBase + resCount)
+        // Wave-uniform state: the reserve [resBase, resBase + resCount), whether the global counter
+        // has run past n (no further atomics), and whether no ray is left to hand out at all.
+        unsigned long long resBase = 0, resCount = 0;
+        bool counterDone = false, exhausted = false;
         const int laneId = (int)(threadIdx.x & 63);
         const uint64_t below = laneId == 0 ? 0ull : (~0ull >> (64 - laneId));
         const unsigned long long un = (unsigned long long)n;
@@ -301,39 +303,40 @@ __global__ __launch_bounds__(kBlock) void k_trace_persistent(
                         const int nIdle = __popcll(idle);
                         if (nIdle >= refill || nIdle == 64) {
                                 const unsigned long long rank = (unsigned long long)__popcll(idle & below);
-                                unsigned long long fresh = 0;
                                 const unsigned long long need = (unsigned long long)nIdle;
-                                if (resCount < need) {
-                                        // Top up with a new chunk (at least enough for every idle lane).
-                                        const unsigned long long take =
-                                                need - resCount > (unsigned long long)chunk ? need - resCount
-                                                                                             : (unsigned long long)chunk;
+                                unsigned long long fresh = 0, take = 0;
+                                if (resCount < need && !counterDone) {
+                                        // Top up with a new chunk, at least enough for every idle lane.
+                                        take = need - resCount > (unsigned long long)chunk ? need - resCount
+                                                                                        : (unsigned long long)chunk;
                                         const int leader = __ffsll((unsigned long long)idle) - 1;
                                         if (laneId == leader) fresh = atomicAdd(counter, take);
                                         fresh = __shfl(fresh, leader);
-                                        if (ray < 0) {
-                                                const unsigned long long mine =
-                                                        rank < resCount ? resBase + rank : fresh + (rank - resCount);
-                                                if (mine < un) {
-                                                        ray = (int64_t)mine;
-                                                        lane_start(L, rays, tMaxIn, ray, rootDesc, sceneSafe);
-                                                }
+                                        if (fresh + take >= un) counterDone = true;
+                                }
+                                if (ray < 0) {
+                                        // The first resCount idle lanes drain the reserve, the others
+                                        // start on the fresh chunk.
+                                        unsigned long long mine = ~0ull;
+                                        if (rank < resCount)
+                                                mine = resBase + rank;
+                                        else if (rank - resCount < take)
+                                                mine = fresh + (rank - resCount);
+                                        if (mine < un) {
+                                                ray = (int64_t)mine;
+                                                lane_start(L, rays, tMaxIn, ray, rootDesc, sceneSafe);
                                         }
-                                        const unsigned long long used = need - resCount;
+                                }
+                                if (take > 0) {
+                                        const unsigned long long used = need - resCount;  // <= take
                                         resBase = fresh + used;
                                         resCount = take - used;
-                                        if (fresh + take >= un) exhausted = true;
                                 } else {
-                                        if (ray < 0) {
-                                                const unsigned long long mine = resBase + rank;
-                                                if (mine < un) {
-                                                        ray = (int64_t)mine;
-                                                        lane_start(L, rays, tMaxIn, ray, rootDesc, sceneSafe);
-                                                }
-                                        }
-                                        resBase += need;
-                                        resCount -= need;
+                                        const unsigned long long used = need < resCount ? need : resCount;
+                                        resBase += used;
+                                        resCount -= used;
                                 }
+                                if (counterDone && (resCount == 0 || resBase >= un)) exhausted = true;
                         }
                 }
                 if (ray >= 0) {
