@@ -35,6 +35,22 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_k_trace_closest_incoherent_1M.md")
+
+
+def measured_traffic() -> float | None:
+    """HBM bytes per closest-hit launch on this workload, from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy); None if it is absent."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +115,12 @@ def main() -> int:
     d_stats = torch.empty(n * 8, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
 
+    # Launch configuration picked by the autotuner on a 2M-ray sample of this batch (untimed).
+    from gonzales_amd.autotune import autotune, describe
+
+    best, timings = autotune(bh, d_rays, d_tmax, any_hit=False, sample=1 << 21)
+    log(f"autotuned launch config: {best}")
+
     # Work counts for the algorithmic bytes (untimed counting pass).
     bh.closest_device(d_rays, d_tmax, d_hits, d_stats)
     bh.check_traversal()
@@ -131,7 +153,9 @@ def main() -> int:
     extra = {"host_build_s": round(t_build, 3), "upload_s": round(t_upload, 3),
              "bytes_per_ray": round(algo_bytes / n, 1), "interior_per_ray": round(sum_interior / n, 2),
              "tris_per_ray": round(sum_tris / n, 3), "hit_rate": round(hit_rate, 4),
-             "wall_mrays_per_s": round(n * args.steps / wall / 1e6, 2)}
+             "wall_mrays_per_s": round(n * args.steps / wall / 1e6, 2),
+             "launch_config": describe({best: timings[best]})[0] if best in timings else None,
+             "autotune_ms_on_2M_sample": describe(timings)}
 
     def timed(fn, reps=5):
         for _ in range(2):
@@ -155,10 +179,13 @@ def main() -> int:
         d_ctmax = torch.from_numpy(ctmax).to(dev)
         nc = len(crays)
         d_chits = torch.empty(nc * 16, dtype=torch.uint8, device=dev)
+        cbest, _ = autotune(bh, d_crays, d_ctmax, any_hit=False, sample=1 << 21)
         ms_c = timed(lambda: bh.closest_device(d_crays, d_ctmax, d_chits))
         ms_ca = timed(lambda: bh.any_device(d_crays, d_ctmax, d_occ))
         extra["coherent_closest_mrays_per_s"] = round(nc / (ms_c * 1e-3) / 1e6, 2)
         extra["coherent_anyhit_mrays_per_s"] = round(nc / (ms_ca * 1e-3) / 1e6, 2)
+        extra["coherent_launch_config"] = describe({cbest: 0.0})[0]
+        bh.set_trace_config(best.persistent, best.lds_stack, best.refill_lanes, best.grid_scale, best.chunk)
         # End to end through host buffers (H2D + trace + Intersection_C expansion + D2H), 1M rays.
         m = 1 << 20
         t1 = time.perf_counter()
@@ -220,7 +247,8 @@ def main() -> int:
                                "tMax U[0.1,2])",
                    "tris": args.tris, "rays": n, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+                     "algorithmic_bytes_per_launch": algo_bytes},
         "cpu_baseline": cpu,
         "extra": extra,
     }

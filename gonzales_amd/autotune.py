@@ -1,0 +1,104 @@
+"""Launch-configuration autotuner for the traversal kernels.
+
+The results of a traversal never depend on its launch configuration (every configuration runs
+the same per-ray algorithm); only the speed does, and the best choice depends on the scene and on
+how coherent the rays are (refilling idle lanes often pays off for coherent batches, rarely for
+incoherent ones). ``autotune`` times a small set of candidate configurations on a sample of the
+caller's batch, keeps the fastest, sets it on the scene handle and can persist the choice as JSON
+keyed by (scene signature, query kind, sample size).
+"""
+from __future__ import annotations
+
+import json
+import os
+from dataclasses import asdict, dataclass
+from pathlib import Path
+
+
+
+@dataclass(frozen=True)
+class TraceConfig:
+    persistent: int = 1
+    lds_stack: int = 16
+    refill_lanes: int = 56
+    grid_scale: int = 1
+    chunk: int = 64
+
+
+CANDIDATES = (
+    TraceConfig(0, 16, 16, 1, 64),
+    TraceConfig(1, 16, 16, 1, 64),
+    TraceConfig(1, 16, 32, 1, 64),
+    TraceConfig(1, 16, 48, 1, 64),
+    TraceConfig(1, 16, 56, 1, 64),
+    TraceConfig(1, 8, 56, 1, 64),
+    TraceConfig(2, 16, 16, 2, 64),
+)
+
+
+def scene_signature(bh) -> str:
+    info = bh.info()
+    return f"nodes{info.nodeCount}-prims{info.primCount}-depth{info.maxDepth}"
+
+
+def _time(fn, reps: int) -> float:
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    for _ in range(reps):
+        fn()
+    b.record(s)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def autotune(bh, d_rays, d_tmax, any_hit: bool = False, sample: int = 1 << 21, reps: int = 3,
+             candidates=CANDIDATES, cache: str | os.PathLike | None = None) -> tuple[TraceConfig, dict]:
+    """Times every candidate on the first `sample` rays (device tensors) and keeps the fastest.
+
+    Returns (best config, {config: ms}). With `cache`, a previous choice for the same scene, query
+    and sample size is reused without timing.
+    """
+    import torch
+
+    key = f"{scene_signature(bh)}|{'any' if any_hit else 'closest'}|{sample}"
+    if cache is not None and Path(cache).exists():
+        saved = json.loads(Path(cache).read_text())
+        if key in saved:
+            cfg = TraceConfig(**saved[key])
+            bh.set_trace_config(cfg.persistent, cfg.lds_stack, cfg.refill_lanes, cfg.grid_scale, cfg.chunk)
+            return cfg, {}
+    n = int(d_tmax.numel())
+    m = min(n, sample)
+    rays = d_rays[: m * 24]
+    tmax = d_tmax[:m]
+    out = torch.empty(m * (1 if any_hit else 16), dtype=torch.uint8, device=d_tmax.device)
+    timings = {}
+    for cfg in candidates:
+        bh.set_trace_config(cfg.persistent, cfg.lds_stack, cfg.refill_lanes, cfg.grid_scale, cfg.chunk)
+        if any_hit:
+            ms = _time(lambda: bh.any_device(rays, tmax, out), reps)
+        else:
+            ms = _time(lambda: bh.closest_device(rays, tmax, out), reps)
+        timings[cfg] = ms
+    bh.check_traversal()
+    best = min(timings, key=timings.get)
+    bh.set_trace_config(best.persistent, best.lds_stack, best.refill_lanes, best.grid_scale, best.chunk)
+    if cache is not None:
+        p = Path(cache)
+        saved = json.loads(p.read_text()) if p.exists() else {}
+        saved[key] = asdict(best)
+        p.write_text(json.dumps(saved, indent=1, sort_keys=True))
+    return best, timings
+
+
+def describe(timings: dict) -> list[dict]:
+    return [dict(asdict(c), ms=round(float(t), 3)) for c, t in sorted(timings.items(), key=lambda kv: kv[1])]
+
+
+__all__ = ["TraceConfig", "CANDIDATES", "autotune", "describe", "scene_signature"]

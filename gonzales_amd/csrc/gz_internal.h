@@ -76,7 +76,7 @@ struct Scene {
 
         // Traversal launch configuration and the persistent kernel's ray counters (a ring, so
         // launches queued on different streams never share a counter).
-        GzTraceConfig traceConfig{1, 16, 16, 1, 128, {0, 0, 0}};
+        GzTraceConfig traceConfig{1, 16, 56, 1, 64, {0, 0, 0}};
         bool boundsSafe = false;  // every node box coordinate is finite with magnitude <= 2^40
         int cuCount = 256;
         unsigned long long *dCounters = nullptr;
