@@ -66,6 +66,17 @@ def test_table_matches_scipy_joe_kuo(table):
     assert np.array_equal(table.reshape(1024, 52), ref)
 
 
+def test_direction_number_file_equals_the_reference_table():
+    # The committed data file (written from SciPy's copy of Joe & Kuo's table) holds exactly the
+    # first 1024 lines of the reference's Resources/new-joe-kuo-6.21201.
+    ref = Path("/root/reference/Sources/libgonzales/Resources/new-joe-kuo-6.21201")
+    if not ref.exists():
+        pytest.skip("reference tree not mounted")
+    ours = oracle.JOE_KUO.read_text().split("\n")
+    theirs = ref.read_text().split("\n")[:1024]
+    assert [l.split() for l in ours[:1024]] == [l.split() for l in theirs]
+
+
 def test_dimension_zero_is_van_der_corput(table):
     t = table.reshape(1024, 52)
     assert all(int(t[0, i]) == 1 << (31 - i) for i in range(32)) and not t[0, 32:].any()
