@@ -36,6 +36,23 @@ GZ_HIT = np.dtype({"names": ["prim", "t", "u", "v"], "formats": ["<i4", "<f4", "
                    "offsets": [0, 4, 8, 12], "itemsize": 16})
 TRACE_STATS = np.dtype({"names": ["interior", "tris"], "formats": ["<u4", "<u4"], "offsets": [0, 4],
                         "itemsize": 8})
+# MIS integrator records (include/gonzales_amd.h, GzPath / GzMaterial / GzAreaLight).
+GZ_PATH = np.dtype({"names": ["ray", "throughput", "estimate", "albedo", "firstNormal", "filterWeight", "pixelX",
+                              "pixelY", "sampleIndex", "dim", "lightDim", "bounce", "flags"],
+                    "formats": [RAY, ("<f4", 3), ("<f4", 3), ("<f4", 3), ("<f4", 3), "<f4", "<i4", "<i4", "<i4",
+                                "<i4", "<i4", "<i4", "<i4"],
+                    "offsets": [0, 24, 36, 48, 60, 72, 76, 80, 84, 88, 92, 96, 100], "itemsize": 112})
+GZ_MATERIAL = np.dtype({"names": ["type", "reflectance", "eta", "roughness", "conductorEta", "conductorK",
+                                  "remapRoughness"],
+                        "formats": ["<i4", ("<f4", 3), "<f4", "<f4", ("<f4", 3), ("<f4", 3), "<i4"],
+                        "offsets": [0, 4, 16, 20, 24, 36, 48], "itemsize": 64})
+GZ_AREA_LIGHT = np.dtype({"names": ["brightness", "mesh", "triangle"], "formats": [("<f4", 3), "<i4", "<i8"],
+                          "offsets": [0, 12, 16], "itemsize": 24})
+GZ_PATH_ACTIVE = 1
+GZ_PATH_SPECULAR = 2
+GZ_MAT_DIFFUSE = 1
+GZ_MIS_FIX_LIGHT_PDF = 1
+GZ_MIS_FIX_SKY_DIRECTION = 2
 
 GZ_OK = 0
 GZ_E_INVALID = -1
@@ -68,6 +85,11 @@ class GzTraceConfig(ctypes.Structure):
 
 class GzZSobolParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_int64), ("log2SamplesPerPixel", ctypes.c_int32), ("nBase4Digits", ctypes.c_int32)]
+
+
+class GzMisParams(ctypes.Structure):
+    _fields_ = [("maxDepth", ctypes.c_int32), ("flags", ctypes.c_int32), ("zsSeed", ctypes.c_int64),
+                ("zsLog2Spp", ctypes.c_int32), ("zsBase4Digits", ctypes.c_int32), ("rrSeed", ctypes.c_uint64)]
 
 
 class GzError(RuntimeError):
@@ -104,6 +126,9 @@ SIGNATURES = {
     "gz_zsobol_tile_device": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _I32, _P, _I32, _P]),
     "gz_sobol_matrices": (ctypes.c_int, [_P]),
     "gz_bvh2_build": (_I64, [_P, _I64, _P, _P]),
+    "gz_gpu_set_mis_scene": (ctypes.c_int, [_P, _P, _I32, _P, _I32, _P]),
+    "gz_gpu_mis_bounce_device": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
+    "gz_gpu_mis_trace_device": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
 }
 
 _lib = None
@@ -133,6 +158,18 @@ def last_error() -> str:
 def check(code: int, what: str = "") -> None:
     if code != GZ_OK:
         raise GzError(code, f"{what}: {last_error()}" if what else last_error())
+
+
+def concat(parts, dtype: np.dtype) -> np.ndarray:
+    """Concatenates record arrays into `dtype` with its C layout. (np.concatenate of records with
+    explicit offsets returns a packed dtype without the padding the C structs have.)"""
+    parts = [np.asarray(p, dtype) for p in parts]
+    out = np.zeros(sum(len(p) for p in parts), dtype)
+    at = 0
+    for p in parts:
+        out[at:at + len(p)] = p
+        at += len(p)
+    return out
 
 
 def ptr(a) -> int:

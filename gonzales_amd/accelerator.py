@@ -187,7 +187,7 @@ class BoundingHierarchy:
         lights = np.zeros(len(self.scene.light_emission), abi.MATERIAL)
         lights["type"] = 2
         lights["emission"] = self.scene.light_emission
-        self.materials_c = np.concatenate([mats, lights])
+        self.materials_c = abi.concat([mats, lights], abi.MATERIAL)
         base = len(mats)
         sel = self.prim_ids["type"] == PRIM_AREA_LIGHT
         self.prim_ids["materialIndex"][sel] = base + self.prim_ids["id1"][sel]

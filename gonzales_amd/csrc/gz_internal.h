@@ -49,8 +49,13 @@ struct DevMesh {
         int64_t vertexIndexCount;
 };
 
+// Light and material tables of the MIS integrator (gz_mis.hip), set by gz_gpu_set_mis_scene.
+struct MisScene;
+void release_mis(MisScene *m);
+
 struct Scene {
         std::mutex mu;
+        MisScene *mis = nullptr;
         int device = 0;
         hipStream_t stream = nullptr;  // used by the synchronous host-buffer entry points
 

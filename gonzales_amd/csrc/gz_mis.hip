@@ -1,0 +1,751 @@
+// Wavefront MIS path tracing on gfx950: the bounce of the reference's VolumePathIntegrator
+// (Sources/libgonzales/Integrator/VolumePathIntegrator.swift:35-463), one path per lane.
+//
+// A bounce of a path, in the reference's order:
+//   closest hit + surface interaction ......... BoundingHierarchy.swift:387-456, Triangle.swift:363-474
+//   miss: infinite light if bounce 0/specular .. VolumePathIntegrator.swift:380-393
+//   stop at maxDepth ........................... :400-402
+//   emission on bounce 0/specular (one-sided) .. :331-340, AreaLight.swift:13-19
+//   BSDF (area-light surfaces use materials[0]) :341-348
+//   one light by power (own sampler at Morton 0) PowerLightSampler.swift:41-49
+//   light sample + shadow ray (MIS weight) ...... :67-89, :173-195, Visibility.swift:5-24
+//   BSDF sample + ray (MIS weight, light pdf 0
+//     for triangles unless corrected) .......... :91-133, :197-211, Sampleable.swift:49-69
+//   continuation, spawn offset 1e-4 n ........... :358-369, Interaction.swift:32-53
+//   Russian roulette after bounce 1 ............. :269-283 (deterministic stream here)
+// Arithmetic follows the reference's operation order without contraction; sqrt and division are
+// IEEE-correct; cos/sin come from the device math library.
+//
+// The shadow ray and the BSDF ray are traced inline with the same traversal as the batch
+// kernels. A BSDF ray toward a chosen triangle light is skipped when its MIS weight is 0 (the
+// reference's light pdf quirk): it would contribute exactly 0.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "traverse_device.h"
+#include "zsobol_device.h"
+
+#pragma clang fp contract(off)
+
+namespace gz {
+
+const uint32_t *device_sobol_table();  // gz_zsobol.hip
+
+constexpr int kMisBlock = 256;
+constexpr float kPi = 3.14159265358979323846f;
+
+struct f3 {
+        float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 add(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 sub(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 mul(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ f3 divs(f3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ f3 neg(f3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float absdot(f3 a, f3 b) { return fabsf(dot(a, b)); }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+        return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float len2(f3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+__device__ __forceinline__ f3 normalized(f3 a) {  // Three.swift:108-117: v / |v|, or v if |v| = 0
+        const float l = sqrtf(len2(a));
+        return l == 0.0f ? a : divs(a, l);
+}
+// Geometry.swift:63-77
+__device__ __forceinline__ void coordinate_system(f3 v1, f3 &v2, f3 &v3) {
+        if (fabsf(v1.x) > fabsf(v1.y))
+                v2 = divs(mk(-v1.z, 0.0f, v1.x), sqrtf(v1.x * v1.x + v1.z * v1.z));
+        else
+                v2 = divs(mk(0.0f, v1.z, -v1.y), sqrtf(v1.y * v1.y + v1.z * v1.z));
+        v3 = cross(v1, v2);
+}
+
+__device__ __forceinline__ f3 cadd(f3 a, f3 b) { return add(a, b); }
+__device__ __forceinline__ f3 cmul(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ bool cblack(f3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
+__device__ __forceinline__ float cmax(f3 a) {  // max(r, max(g, b)), Swift max(x, y) = y >= x ? y : x
+        const float gb = a.z >= a.y ? a.z : a.y;
+        return gb >= a.x ? gb : a.x;
+}
+
+// Scene data of the MIS integrator on the device.
+struct DevLight {
+        float4 p0;  // w: area
+        float4 p1;
+        float4 p2;
+        float4 L;   // brightness
+};
+
+struct MisArgs {
+        const NodePair *nodes;
+        const TriRec *tris;
+        int rootDesc;
+        bool sceneSafe;
+        bool empty;
+        const PrimId_C *prims;
+        int64_t primCount;
+        const DevMesh *meshes;
+        int64_t meshCount;
+        const GzMaterial *mats;
+        int nMats;
+        const DevLight *lights;
+        int nLights;
+        const float *power;       // per choice (area lights, then the sky)
+        const float *cumulative;  // running sum of power
+        int nChoices;
+        float totalPower;
+        bool hasSky;
+        float sky[3];
+        const uint32_t *sobol;
+        ZSobolParams zs;
+        int maxDepth;
+        int flags;
+        uint64_t rrSeed;
+        int *err;
+};
+
+// Closest or any hit of one ray (the traversal of the batch kernels, inline).
+template <bool kAny>
+__device__ __forceinline__ bool trace_ray(const MisArgs &a, const Ray_C &ray, float tMax, int &prim, float &t,
+                                          float &u, float &v) {
+        if (a.empty) return false;
+        Lane L;
+        lane_init(L, ray, tMax, a.rootDesc, a.sceneSafe);
+        Stack<kLdsMax> S;
+        int st;
+        do {
+                st = step<kAny, false, kLdsMax>(L, S, a.nodes, a.tris);
+        } while (st == kStepGo);
+        if (st == kStepOverflow) {
+                atomicOr(a.err, 1);
+                return false;
+        }
+        prim = L.hitPrim;
+        t = L.tBest;
+        u = L.bu;
+        v = L.bv;
+        return prim >= 0;
+}
+
+struct Interaction {
+        f3 position, normal, shadingNormal, outgoing, dpdu;
+        int material;
+        int areaLight;
+};
+
+__device__ __forceinline__ bool tri_points(const MisArgs &a, const PrimId_C &pr, f3 &p0, f3 &p1, f3 &p2) {
+        int64_t mesh, base;
+        if (pr.type == 0) {
+                mesh = pr.id1;
+                base = pr.id2;
+        } else {
+                mesh = pr.id2 >> 32;
+                base = (pr.id2 & 0xFFFFFFFFLL) * 3;
+        }
+        if (mesh < 0 || mesh >= a.meshCount) return false;
+        const DevMesh m = a.meshes[mesh];
+        if (base < 0 || base + 2 >= m.vertexIndexCount) return false;
+        const int64_t i0 = m.vertexIndices[base], i1 = m.vertexIndices[base + 1], i2 = m.vertexIndices[base + 2];
+        if (i0 < 0 || i1 < 0 || i2 < 0 || i0 * 4 + 2 >= m.pointFloats || i1 * 4 + 2 >= m.pointFloats ||
+            i2 * 4 + 2 >= m.pointFloats)
+                return false;
+        p0 = mk(m.points[i0 * 4], m.points[i0 * 4 + 1], m.points[i0 * 4 + 2]);
+        p1 = mk(m.points[i1 * 4], m.points[i1 * 4 + 1], m.points[i1 * 4 + 2]);
+        p2 = mk(m.points[i2 * 4], m.points[i2 * 4 + 1], m.points[i2 * 4 + 2]);
+        return true;
+}
+
+// BoundingHierarchy.intersect -> Triangle.computeSurfaceInteraction for meshes without vertex
+// normals or uvs (Triangle.swift:363-474). false where the reference returns nil.
+__device__ bool surface_interaction(const MisArgs &a, const Ray_C &ray, int prim, float u, float v, Interaction &si) {
+        if (prim < 0 || prim >= a.primCount) return false;
+        const PrimId_C pr = a.prims[prim];
+        f3 p0, p1, p2;
+        if (!tri_points(a, pr, p0, p1, p2)) {
+                atomicOr(a.err, 2);
+                return false;
+        }
+        const float b0 = 1.0f - u - v, b1 = u, b2 = v;
+        const f3 dp02 = sub(p0, p2), dp12 = sub(p1, p2);
+        const f3 pHit = add(add(mul(p0, b0), mul(p1, b1)), mul(p2, b2));
+        const f3 n = normalized(cross(dp02, dp12));
+        const f3 gn = cross(sub(p2, p0), sub(p1, p0));
+        if (len2(gn) == 0.0f) return false;
+        f3 dpdu, dpdv;
+        coordinate_system(normalized(gn), dpdu, dpdv);
+        const f3 sn = n;
+        f3 S = normalized(dpdu);
+        f3 T = cross(S, sn);
+        if (len2(T) > 0.0f) {
+                T = normalized(T);
+                S = cross(T, sn);
+        } else {
+                coordinate_system(sn, S, T);
+        }
+        si.position = pHit;
+        si.normal = normalized(n);
+        si.shadingNormal = normalized(sn);
+        si.outgoing = normalized(neg(mk(ray.dirX, ray.dirY, ray.dirZ)));
+        si.dpdu = S;
+        if (pr.type == 3) {
+                si.areaLight = (int)pr.id1;
+                si.material = 0;
+        } else {
+                si.areaLight = -1;
+                si.material = (int)pr.materialIndex;
+        }
+        return true;
+}
+
+// Diffuse BSDF in its shading frame (BsdfFrame.swift:96-104, ShadingFrame.swift:27-77,
+// DiffuseBsdf.swift:15-21, Bsdf.swift:123-151, FramedBsdf.swift:18-83).
+struct Bsdf {
+        f3 x, y, z, ng;
+        f3 R;
+        int type;
+};
+
+__device__ __forceinline__ void bsdf_make(const MisArgs &a, const Interaction &si, Bsdf &b) {
+        b.y = normalized(si.dpdu);
+        b.z = si.shadingNormal;
+        b.x = cross(b.y, b.z);
+        b.ng = si.normal;
+        const int mi = si.material >= 0 && si.material < a.nMats ? si.material : 0;
+        const GzMaterial m = a.mats[mi];
+        b.type = m.type;
+        b.R = mk(m.reflectance[0], m.reflectance[1], m.reflectance[2]);
+}
+__device__ __forceinline__ f3 to_local(const Bsdf &b, f3 w) { return normalized(mk(dot(w, b.x), dot(w, b.y), dot(w, b.z))); }
+__device__ __forceinline__ f3 to_world(const Bsdf &b, f3 l) { return add(add(mul(b.x, l.x), mul(b.y, l.y)), mul(b.z, l.z)); }
+__device__ __forceinline__ f3 bsdf_f(const Bsdf &b) { return divs(b.R, kPi); }
+__device__ __forceinline__ f3 bsdf_eval_world(const Bsdf &b, f3 wo, f3 wi) {
+        const bool reflect = dot(wi, b.ng) * dot(wo, b.ng) > 0.0f;
+        return reflect ? bsdf_f(b) : mk(0.0f, 0.0f, 0.0f);
+}
+__device__ __forceinline__ float bsdf_pdf_world(const Bsdf &b, f3 wo, f3 wi) {
+        const f3 il = to_local(b, wi), ol = to_local(b, wo);
+        if (ol.z == 0.0f) return 0.0f;
+        if (!(ol.z * il.z > 0.0f)) return 0.0f;
+        return fabsf(il.z) / kPi;
+}
+// Sampling.swift:3-25
+__device__ __forceinline__ f3 cosine_hemisphere(float u0, float u1) {
+        const float ox = 2.0f * u0 - 1.0f, oy = 2.0f * u1 - 1.0f;
+        float x = 0.0f, y = 0.0f;
+        if (!(ox == 0.0f && oy == 0.0f)) {
+                float theta, r;
+                if (fabsf(ox) > fabsf(oy)) {
+                        r = ox;
+                        theta = (kPi / 4.0f) * (oy / ox);
+                } else {
+                        r = oy;
+                        theta = (kPi / 2.0f) - (kPi / 4.0f) * (ox / oy);
+                }
+                x = r * cosf(theta);
+                y = r * sinf(theta);
+        }
+        const float z2 = 1.0f - x * x - y * y;
+        return mk(x, y, sqrtf(z2 > 0.0f ? z2 : 0.0f));
+}
+__device__ __forceinline__ void bsdf_sample_world(const Bsdf &b, f3 wo, float u0, float u1, f3 &f, f3 &wi, float &pdf) {
+        const f3 ol = to_local(b, wo);
+        f3 il = cosine_hemisphere(u0, u1);
+        if (ol.z < 0.0f) il.z = -il.z;
+        pdf = (ol.z * il.z > 0.0f) ? fabsf(il.z) / kPi : 0.0f;
+        f = bsdf_f(b);
+        wi = to_world(b, il);
+}
+__device__ __forceinline__ float power_heuristic(float f, float g) {
+        if (f == 0.0f || g == 0.0f) return 0.0f;
+        return (f * f) / (f * f + g * g);
+}
+
+// PowerLightSampler.chooseLight + lowerBound (PowerLightSampler.swift:4-49)
+__device__ __forceinline__ int choose_light(const MisArgs &a, float u, float &pdf) {
+        const float key = u * a.totalPower;
+        int lo = 0, hi = a.nChoices, idx = -1;
+        while (lo < hi) {
+                const int mid = lo + (hi - lo) / 2;
+                const float cm = a.cumulative[mid];
+                if (cm == key) {
+                        idx = mid;
+                        break;
+                }
+                if (cm < key)
+                        lo = mid + 1;
+                else
+                        hi = mid;
+        }
+        if (idx < 0) idx = lo;
+        if (idx >= a.nChoices) idx = a.nChoices - 1;
+        pdf = a.power[idx] / a.totalPower;
+        return idx;
+}
+
+// InfiniteLight.equalAreaSquareToSphere (InfiniteLight.swift:112-127); `unit` applies the factor
+// sqrt(2 - r^2) to x and y that the reference omits (GZ_MIS_FIX_SKY_DIRECTION).
+__device__ __forceinline__ f3 equal_area_square_to_sphere(float tu, float tv, bool unit) {
+        const float u = 2.0f * tu - 1.0f, v = 2.0f * tv - 1.0f;
+        const float au = fabsf(u), av = fabsf(v);
+        const float sd = 1.0f - (au + av);
+        const float ad = fabsf(sd);
+        const float r = 1.0f - ad;
+        const float phi = (r == 0.0f ? 1.0f : (av - au) / r + 1.0f) * (kPi / 4.0f);
+        const float zz = 1.0f - r * r;
+        const float z = copysignf(zz > 0.0f ? zz : 0.0f, sd);
+        const float cp = cosf(phi), sp = sinf(phi);
+        float rr = r;
+        if (unit) {
+                const float s = 2.0f - r * r;
+                rr = r * sqrtf(s > 0.0f ? s : 0.0f);
+        }
+        return mk(copysignf(rr * cp, u), copysignf(rr * sp, v), z);
+}
+
+struct LightSample {
+        f3 L, dir, target;
+        float pdf;
+};
+
+// AreaLight.sample (AreaLight.swift:21-32) with Triangle.sample and Sampleable.sample(point:)
+// (Triangle.swift:517-544, Sampleable.swift:33-44), or the constant sky (InfiniteLight.swift:39-65).
+__device__ __forceinline__ void light_sample(const MisArgs &a, int choice, f3 point, float u0, float u1, LightSample &ls) {
+        if (choice < a.nLights) {
+                const DevLight dl = a.lights[choice];
+                const f3 p0 = mk(dl.p0.x, dl.p0.y, dl.p0.z), p1 = mk(dl.p1.x, dl.p1.y, dl.p1.z);
+                const f3 p2 = mk(dl.p2.x, dl.p2.y, dl.p2.z);
+                const float su0 = sqrtf(u0);
+                const float b0 = 1.0f - su0, b1 = u1 * su0;
+                const f3 pos = add(add(mul(p0, b0), mul(p1, b1)), mul(p2, 1.0f - b0 - b1));
+                const f3 ln = normalized(cross(sub(p1, p0), sub(p2, p0)));
+                float pdf = 1.0f / dl.p0.w;
+                const f3 inc = normalized(sub(pos, point));
+                const float d2 = len2(sub(point, pos));
+                const float angle = absdot(ln, neg(inc));
+                pdf = pdf * (d2 / angle);
+                ls.dir = normalized(sub(pos, point));
+                ls.target = pos;
+                ls.L = dot(ln, neg(ls.dir)) > 0.0f ? mk(dl.L.x, dl.L.y, dl.L.z) : mk(0.0f, 0.0f, 0.0f);
+                ls.pdf = pdf;
+                return;
+        }
+        const f3 d = equal_area_square_to_sphere(u0, u1, (a.flags & GZ_MIS_FIX_SKY_DIRECTION) != 0);
+        ls.dir = d;
+        ls.pdf = 1.0f / (4.0f * kPi);
+        ls.target = add(point, mul(d, 100.0f));
+        ls.L = mk(a.sky[0], a.sky[1], a.sky[2]);
+}
+
+// Visibility.occluded via spawnRay(from:target:) (Visibility.swift:5-24, Interaction.swift:15-24):
+// origin from + 1e-4 (target - from), unnormalised direction, tMax 1 - 1e-5.
+__device__ __forceinline__ bool occluded(const MisArgs &a, f3 from, f3 target) {
+        const f3 o = add(from, mul(sub(target, from), 0.0001f));
+        const f3 d = sub(target, o);
+        Ray_C r{o.x, o.y, o.z, d.x, d.y, d.z};
+        int prim;
+        float t, u, v;
+        return trace_ray<true>(a, r, 1.0f - 0.00001f, prim, t, u, v);
+}
+
+// Interaction.spawnRay(inDirection:) (Interaction.swift:32-53)
+__device__ __forceinline__ Ray_C spawn(const Interaction &si, f3 dir) {
+        const f3 off = mul(si.normal, 0.0001f);
+        const f3 o = dot(dir, si.normal) > 0.0f ? add(si.position, off) : add(si.position, neg(off));
+        return Ray_C{o.x, o.y, o.z, dir.x, dir.y, dir.z};
+}
+
+// Light pdf of a BSDF-sampled direction (Sampleable.swift:49-69): 0 for triangles in the
+// reference (tHit = 0); with GZ_MIS_FIX_LIGHT_PDF the chosen triangle is intersected.
+__device__ float light_pdf(const MisArgs &a, int choice, const Interaction &si, f3 dir) {
+        if (choice >= a.nLights) return 1.0f / (4.0f * kPi);
+        if (!(a.flags & GZ_MIS_FIX_LIGHT_PDF)) return 0.0f;
+        const DevLight dl = a.lights[choice];
+        const f3 p0 = mk(dl.p0.x, dl.p0.y, dl.p0.z), p1 = mk(dl.p1.x, dl.p1.y, dl.p1.z);
+        const f3 p2 = mk(dl.p2.x, dl.p2.y, dl.p2.z);
+        const Ray_C r = spawn(si, dir);
+        const f3 e1 = sub(p1, p0), e2 = sub(p2, p0);
+        const f3 d = mk(r.dirX, r.dirY, r.dirZ), o = mk(r.orgX, r.orgY, r.orgZ);
+        const f3 pv = cross(d, e2);
+        const float det = dot(e1, pv);
+        if (det > -1e-12f && det < 1e-12f) return 0.0f;
+        const float inv = 1.0f / det;
+        const f3 tv = sub(o, p0);
+        const float uu = dot(tv, pv) * inv;
+        if (uu < 0.0f || uu > 1.0f) return 0.0f;
+        const f3 qv = cross(tv, e1);
+        const float vv = dot(d, qv) * inv;
+        if (vv < 0.0f || uu + vv > 1.0f) return 0.0f;
+        const float t = dot(e2, qv) * inv;
+        if (t <= 0.0f) return 0.0f;
+        const f3 hitp = add(o, mul(d, t));
+        const float d2 = len2(sub(si.position, hitp));
+        const f3 ln = normalized(cross(sub(p0, p2), sub(p1, p2)));
+        const float angle = absdot(ln, neg(dir));
+        const float density = d2 / (angle * dl.p0.w);
+        return isinf(density) ? 0.0f : density;
+}
+
+// Deterministic Russian-roulette draw: top 24 bits of SplitMix64(seed, pixel, sample, bounce).
+__device__ __forceinline__ float rr_draw(uint64_t seed, int px, int py, int si, int bounce) {
+        uint64_t z = seed ^ ((uint64_t)(uint32_t)px << 40) ^ ((uint64_t)(uint32_t)py << 20) ^ (uint64_t)(uint32_t)si;
+        z += 0x9E3779B97F4A7C15ull * (uint64_t)(uint32_t)(bounce + 1);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z = z ^ (z >> 31);
+        return (float)(uint32_t)(z >> 40) * (1.0f / 16777216.0f);
+}
+
+__global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__restrict__ paths, int64_t n) {
+        const int64_t i = (int64_t)blockIdx.x * kMisBlock + threadIdx.x;
+        if (i >= n) return;
+        GzPath p = paths[i];
+        if (!(p.flags & GZ_PATH_ACTIVE)) return;
+        if (p.bounce > a.maxDepth) {
+                paths[i].flags = p.flags & ~GZ_PATH_ACTIVE;
+                return;
+        }
+        f3 T = mk(p.throughput[0], p.throughput[1], p.throughput[2]);
+        f3 E = mk(p.estimate[0], p.estimate[1], p.estimate[2]);
+        bool specular = (p.flags & GZ_PATH_SPECULAR) != 0;
+        bool go = false;
+        int prim = -1;
+        float th, hu, hv;
+        Interaction si;
+        const bool hit = trace_ray<false>(a, p.ray, INFINITY, prim, th, hu, hv) &&
+                         surface_interaction(a, p.ray, prim, hu, hv, si);
+        ZSobolState zs;
+        zs.morton = (a.zs.log2spp >= 64 ? 0ull : zs_morton2((uint32_t)p.pixelX, (uint32_t)p.pixelY) << a.zs.log2spp) |
+                    (uint64_t)(uint32_t)p.sampleIndex;
+        zs.dim = p.dim;
+        ZSobolState lzs;
+        lzs.morton = 0;  // the light chooser's sampler never starts a pixel sample
+        lzs.dim = p.lightDim;
+        if (!hit) {
+                if ((p.bounce == 0 || specular) && a.hasSky) E = cadd(E, cmul(T, mk(a.sky[0], a.sky[1], a.sky[2])));
+        } else if (!cblack(T) && p.bounce < a.maxDepth) {
+                if ((p.bounce == 0 || specular) && si.areaLight >= 0 && si.areaLight < a.nLights) {
+                        const float4 Lb = a.lights[si.areaLight].L;
+                        const f3 Le = dot(si.normal, si.outgoing) > 0.0f ? mk(Lb.x, Lb.y, Lb.z) : mk(0.0f, 0.0f, 0.0f);
+                        E = cadd(E, cmul(T, Le));
+                }
+                Bsdf b;
+                bsdf_make(a, si, b);
+                if (p.bounce == 0) {
+                        p.albedo[0] = b.R.x;
+                        p.albedo[1] = b.R.y;
+                        p.albedo[2] = b.R.z;
+                        p.firstNormal[0] = si.normal.x;
+                        p.firstNormal[1] = si.normal.y;
+                        p.firstNormal[2] = si.normal.z;
+                }
+                // sampleOneLight -> estimateDirect -> sampleMultipleImportance
+                float choicePdf;
+                const float ul = zs_get1d(a.zs, lzs, a.sobol);
+                const int choice = choose_light(a, ul, choicePdf);
+                float u0, u1;
+                zs_get2d(a.zs, zs, a.sobol, u0, u1);
+                LightSample ls;
+                light_sample(a, choice, si.position, u0, u1, ls);
+                f3 Lest = mk(0.0f, 0.0f, 0.0f), linc = mk(0.0f, 0.0f, 0.0f);
+                float lpdf = 0.0f;
+                if (!cblack(ls.L) && !isinf(ls.pdf) && !occluded(a, si.position, ls.target)) {
+                        const f3 scatter = mul(bsdf_eval_world(b, si.outgoing, ls.dir), absdot(ls.dir, si.shadingNormal));
+                        Lest = cmul(scatter, ls.L);
+                        lpdf = ls.pdf;
+                        linc = ls.dir;
+                }
+                const float bpdfL = bsdf_pdf_world(b, si.outgoing, linc);
+                const float wl = power_heuristic(lpdf, bpdfL);
+                const f3 lc = lpdf == 0.0f ? mk(0.0f, 0.0f, 0.0f) : divs(mul(Lest, wl), lpdf);
+                float v0, v1, v2;
+                zs_get3d(a.zs, zs, a.sobol, v0, v1, v2);
+                f3 bf, bwi;
+                float bpdf;
+                bsdf_sample_world(b, si.outgoing, v0, v1, bf, bwi, bpdf);
+                bf = mul(bf, absdot(bwi, si.shadingNormal));
+                f3 bc = mk(0.0f, 0.0f, 0.0f);
+                const bool weighted = choice >= a.nLights || (a.flags & GZ_MIS_FIX_LIGHT_PDF);
+                if (weighted && !cblack(bf) && bpdf > 0.0f) {
+                        const Ray_C r = spawn(si, bwi);
+                        int bprim = -1;
+                        float bt, bu, bv;
+                        Interaction bsi;
+                        f3 bL = mk(0.0f, 0.0f, 0.0f);
+                        bool ok = false;
+                        if (trace_ray<false>(a, r, INFINITY, bprim, bt, bu, bv) && surface_interaction(a, r, bprim, bu, bv, bsi)) {
+                                if (bsi.areaLight >= 0 && bsi.areaLight < a.nLights) {
+                                        const float4 Lb = a.lights[bsi.areaLight].L;
+                                        bL = dot(bsi.normal, bwi) > 0.0f ? mk(Lb.x, Lb.y, Lb.z) : mk(0.0f, 0.0f, 0.0f);
+                                        ok = true;
+                                }
+                        } else if (a.hasSky) {
+                                bL = mk(a.sky[0], a.sky[1], a.sky[2]);
+                                ok = true;
+                        }
+                        if (ok) {
+                                const f3 best = cmul(bf, bL);
+                                const float lden = light_pdf(a, choice, si, bwi);
+                                const float wb = power_heuristic(bpdf, lden);
+                                bc = divs(mul(best, wb), bpdf);
+                        }
+                }
+                const f3 est = cadd(lc, bc);
+                E = cadd(E, cmul(T, divs(est, choicePdf)));
+                // continuation: bsdf.sampleWorldSpace(outgoing:uSample: get3D())
+                float w0, w1, w2;
+                zs_get3d(a.zs, zs, a.sobol, w0, w1, w2);
+                f3 f, wi;
+                float pdf;
+                bsdf_sample_world(b, si.outgoing, w0, w1, f, wi, pdf);
+                if (pdf != 0.0f && !isnan(pdf)) {
+                        specular = false;
+                        T = cmul(T, divs(mul(f, absdot(wi, si.normal)), pdf));
+                        p.ray = spawn(si, wi);
+                        go = true;
+                        if (cmax(T) < 1.0f && p.bounce > 1) {
+                                const float m = cmax(T);
+                                const float q = 1.0f - m > 0.0f ? 1.0f - m : 0.0f;
+                                const float u = rr_draw(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce);
+                                if (u < q)
+                                        go = false;
+                                else
+                                        T = divs(T, 1.0f - q);
+                        }
+                }
+        }
+        p.throughput[0] = T.x;
+        p.throughput[1] = T.y;
+        p.throughput[2] = T.z;
+        p.estimate[0] = E.x;
+        p.estimate[1] = E.y;
+        p.estimate[2] = E.z;
+        p.dim = zs.dim;
+        p.lightDim = lzs.dim;
+        p.flags = (go ? GZ_PATH_ACTIVE : 0) | (specular ? GZ_PATH_SPECULAR : 0);
+        if (go) p.bounce += 1;
+        paths[i] = p;
+}
+
+// Light and material tables of one scene handle.
+struct MisScene {
+        GzMaterial *dMats = nullptr;
+        int nMats = 0;
+        DevLight *dLights = nullptr;
+        int nLights = 0;
+        float *dPower = nullptr;
+        float *dCumulative = nullptr;
+        int nChoices = 0;
+        float totalPower = 0.0f;
+        bool hasSky = false;
+        float sky[3] = {0, 0, 0};
+};
+
+void release_mis(MisScene *m) {
+        if (!m) return;
+        void *ptrs[] = {m->dMats, m->dLights, m->dPower, m->dCumulative};
+        for (void *p : ptrs)
+                if (p) (void)hipFree(p);
+        delete m;
+}
+
+// Reads one light triangle's vertices from the device-resident mesh arrays (validated).
+static int light_vertices(Scene *s, int32_t i, const GzAreaLight &L, float p[3][3]) {
+        if (L.mesh < 0 || L.mesh >= s->meshCount) {
+                set_error("gz_gpu_set_mis_scene: light %d names mesh %d of %lld", i, L.mesh, (long long)s->meshCount);
+                return GZ_E_INVALID;
+        }
+        DevMesh m;
+        if (!hip_ok(hipMemcpy(&m, s->dMeshes + L.mesh, sizeof m, hipMemcpyDeviceToHost), "read mesh")) return GZ_E_HIP;
+        const int64_t base = 3 * L.triangle;
+        if (L.triangle < 0 || base + 2 >= m.vertexIndexCount) {
+                set_error("gz_gpu_set_mis_scene: light %d triangle %lld is outside mesh %d", i, (long long)L.triangle,
+                          L.mesh);
+                return GZ_E_INVALID;
+        }
+        int64_t vi[3];
+        if (!hip_ok(hipMemcpy(vi, m.vertexIndices + base, sizeof vi, hipMemcpyDeviceToHost), "read indices"))
+                return GZ_E_HIP;
+        for (int k = 0; k < 3; ++k) {
+                if (vi[k] < 0 || vi[k] * 4 + 2 >= m.pointFloats) {
+                        set_error("gz_gpu_set_mis_scene: light %d has a vertex index out of range", i);
+                        return GZ_E_INVALID;
+                }
+                if (!hip_ok(hipMemcpy(p[k], m.points + vi[k] * 4, 3 * sizeof(float), hipMemcpyDeviceToHost),
+                            "read points"))
+                        return GZ_E_HIP;
+        }
+        return GZ_OK;
+}
+
+template <class T>
+static bool put(T **dst, const T *src, size_t count, const char *what) {
+        return hip_ok(hipMalloc((void **)dst, sizeof(T) * (count > 0 ? count : 1)), what) &&
+               (count == 0 || hip_ok(hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice), what));
+}
+
+static int set_mis_scene(Scene *s, const GzMaterial *materials, int32_t materialCount, const GzAreaLight *lights,
+                         int32_t lightCount, const float *skyRadiance) {
+        std::vector<DevLight> dl((size_t)lightCount);
+        std::vector<float> power;
+        // Light powers and the power CDF in the reference's float operation order
+        // (AreaLight.swift:47-49, InfiniteLight.swift:135-138, PowerLightSampler.swift:4-20).
+        for (int32_t i = 0; i < lightCount; ++i) {
+                float p[3][3];
+                const int rc = light_vertices(s, i, lights[i], p);
+                if (rc != GZ_OK) return rc;
+                const float e1x = p[1][0] - p[0][0], e1y = p[1][1] - p[0][1], e1z = p[1][2] - p[0][2];
+                const float e2x = p[2][0] - p[0][0], e2y = p[2][1] - p[0][1], e2z = p[2][2] - p[0][2];
+                const float cx = e1y * e2z - e1z * e2y, cy = e1z * e2x - e1x * e2z, cz = e1x * e2y - e1y * e2x;
+                const float area = 0.5f * sqrtf(cx * cx + cy * cy + cz * cz);  // Triangle.swift:522-525
+                const float *b = lights[i].brightness;
+                dl[(size_t)i].p0 = make_float4(p[0][0], p[0][1], p[0][2], area);
+                dl[(size_t)i].p1 = make_float4(p[1][0], p[1][1], p[1][2], 0.0f);
+                dl[(size_t)i].p2 = make_float4(p[2][0], p[2][1], p[2][2], 0.0f);
+                dl[(size_t)i].L = make_float4(b[0], b[1], b[2], 0.0f);
+                const float avg = (b[0] + b[1] + b[2]) / 3.0f;
+                power.push_back(avg * area * kPi);
+        }
+        if (skyRadiance) {
+                const float avg = (skyRadiance[0] + skyRadiance[1] + skyRadiance[2]) / 3.0f;
+                const float r = 100.0f / 2.0f;
+                power.push_back(kPi * (r * r) * avg);
+        }
+        std::vector<float> cum(power.size());
+        float total = 0.0f;
+        for (size_t k = 0; k < power.size(); ++k) total = total + power[k];
+        for (size_t k = 0; k < power.size(); ++k) cum[k] = k == 0 ? power[0] : cum[k - 1] + power[k];
+        if (!(total > 0.0f) || !std::isfinite(total)) {
+                set_error("gz_gpu_set_mis_scene: total light power %g is not positive and finite", (double)total);
+                return GZ_E_INVALID;
+        }
+        MisScene *m = new MisScene;
+        m->nMats = materialCount;
+        m->nLights = lightCount;
+        m->nChoices = (int)power.size();
+        m->totalPower = total;
+        m->hasSky = skyRadiance != nullptr;
+        if (skyRadiance)
+                for (int k = 0; k < 3; ++k) m->sky[k] = skyRadiance[k];
+        const bool ok = put(&m->dMats, materials, (size_t)materialCount, "upload materials") &&
+                        put(&m->dLights, dl.data(), dl.size(), "upload lights") &&
+                        put(&m->dPower, power.data(), power.size(), "upload light powers") &&
+                        put(&m->dCumulative, cum.data(), cum.size(), "upload light CDF");
+        if (!ok) {
+                release_mis(m);
+                return GZ_E_HIP;
+        }
+        if (s->stream) (void)hipStreamSynchronize(s->stream);
+        release_mis(s->mis);
+        s->mis = m;
+        return GZ_OK;
+}
+
+}  // namespace gz
+
+using namespace gz;
+
+extern "C" int gz_gpu_set_mis_scene(void *handle, const GzMaterial *materials, int32_t materialCount,
+                                    const GzAreaLight *lights, int32_t lightCount, const float *skyRadiance) {
+        clear_error();
+        if (!handle || materialCount < 1 || !materials || lightCount < 0 || (lightCount > 0 && !lights)) {
+                set_error("gz_gpu_set_mis_scene: need a handle, at least one material and, for lightCount > 0, lights");
+                return GZ_E_INVALID;
+        }
+        if (lightCount == 0 && !skyRadiance) {
+                set_error("gz_gpu_set_mis_scene: the scene has no light");
+                return GZ_E_INVALID;
+        }
+        for (int32_t i = 0; i < materialCount; ++i)
+                if (materials[i].type != GZ_MAT_DIFFUSE) {
+                        set_error("gz_gpu_set_mis_scene: material %d has type %d; only diffuse (%d) is supported", i,
+                                  materials[i].type, GZ_MAT_DIFFUSE);
+                        return GZ_E_INVALID;
+                }
+        Scene *s = static_cast<Scene *>(handle);
+        std::lock_guard<std::mutex> lock(s->mu);
+        if (!enter_device(s)) return GZ_E_HIP;
+        return set_mis_scene(s, materials, materialCount, lights, lightCount, skyRadiance);
+}
+
+static bool mis_args(Scene *s, const GzMisParams *params, MisArgs &a) {
+        const MisScene *ms = s->mis;
+        if (!ms) {
+                set_error("no MIS scene on this handle: call gz_gpu_set_mis_scene first");
+                return false;
+        }
+        if (!params || params->maxDepth < 0 || params->maxDepth > 1024 || params->zsLog2Spp < 0 ||
+            params->zsLog2Spp > 64 || params->zsBase4Digits < 0 || params->zsBase4Digits > 64 || params->zsSeed < 0) {
+                set_error("bad MIS parameters");
+                return false;
+        }
+        const uint32_t *table = device_sobol_table();
+        if (!table) return false;
+        a.nodes = s->dNodes;
+        a.tris = s->dTris;
+        a.rootDesc = s->rootDesc;
+        a.sceneSafe = s->boundsSafe;
+        a.empty = s->empty;
+        a.prims = s->dPrims;
+        a.primCount = s->primCount;
+        a.meshes = s->dMeshes;
+        a.meshCount = s->meshCount;
+        a.mats = ms->dMats;
+        a.nMats = ms->nMats;
+        a.lights = ms->dLights;
+        a.nLights = ms->nLights;
+        a.power = ms->dPower;
+        a.cumulative = ms->dCumulative;
+        a.nChoices = ms->nChoices;
+        a.totalPower = ms->totalPower;
+        a.hasSky = ms->hasSky;
+        for (int k = 0; k < 3; ++k) a.sky[k] = ms->sky[k];
+        a.sobol = table;
+        a.zs = ZSobolParams{params->zsSeed, params->zsLog2Spp, params->zsBase4Digits};
+        a.maxDepth = params->maxDepth;
+        a.flags = params->flags;
+        a.rrSeed = params->rrSeed;
+        a.err = s->dErr;
+        return true;
+}
+
+static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisParams *params, void *stream,
+                      bool toEnd, const char *who) {
+        clear_error();
+        if (!handle || count < 0 || (count > 0 && !d_paths)) {
+                set_error("%s: bad arguments", who);
+                return GZ_E_INVALID;
+        }
+        Scene *s = static_cast<Scene *>(handle);
+        std::lock_guard<std::mutex> lock(s->mu);
+        if (!enter_device(s)) return GZ_E_HIP;
+        MisArgs a;
+        if (!mis_args(s, params, a)) return GZ_E_INVALID;
+        if (count == 0) return GZ_OK;
+        const int64_t blocks = (count + kMisBlock - 1) / kMisBlock;
+        if (blocks > 0x7fffffffLL) {
+                set_error("%s: %lld paths exceed one launch", who, (long long)count);
+                return GZ_E_INVALID;
+        }
+        // A path makes at most maxDepth + 1 bounces (bounce 0 .. maxDepth); a launch after the
+        // last finds every path inactive.
+        const int launches = toEnd ? params->maxDepth + 1 : 1;
+        for (int b = 0; b < launches; ++b) {
+                hipLaunchKernelGGL(k_mis_bounce, dim3((unsigned)blocks), dim3(kMisBlock), 0,
+                                   static_cast<hipStream_t>(stream), a, d_paths, count);
+                if (!hip_ok(hipGetLastError(), "MIS bounce launch")) return GZ_E_HIP;
+        }
+        return GZ_OK;
+}
+
+extern "C" int gz_gpu_mis_bounce_device(void *handle, GzPath *d_paths, int64_t count, const GzMisParams *params,
+                                        void *stream) {
+        return mis_launch(handle, d_paths, count, params, stream, false, "gz_gpu_mis_bounce_device");
+}
+
+extern "C" int gz_gpu_mis_trace_device(void *handle, GzPath *d_paths, int64_t count, const GzMisParams *params,
+                                       void *stream) {
+        return mis_launch(handle, d_paths, count, params, stream, true, "gz_gpu_mis_trace_device");
+}

@@ -65,6 +65,7 @@ static void free_scene(Scene *s) {
         int cur = -1;
         if (hipGetDevice(&cur) == hipSuccess && cur != s->device) (void)hipSetDevice(s->device);
         if (s->stream) (void)hipStreamSynchronize(s->stream);
+        release_mis(s->mis);
         void *ptrs[] = {s->dNodes, s->dTris, s->dPrims, s->dMeshes, s->dMaterials, s->dErr, s->dStage, s->dCounters};
         for (void *p : ptrs)
                 if (p) (void)hipFree(p);

@@ -166,7 +166,7 @@ def cornell_box():
     lv, lidx = _quad([-0.24, 1.98, -0.22], [0.23, 1.98, -0.22], [0.23, 1.98, 0.16], [-0.24, 1.98, 0.16])
     meshes.append(TriangleMesh.from_xyz(np.asarray(lv, np.float32), np.asarray(lidx, np.int64)))
     parts.append(Primitives.area_lights(li, 2, 0))
-    scene = Scene(meshes, np.concatenate(mats),
+    scene = Scene(meshes, abi.concat(mats, abi.MATERIAL),
                   light_emission=np.array([[17, 12, 4], [17, 12, 4]], np.float32))
     camera = {"eye": (0.0, 1.0, 6.8), "look": (0.0, 1.0, 0.0), "up": (0.0, 1.0, 0.0), "fov": 19.5}
     return scene, Primitives.concat(parts), camera

@@ -257,6 +257,87 @@ int gz_gpu_shade_device(void *handle, struct PathState_C *d_paths, int64_t count
                         const struct Intersection_C *d_isects, void *stream);
 
 /* ---------------------------------------------------------------------------------------- */
+/* MIS path tracing: the bounce of VolumePathIntegrator (Integrator/VolumePathIntegrator.swift) */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Material parameters the MIS integrator shades with (the reference's Material/ types, resolved
+ * on the host: constant textures only). type: GZ_MAT_DIFFUSE (Diffuse.swift:138-157). */
+struct GzMaterial {
+        int32_t type;
+        float reflectance[3];
+        float eta;
+        float roughness;
+        float conductorEta[3];
+        float conductorK[3];
+        int32_t remapRoughness;
+        int32_t reserved[3];
+};
+
+/* One triangle area light (AreaLight.swift:3-49): constant radiance on triangle `triangle` of
+ * mesh `mesh`; its index is the id1 of its type-3 PrimId. */
+struct GzAreaLight {
+        float brightness[3];
+        int32_t mesh;
+        int64_t triangle;
+};
+
+/* Path state of the wavefront MIS integrator (PathState, VolumePathIntegrator.swift:19-31, plus
+ * the Z-Sobol sampler state of the path and of the light chooser). 112 bytes. */
+struct GzPath {
+        struct Ray_C ray;
+        float throughput[3];
+        float estimate[3];
+        float albedo[3];
+        float firstNormal[3];
+        float filterWeight;
+        int32_t pixelX, pixelY, sampleIndex;
+        int32_t dim;      /* next dimension of the path's Z-Sobol sampler */
+        int32_t lightDim; /* next dimension of the light chooser's sampler (Morton index 0) */
+        int32_t bounce;
+        int32_t flags;    /* GZ_PATH_ACTIVE | GZ_PATH_SPECULAR */
+        int32_t pad[2];
+};
+
+struct GzMisParams {
+        int32_t maxDepth;       /* Integrator "maxdepth" (RenderConfiguration.swift:143-146) */
+        int32_t flags;          /* GZ_MIS_FIX_LIGHT_PDF | GZ_MIS_FIX_SKY_DIRECTION */
+        int64_t zsSeed;         /* Z-Sobol seed and constants, as gz_zsobol_params returns them */
+        int32_t zsLog2Spp;
+        int32_t zsBase4Digits;
+        uint64_t rrSeed;        /* Russian-roulette stream */
+};
+
+enum { GZ_PATH_ACTIVE = 1, GZ_PATH_SPECULAR = 2 };
+enum { GZ_MAT_DIFFUSE = 1 };
+/* Off (the default) reproduces the reference: the light pdf of a BSDF-sampled direction toward a
+ * triangle light is 0 (Sampleable.swift:57-58 passes tHit = 0), so BSDF samples of area lights
+ * get MIS weight 0. On, the pdf is computed from the chosen triangle.
+ * GZ_MIS_FIX_SKY_DIRECTION: off reproduces InfiniteLight.equalAreaSquareToSphere
+ * (InfiniteLight.swift:112-127), whose x and y lack the factor sqrt(2 - r^2), so its directions
+ * are shorter than 1 away from the poles and equator; on, the factor is applied (unit directions,
+ * uniform on the sphere). */
+enum { GZ_MIS_FIX_LIGHT_PDF = 1, GZ_MIS_FIX_SKY_DIRECTION = 2 };
+
+/* Sets the shading scene of a handle: materials indexed by PrimId.materialIndex (area-light
+ * surfaces shade with materials[0], as AreaLight.computeSurfaceInteraction leaves the index at 0),
+ * the triangle area lights indexed by their PrimId.id1, and an optional constant infinite light
+ * (skyRadiance: 3 floats, NULL = none). Lights are chosen by power (PowerLightSampler.swift):
+ * the area lights in index order, then the sky. */
+int gz_gpu_set_mis_scene(void *handle, const struct GzMaterial *materials, int32_t materialCount,
+                         const struct GzAreaLight *lights, int32_t lightCount, const float *skyRadiance);
+
+/* One bounce (oneBounce + shadeBounce, VolumePathIntegrator.swift:373-445) of every active path:
+ * closest hit, emission, one light sample with MIS (shadow ray + BSDF ray), continuation and
+ * Russian roulette. Paths that stop clear GZ_PATH_ACTIVE. Asynchronous on `stream`. */
+int gz_gpu_mis_bounce_device(void *handle, struct GzPath *d_paths, int64_t count,
+                             const struct GzMisParams *params, void *stream);
+
+/* Runs bounces until every path has stopped (VolumePathIntegrator.bounces, :448-463).
+ * Asynchronous on `stream`. */
+int gz_gpu_mis_trace_device(void *handle, struct GzPath *d_paths, int64_t count,
+                            const struct GzMisParams *params, void *stream);
+
+/* ---------------------------------------------------------------------------------------- */
 /* Z-Sobol sampler (replaces the Swift ZSobolSampler, Sampler/ZSobolSampler.swift).            */
 /* ---------------------------------------------------------------------------------------- */
 
@@ -323,6 +404,10 @@ static_assert(offsetof(struct PathState_C, active) == 80, "PathState_C.active");
 static_assert(sizeof(struct BVH2Node) == 32, "BVH2Node layout");
 static_assert(sizeof(struct SceneDescriptor2_C) == 32, "SceneDescriptor2_C layout");
 static_assert(sizeof(struct GzHit) == 16, "GzHit layout");
+static_assert(sizeof(struct GzMaterial) == 64, "GzMaterial layout");
+static_assert(sizeof(struct GzAreaLight) == 24, "GzAreaLight layout");
+static_assert(sizeof(struct GzPath) == 112, "GzPath layout");
+static_assert(sizeof(struct GzMisParams) == 32, "GzMisParams layout");
 #endif
 
 #endif /* GONZALES_AMD_H */

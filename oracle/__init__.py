@@ -1,7 +1,8 @@
 """ORACLE — test infrastructure only.
 
 ctypes front end of ``oracle/liboracle.so``, the CPU restatement of the reference's hot path
-(Sources/mojoKernel/kernel.mojo traversal + shade, Sampler/ZSobolSampler.swift). Imported only by
+(Sources/mojoKernel/kernel.mojo traversal + shade, Sampler/ZSobolSampler.swift,
+Integrator/VolumePathIntegrator.swift MIS bounce). Imported only by
 tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg, always as the checker; the
 product (``gonzales_amd``) never imports it.
 """
@@ -43,6 +44,13 @@ _SIGS = {
     "orc_zsobol_batch": (None, [_I64, _I64, _I64, _I64, _P, _P, _P, _P, _I64, _I32, _P, _I32, _P]),
     "orc_pcg32": (None, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _P]),
     "orc_shade_batch": (None, [_P, _I64, _P, _P, _P]),
+    "orc_mis_trace_paths": (ctypes.c_int, [_P, _I64, _P, _I32, _P, _I32, _P, _P, _P, _I64, ctypes.c_int]),
+    "orc_power_heuristic": (ctypes.c_float, [ctypes.c_float, ctypes.c_float]),
+    "orc_concentric_disk": (None, [ctypes.c_float, ctypes.c_float, _P]),
+    "orc_cosine_hemisphere": (None, [ctypes.c_float, ctypes.c_float, _P]),
+    "orc_equal_area_square_to_sphere": (None, [ctypes.c_float, ctypes.c_float, ctypes.c_int, _P]),
+    "orc_rr_draw": (ctypes.c_float, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_int32]),
 }
 
 _lib = None
@@ -158,3 +166,49 @@ def pcg32(initstate: int, initseq: int, n: int) -> np.ndarray:
 def shade(paths: np.ndarray, isects: np.ndarray, meshes_c_addr: int, materials: np.ndarray) -> None:
     """mojo_cpu_shade_batch: updates `paths` (PathState_C records) in place."""
     lib().orc_shade_batch(_p(paths), len(paths), _p(isects), meshes_c_addr, _p(materials))
+
+
+# ---- MIS path integrator (mode B) ------------------------------------------------------------
+
+class MisParams(ctypes.Structure):
+    """mb_params of oracle_mis.c: the device GzMisParams plus the constant sky radiance."""
+    _fields_ = [("maxDepth", ctypes.c_int32), ("flags", ctypes.c_int32), ("zsSeed", ctypes.c_int64),
+                ("zsLog2Spp", ctypes.c_int32), ("zsBase4Digits", ctypes.c_int32), ("rrSeed", ctypes.c_uint64),
+                ("sky", ctypes.c_float * 3), ("hasSky", ctypes.c_int32)]
+
+
+def mis_trace(desc_ptr: int, node_count: int, materials: np.ndarray, lights: np.ndarray, params: MisParams,
+              paths: np.ndarray, nthreads: int = 1, table=None) -> None:
+    """Runs every active GzPath record to completion in place (VolumePathIntegrator bounces)."""
+    table = sobol_matrices() if table is None else table
+    rc = lib().orc_mis_trace_paths(desc_ptr, node_count, _p(materials), len(materials),
+                                   _p(lights) if len(lights) else None, len(lights), _p(table),
+                                   ctypes.byref(params), _p(paths), len(paths), nthreads)
+    if rc != 0:
+        raise RuntimeError("oracle MIS: bad arguments")
+
+
+def power_heuristic(f: float, g: float) -> float:
+    return lib().orc_power_heuristic(f, g)
+
+
+def concentric_disk(u0: float, u1: float) -> np.ndarray:
+    out = np.zeros(2, np.float32)
+    lib().orc_concentric_disk(u0, u1, _p(out))
+    return out
+
+
+def cosine_hemisphere(u0: float, u1: float) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib().orc_cosine_hemisphere(u0, u1, _p(out))
+    return out
+
+
+def equal_area_square_to_sphere(u: float, v: float, unit: bool = False) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib().orc_equal_area_square_to_sphere(u, v, int(unit), _p(out))
+    return out
+
+
+def rr_draw(seed: int, px: int, py: int, si: int, bounce: int) -> float:
+    return lib().orc_rr_draw(seed, px, py, si, bounce)

@@ -283,7 +283,7 @@ static int o_run(o_batch *b, int nthreads) {
 /* mojo_cpu_traverse_batch (kernel.mojo:378-390): closest hit for n rays; stats may be NULL. */
 int orc_traverse_batch(const o_scene *s, int64_t nodeCount, const o_ray *rays, const float *tMax,
                        int64_t n, o_isect *out, o_stats *stats, int nthreads) {
-        o_batch b = {s, nodeCount, rays, tMax, n, out, NULL, stats};
+        o_batch b = {s, nodeCount, rays, tMax, n, out, NULL, stats, 0, 0};
         return o_run(&b, nthreads);
 }
 
@@ -291,7 +291,7 @@ int orc_traverse_batch(const o_scene *s, int64_t nodeCount, const o_ray *rays, c
  * an early exit at the first accepted triangle; stats counted up to that exit. */
 int orc_occluded_batch(const o_scene *s, int64_t nodeCount, const o_ray *rays, const float *tMax,
                        int64_t n, uint8_t *occluded, o_stats *stats, int nthreads) {
-        o_batch b = {s, nodeCount, rays, tMax, n, NULL, occluded, stats};
+        o_batch b = {s, nodeCount, rays, tMax, n, NULL, occluded, stats, 0, 0};
         return o_run(&b, nthreads);
 }
 
