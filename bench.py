@@ -61,9 +61,107 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-path", action="store_true", help="skip the MIS path-tracing leg")
+    ap.add_argument("--path-res", type=int, default=512)
+    ap.add_argument("--path-spp", type=int, default=16)
+    ap.add_argument("--path-steps", type=int, default=5)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device copy (reference dispatch for HBM byte counters)")
     return ap.parse_args()
+
+
+PATH_RECORD_BYTES = 224  # GzPath read + write per bounce
+SHADING_FETCH_BYTES = 160  # PrimId 32 + mesh view 40 + 3 vertex indices 24 + 3 points 48 + material 16
+
+
+def path_leg(args, dev, stream, cores: int, run_cpu: bool) -> dict:
+    """Cornell box, 512 x 512 x 16 spp, max depth 5: the MIS integrator's path samples/s.
+
+    One step = reset the 4,194,304 path records from the camera batch resident in HBM (a device
+    copy) + every bounce of every path (shadow rays and MIS BSDF rays included). Algorithmic bytes
+    per path = sum over its bounces of (224 path record + 160 shading fetch) + 64 x interior visits
+    + 40 x triangle tests of all its rays, from the counting variant of the same kernel."""
+    import ctypes
+
+    import torch
+
+    from gonzales_amd import abi, scenes
+    from gonzales_amd.integrator import MisOptions, VolumePathIntegrator, new_paths, pinhole_rays
+
+    w = h = args.path_res
+    spp = args.path_spp
+    bh, cam = scenes.cornell_hierarchy()
+    bh.upload_to_gpu()
+    opt = MisOptions.for_image(w, h, spp, seed=0, max_depth=5)
+    integ = VolumePathIntegrator(bh, opt)
+    rays, px, py = pinhole_rays(cam, w, h)
+    host = abi.concat([new_paths(rays, px, py, s, start_dim=4) for s in range(spp)], abi.GZ_PATH)
+    n = len(host)
+    d_init = torch.from_numpy(host.view(np.uint8)).to(dev)
+    d_paths = torch.empty_like(d_init)
+    d_stats = torch.zeros(n * 3, dtype=torch.int32, device=dev)
+    d_paths.copy_(d_init)
+    integ.trace_stats(d_paths, d_stats)
+    bh.check_traversal()
+    st = d_stats.view(n, 3).to(torch.int64).sum(0).tolist()
+    algo = (PATH_RECORD_BYTES + SHADING_FETCH_BYTES) * st[2] + 64 * st[0] + 40 * st[1]
+
+    def step():
+        d_paths.copy_(d_init)
+        integ.trace(d_paths)
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    steps = args.path_steps
+    a.record(stream)
+    for _ in range(steps):
+        step()
+    b.record(stream)
+    torch.cuda.synchronize()
+    bh.check_traversal()
+    ms = a.elapsed_time(b) / steps
+    got = d_paths.cpu().numpy().view(abi.GZ_PATH)
+    img = got["estimate"].astype(np.float64).reshape(spp, h, w, 3).mean(0)
+    res = {"workload": f"Cornell box (34 triangles, 2 area-light triangles), {w}x{h}, {spp} spp, max depth 5, "
+                       "Z-Sobol seed 0, RR after bounce 1, diffuse",
+           "path_samples_per_s": round(n / (ms * 1e-3), 1), "ms_per_frame": round(ms, 3),
+           "bounces_per_path": round(st[2] / n, 3), "interior_per_path": round(st[0] / n, 2),
+           "tris_per_path": round(st[1] / n, 2), "bytes_per_path": round(algo / n, 1),
+           "roofline": {"bound": "hbm", "achieved": round(algo / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+           "image_mean_rgb": [round(float(x), 5) for x in img.reshape(-1, 3).mean(0)]}
+    log(f"path leg: {res['path_samples_per_s'] / 1e6:.2f} M path samples/s, {ms:.2f} ms/frame, "
+        f"{res['bounces_per_path']} bounces/path, {res['bytes_per_path']} B/path")
+    if run_cpu:
+        import oracle
+
+        desc, keep = bh.descriptor()
+        table = oracle.sobol_matrices()
+        prm = oracle.MisParams(opt.max_depth, 0, opt.zsobol_seed, opt.log2_spp, opt.base4_digits, opt.rr_seed,
+                               (ctypes.c_float * 3)(0, 0, 0), 0)
+        calib = host[:1 << 14].copy()
+        t1 = time.perf_counter()
+        oracle.mis_trace(ctypes.addressof(desc), len(bh.nodes), integ.materials, integ.lights, prm, calib, cores, table)
+        rate = len(calib) / (time.perf_counter() - t1)
+        m = int(min(n, max(1 << 14, rate * args.cpu_seconds / 2)))
+        sample = host[:m].copy()
+        t1 = time.perf_counter()
+        oracle.mis_trace(ctypes.addressof(desc), len(bh.nodes), integ.materials, integ.lights, prm, sample, cores,
+                         table)
+        dt = time.perf_counter() - t1
+        g = got[:m]
+        e_g, e_r = g["estimate"].astype(np.float64), sample["estimate"].astype(np.float64)
+        agree = ((np.abs(e_g - e_r) <= 1e-4 * np.abs(e_r) + 1e-6).all(1) & (g["bounce"] == sample["bounce"]))
+        res["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "path samples/s", "cores": cores, "kind": "port",
+                               "sample": f"first {m} of the {n} path samples (sample-major order), same scene, "
+                                         f"{cores} threads, {dt:.1f} s"}
+        res["cpu_sample_paths_agreeing"] = round(float(agree.mean()), 5)
+        log(f"path leg cpu oracle {m / dt:.0f} path samples/s on {cores} threads, agreeing {agree.mean():.5f}")
+    bh.free()
+    return res
 
 
 def main() -> int:
@@ -228,6 +326,10 @@ def main() -> int:
                "sample": f"first {m} of the {n} incoherent rays, same scene and BVH2, "
                          f"{cores} threads over 4096-ray chunks, {dt:.1f} s"}
         log(f"cpu oracle {cpu_rate:.3f} Mrays/s on {cores} threads ({m} rays, {dt:.1f}s), parity {ok}")
+
+    if not args.no_path:
+        pcores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        extra["path_trace"] = path_leg(args, dev, stream, max(1, min(pcores, 16)), not args.no_cpu)
 
     out = {
         "metric": METRIC,

@@ -48,6 +48,8 @@ GZ_MATERIAL = np.dtype({"names": ["type", "reflectance", "eta", "roughness", "co
                         "offsets": [0, 4, 16, 20, 24, 36, 48], "itemsize": 64})
 GZ_AREA_LIGHT = np.dtype({"names": ["brightness", "mesh", "triangle"], "formats": [("<f4", 3), "<i4", "<i8"],
                           "offsets": [0, 12, 16], "itemsize": 24})
+PATH_STATS = np.dtype({"names": ["interior", "tris", "bounces"], "formats": ["<u4", "<u4", "<u4"],
+                       "offsets": [0, 4, 8], "itemsize": 12})
 GZ_PATH_ACTIVE = 1
 GZ_PATH_SPECULAR = 2
 GZ_MAT_DIFFUSE = 1
@@ -129,6 +131,7 @@ SIGNATURES = {
     "gz_gpu_set_mis_scene": (ctypes.c_int, [_P, _P, _I32, _P, _I32, _P]),
     "gz_gpu_mis_bounce_device": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
     "gz_gpu_mis_trace_device": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
+    "gz_gpu_mis_trace_stats_device": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
 }
 
 _lib = None

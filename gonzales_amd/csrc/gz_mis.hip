@@ -107,18 +107,29 @@ struct MisArgs {
         int *err;
 };
 
+// Per-lane traversal state shared by the rays of one bounce: one stack (so the three inline
+// traversals do not each get their own private array) and the work counters (interior visits
+// and triangle tests, all rays).
+struct Work {
+        Stack<kLdsMax> S;
+        uint32_t interior = 0, tris = 0;
+};
+
 // Closest or any hit of one ray (the traversal of the batch kernels, inline).
-template <bool kAny>
+template <bool kAny, bool kStats>
 __device__ __forceinline__ bool trace_ray(const MisArgs &a, const Ray_C &ray, float tMax, int &prim, float &t,
-                                          float &u, float &v) {
+                                          float &u, float &v, Work &w) {
         if (a.empty) return false;
         Lane L;
         lane_init(L, ray, tMax, a.rootDesc, a.sceneSafe);
-        Stack<kLdsMax> S;
         int st;
         do {
-                st = step<kAny, false, kLdsMax>(L, S, a.nodes, a.tris);
+                st = step<kAny, kStats, kLdsMax>(L, w.S, a.nodes, a.tris);
         } while (st == kStepGo);
+        if (kStats) {
+                w.interior += L.nInterior;
+                w.tris += L.nTris;
+        }
         if (st == kStepOverflow) {
                 atomicOr(a.err, 1);
                 return false;
@@ -341,13 +352,14 @@ __device__ __forceinline__ void light_sample(const MisArgs &a, int choice, f3 po
 
 // Visibility.occluded via spawnRay(from:target:) (Visibility.swift:5-24, Interaction.swift:15-24):
 // origin from + 1e-4 (target - from), unnormalised direction, tMax 1 - 1e-5.
-__device__ __forceinline__ bool occluded(const MisArgs &a, f3 from, f3 target) {
+template <bool kStats>
+__device__ __forceinline__ bool occluded(const MisArgs &a, f3 from, f3 target, Work &w) {
         const f3 o = add(from, mul(sub(target, from), 0.0001f));
         const f3 d = sub(target, o);
         Ray_C r{o.x, o.y, o.z, d.x, d.y, d.z};
         int prim;
         float t, u, v;
-        return trace_ray<true>(a, r, 1.0f - 0.00001f, prim, t, u, v);
+        return trace_ray<true, kStats>(a, r, 1.0f - 0.00001f, prim, t, u, v, w);
 }
 
 // Interaction.spawnRay(inDirection:) (Interaction.swift:32-53)
@@ -398,7 +410,11 @@ __device__ __forceinline__ float rr_draw(uint64_t seed, int px, int py, int si, 
         return (float)(uint32_t)(z >> 40) * (1.0f / 16777216.0f);
 }
 
-__global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__restrict__ paths, int64_t n) {
+// One bounce of every active path. kStats: also adds the bounce's traversal work (all its rays)
+// to stats[i] and counts the bounce in stats[i] (interior, tris, bounces as a third word).
+template <bool kStats>
+__global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__restrict__ paths, int64_t n,
+                                                          uint32_t *__restrict__ stats) {
         const int64_t i = (int64_t)blockIdx.x * kMisBlock + threadIdx.x;
         if (i >= n) return;
         GzPath p = paths[i];
@@ -407,6 +423,7 @@ __global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__r
                 paths[i].flags = p.flags & ~GZ_PATH_ACTIVE;
                 return;
         }
+        Work w;
         f3 T = mk(p.throughput[0], p.throughput[1], p.throughput[2]);
         f3 E = mk(p.estimate[0], p.estimate[1], p.estimate[2]);
         bool specular = (p.flags & GZ_PATH_SPECULAR) != 0;
@@ -414,7 +431,7 @@ __global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__r
         int prim = -1;
         float th, hu, hv;
         Interaction si;
-        const bool hit = trace_ray<false>(a, p.ray, INFINITY, prim, th, hu, hv) &&
+        const bool hit = trace_ray<false, kStats>(a, p.ray, INFINITY, prim, th, hu, hv, w) &&
                          surface_interaction(a, p.ray, prim, hu, hv, si);
         ZSobolState zs;
         zs.morton = (a.zs.log2spp >= 64 ? 0ull : zs_morton2((uint32_t)p.pixelX, (uint32_t)p.pixelY) << a.zs.log2spp) |
@@ -451,7 +468,7 @@ __global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__r
                 light_sample(a, choice, si.position, u0, u1, ls);
                 f3 Lest = mk(0.0f, 0.0f, 0.0f), linc = mk(0.0f, 0.0f, 0.0f);
                 float lpdf = 0.0f;
-                if (!cblack(ls.L) && !isinf(ls.pdf) && !occluded(a, si.position, ls.target)) {
+                if (!cblack(ls.L) && !isinf(ls.pdf) && !occluded<kStats>(a, si.position, ls.target, w)) {
                         const f3 scatter = mul(bsdf_eval_world(b, si.outgoing, ls.dir), absdot(ls.dir, si.shadingNormal));
                         Lest = cmul(scatter, ls.L);
                         lpdf = ls.pdf;
@@ -475,7 +492,8 @@ __global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__r
                         Interaction bsi;
                         f3 bL = mk(0.0f, 0.0f, 0.0f);
                         bool ok = false;
-                        if (trace_ray<false>(a, r, INFINITY, bprim, bt, bu, bv) && surface_interaction(a, r, bprim, bu, bv, bsi)) {
+                        if (trace_ray<false, kStats>(a, r, INFINITY, bprim, bt, bu, bv, w) &&
+                            surface_interaction(a, r, bprim, bu, bv, bsi)) {
                                 if (bsi.areaLight >= 0 && bsi.areaLight < a.nLights) {
                                         const float4 Lb = a.lights[bsi.areaLight].L;
                                         bL = dot(bsi.normal, bwi) > 0.0f ? mk(Lb.x, Lb.y, Lb.z) : mk(0.0f, 0.0f, 0.0f);
@@ -527,6 +545,11 @@ __global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__r
         p.flags = (go ? GZ_PATH_ACTIVE : 0) | (specular ? GZ_PATH_SPECULAR : 0);
         if (go) p.bounce += 1;
         paths[i] = p;
+        if (kStats) {
+                stats[3 * i] += w.interior;
+                stats[3 * i + 1] += w.tris;
+                stats[3 * i + 2] += 1;
+        }
 }
 
 // Light and material tables of one scene handle.
@@ -711,8 +734,8 @@ static bool mis_args(Scene *s, const GzMisParams *params, MisArgs &a) {
         return true;
 }
 
-static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisParams *params, void *stream,
-                      bool toEnd, const char *who) {
+static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisParams *params,
+                      GzPathStats *d_stats, void *stream, bool toEnd, const char *who) {
         clear_error();
         if (!handle || count < 0 || (count > 0 && !d_paths)) {
                 set_error("%s: bad arguments", who);
@@ -732,9 +755,15 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
         // A path makes at most maxDepth + 1 bounces (bounce 0 .. maxDepth); a launch after the
         // last finds every path inactive.
         const int launches = toEnd ? params->maxDepth + 1 : 1;
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        uint32_t *stats = reinterpret_cast<uint32_t *>(d_stats);
         for (int b = 0; b < launches; ++b) {
-                hipLaunchKernelGGL(k_mis_bounce, dim3((unsigned)blocks), dim3(kMisBlock), 0,
-                                   static_cast<hipStream_t>(stream), a, d_paths, count);
+                if (d_stats)
+                        hipLaunchKernelGGL(k_mis_bounce<true>, dim3((unsigned)blocks), dim3(kMisBlock), 0, st, a, d_paths,
+                                           count, stats);
+                else
+                        hipLaunchKernelGGL(k_mis_bounce<false>, dim3((unsigned)blocks), dim3(kMisBlock), 0, st, a,
+                                           d_paths, count, stats);
                 if (!hip_ok(hipGetLastError(), "MIS bounce launch")) return GZ_E_HIP;
         }
         return GZ_OK;
@@ -742,10 +771,19 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
 
 extern "C" int gz_gpu_mis_bounce_device(void *handle, GzPath *d_paths, int64_t count, const GzMisParams *params,
                                         void *stream) {
-        return mis_launch(handle, d_paths, count, params, stream, false, "gz_gpu_mis_bounce_device");
+        return mis_launch(handle, d_paths, count, params, nullptr, stream, false, "gz_gpu_mis_bounce_device");
 }
 
 extern "C" int gz_gpu_mis_trace_device(void *handle, GzPath *d_paths, int64_t count, const GzMisParams *params,
                                        void *stream) {
-        return mis_launch(handle, d_paths, count, params, stream, true, "gz_gpu_mis_trace_device");
+        return mis_launch(handle, d_paths, count, params, nullptr, stream, true, "gz_gpu_mis_trace_device");
+}
+
+extern "C" int gz_gpu_mis_trace_stats_device(void *handle, GzPath *d_paths, int64_t count,
+                                             const GzMisParams *params, GzPathStats *d_stats, void *stream) {
+        if (count > 0 && !d_stats) {
+                set_error("gz_gpu_mis_trace_stats_device: null stats array");
+                return GZ_E_INVALID;
+        }
+        return mis_launch(handle, d_paths, count, params, d_stats, stream, true, "gz_gpu_mis_trace_stats_device");
 }

@@ -123,6 +123,15 @@ class VolumePathIntegrator:
                                                       ctypes.byref(self._params), abi.stream_ptr(stream)),
                   "gz_gpu_mis_trace_device")
 
+    def trace_stats(self, paths_t, stats_t, count: int | None = None, stream=None) -> None:
+        """``trace`` that also adds each path's traversal work to stats_t (abi.PATH_STATS records,
+        zeroed by the caller)."""
+        n = int(paths_t.numel() * paths_t.element_size() // abi.GZ_PATH.itemsize) if count is None else count
+        abi.check(abi.load().gz_gpu_mis_trace_stats_device(self.bh.handle, abi.ptr(paths_t), n,
+                                                            ctypes.byref(self._params), abi.ptr(stats_t),
+                                                            abi.stream_ptr(stream)),
+                  "gz_gpu_mis_trace_stats_device")
+
 
 def pinhole_rays(camera: dict, width: int, height: int, jitter: np.ndarray | None = None) -> tuple:
     """Pinhole camera rays through pixel centres (or jittered positions), one per pixel, row-major.

@@ -337,6 +337,19 @@ int gz_gpu_mis_bounce_device(void *handle, struct GzPath *d_paths, int64_t count
 int gz_gpu_mis_trace_device(void *handle, struct GzPath *d_paths, int64_t count,
                             const struct GzMisParams *params, void *stream);
 
+/* Traversal work of one path, summed over its bounces and all rays of each bounce (closest hit,
+ * shadow ray, BSDF ray): interior-node visits, triangle tests, bounces run. */
+struct GzPathStats {
+        uint32_t interior;
+        uint32_t tris;
+        uint32_t bounces;
+};
+
+/* gz_gpu_mis_trace_device that also adds each path's work to d_stats[i] (zero it first). For
+ * measurement: the counting costs a little time. No counterpart in the reference. */
+int gz_gpu_mis_trace_stats_device(void *handle, struct GzPath *d_paths, int64_t count,
+                                  const struct GzMisParams *params, struct GzPathStats *d_stats, void *stream);
+
 /* ---------------------------------------------------------------------------------------- */
 /* Z-Sobol sampler (replaces the Swift ZSobolSampler, Sampler/ZSobolSampler.swift).            */
 /* ---------------------------------------------------------------------------------------- */
@@ -408,6 +421,7 @@ static_assert(sizeof(struct GzMaterial) == 64, "GzMaterial layout");
 static_assert(sizeof(struct GzAreaLight) == 24, "GzAreaLight layout");
 static_assert(sizeof(struct GzPath) == 112, "GzPath layout");
 static_assert(sizeof(struct GzMisParams) == 32, "GzMisParams layout");
+static_assert(sizeof(struct GzPathStats) == 12, "GzPathStats layout");
 #endif
 
 #endif /* GONZALES_AMD_H */

@@ -119,6 +119,27 @@ def test_bounce_steps_equal_trace(gpu):
     assert ga.tobytes() == gb.tobytes()
 
 
+def test_trace_stats_counts_work(gpu):
+    import torch
+    bh, cam = scenes.cornell_hierarchy()
+    bh.upload_to_gpu()
+    rays, px, py = pinhole_rays(cam, 32, 32)
+    paths = new_paths(rays, px, py, 2, start_dim=4)
+    opt = MisOptions.for_image(32, 32, 4, max_depth=5)
+    integ = VolumePathIntegrator(bh, opt)
+    a, b = _to_device(paths), _to_device(paths)
+    stats = torch.zeros(len(paths) * 3, dtype=torch.int32, device="cuda")
+    integ.trace(a)
+    integ.trace_stats(b, stats)
+    ga, gb = _to_host(a), _to_host(b)
+    assert ga.tobytes() == gb.tobytes()
+    st = stats.cpu().numpy().view(abi.PATH_STATS)
+    assert np.array_equal(st["bounces"], gb["bounce"] + 1)
+    # every bounce traces at least its closest-hit ray from the root of a 34-triangle BVH
+    assert np.all(st["interior"] >= st["bounces"])
+    assert st["tris"].sum() > 0
+
+
 def test_inactive_paths_are_untouched(gpu):
     bh, cam = scenes.cornell_hierarchy()
     bh.upload_to_gpu()
