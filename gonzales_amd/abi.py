@@ -42,10 +42,12 @@ GZ_PATH = np.dtype({"names": ["ray", "throughput", "estimate", "albedo", "firstN
                     "formats": [RAY, ("<f4", 3), ("<f4", 3), ("<f4", 3), ("<f4", 3), "<f4", "<i4", "<i4", "<i4",
                                 "<i4", "<i4", "<i4", "<i4"],
                     "offsets": [0, 24, 36, 48, 60, 72, 76, 80, 84, 88, 92, 96, 100], "itemsize": 112})
-GZ_MATERIAL = np.dtype({"names": ["type", "reflectance", "eta", "roughness", "conductorEta", "conductorK",
-                                  "remapRoughness"],
-                        "formats": ["<i4", ("<f4", 3), "<f4", "<f4", ("<f4", 3), ("<f4", 3), "<i4"],
-                        "offsets": [0, 4, 16, 20, 24, 36, 48], "itemsize": 64})
+GZ_MATERIAL = np.dtype({"names": ["type", "flags", "reflectance", "eta", "roughness", "conductorRoughness",
+                                  "conductorEta", "conductorK", "thickness", "albedo", "g", "maxDepth",
+                                  "nSamples"],
+                        "formats": ["<i4", "<i4", ("<f4", 3), "<f4", ("<f4", 2), ("<f4", 2), ("<f4", 3),
+                                    ("<f4", 3), "<f4", ("<f4", 3), "<f4", "<i4", "<i4"],
+                        "offsets": [0, 4, 8, 20, 24, 32, 40, 52, 64, 68, 80, 84, 88], "itemsize": 96})
 GZ_AREA_LIGHT = np.dtype({"names": ["brightness", "mesh", "triangle"], "formats": [("<f4", 3), "<i4", "<i8"],
                           "offsets": [0, 12, 16], "itemsize": 24})
 PATH_STATS = np.dtype({"names": ["interior", "tris", "bounces"], "formats": ["<u4", "<u4", "<u4"],
@@ -53,6 +55,10 @@ PATH_STATS = np.dtype({"names": ["interior", "tris", "bounces"], "formats": ["<u
 GZ_PATH_ACTIVE = 1
 GZ_PATH_SPECULAR = 2
 GZ_MAT_DIFFUSE = 1
+GZ_MAT_CONDUCTOR = 2
+GZ_MAT_DIELECTRIC = 3
+GZ_MAT_COATED_CONDUCTOR = 4
+GZ_MAT_REMAP_ROUGHNESS = 1
 GZ_MIS_FIX_LIGHT_PDF = 1
 GZ_MIS_FIX_SKY_DIRECTION = 2
 

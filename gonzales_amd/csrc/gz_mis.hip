@@ -23,6 +23,7 @@
 #include <cstring>
 #include <vector>
 
+#include "bsdf_device.h"
 #include "traverse_device.h"
 #include "zsobol_device.h"
 
@@ -33,44 +34,6 @@ namespace gz {
 const uint32_t *device_sobol_table();  // gz_zsobol.hip
 
 constexpr int kMisBlock = 256;
-constexpr float kPi = 3.14159265358979323846f;
-
-struct f3 {
-        float x, y, z;
-};
-__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
-__device__ __forceinline__ f3 add(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ f3 sub(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ f3 mul(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ f3 divs(f3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
-__device__ __forceinline__ f3 neg(f3 a) { return mk(-a.x, -a.y, -a.z); }
-__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float absdot(f3 a, f3 b) { return fabsf(dot(a, b)); }
-__device__ __forceinline__ f3 cross(f3 a, f3 b) {
-        return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-__device__ __forceinline__ float len2(f3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
-__device__ __forceinline__ f3 normalized(f3 a) {  // Three.swift:108-117: v / |v|, or v if |v| = 0
-        const float l = sqrtf(len2(a));
-        return l == 0.0f ? a : divs(a, l);
-}
-// Geometry.swift:63-77
-__device__ __forceinline__ void coordinate_system(f3 v1, f3 &v2, f3 &v3) {
-        if (fabsf(v1.x) > fabsf(v1.y))
-                v2 = divs(mk(-v1.z, 0.0f, v1.x), sqrtf(v1.x * v1.x + v1.z * v1.z));
-        else
-                v2 = divs(mk(0.0f, v1.z, -v1.y), sqrtf(v1.y * v1.y + v1.z * v1.z));
-        v3 = cross(v1, v2);
-}
-
-__device__ __forceinline__ f3 cadd(f3 a, f3 b) { return add(a, b); }
-__device__ __forceinline__ f3 cmul(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
-__device__ __forceinline__ bool cblack(f3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
-__device__ __forceinline__ float cmax(f3 a) {  // max(r, max(g, b)), Swift max(x, y) = y >= x ? y : x
-        const float gb = a.z >= a.y ? a.z : a.y;
-        return gb >= a.x ? gb : a.x;
-}
-
 // Scene data of the MIS integrator on the device.
 struct DevLight {
         float4 p0;  // w: area
@@ -211,12 +174,11 @@ __device__ bool surface_interaction(const MisArgs &a, const Ray_C &ray, int prim
         return true;
 }
 
-// Diffuse BSDF in its shading frame (BsdfFrame.swift:96-104, ShadingFrame.swift:27-77,
-// DiffuseBsdf.swift:15-21, Bsdf.swift:123-151, FramedBsdf.swift:18-83).
+// The material's BSDF in its shading frame (BsdfFrame.swift:96-104, ShadingFrame.swift:27-77,
+// FramedBsdf.swift:18-83); the BSDFs themselves are in bsdf_device.h.
 struct Bsdf {
         f3 x, y, z, ng;
-        f3 R;
-        int type;
+        Bx b;
 };
 
 __device__ __forceinline__ void bsdf_make(const MisArgs &a, const Interaction &si, Bsdf &b) {
@@ -225,54 +187,36 @@ __device__ __forceinline__ void bsdf_make(const MisArgs &a, const Interaction &s
         b.x = cross(b.y, b.z);
         b.ng = si.normal;
         const int mi = si.material >= 0 && si.material < a.nMats ? si.material : 0;
-        const GzMaterial m = a.mats[mi];
-        b.type = m.type;
-        b.R = mk(m.reflectance[0], m.reflectance[1], m.reflectance[2]);
+        b.b = bx_from_material(a.mats[mi]);
 }
 __device__ __forceinline__ f3 to_local(const Bsdf &b, f3 w) { return normalized(mk(dot(w, b.x), dot(w, b.y), dot(w, b.z))); }
 __device__ __forceinline__ f3 to_world(const Bsdf &b, f3 l) { return add(add(mul(b.x, l.x), mul(b.y, l.y)), mul(b.z, l.z)); }
-__device__ __forceinline__ f3 bsdf_f(const Bsdf &b) { return divs(b.R, kPi); }
-__device__ __forceinline__ f3 bsdf_eval_world(const Bsdf &b, f3 wo, f3 wi) {
+// evaluateWorldSpace: every BSDF keeps the protocol's isReflective = true and
+// isTransmissive = false, so only the reflection side is evaluated.
+template <bool kAll>
+__device__ __forceinline__ f3 bsdf_eval_world(const Bsdf &b, f3 wo, f3 wi, LayerRng &r) {
+        const f3 ol = to_local(b, wo), il = to_local(b, wi);
         const bool reflect = dot(wi, b.ng) * dot(wo, b.ng) > 0.0f;
-        return reflect ? bsdf_f(b) : mk(0.0f, 0.0f, 0.0f);
+        return reflect ? bx_f<kAll>(b.b, ol, il, r) : mk(0.0f, 0.0f, 0.0f);
 }
-__device__ __forceinline__ float bsdf_pdf_world(const Bsdf &b, f3 wo, f3 wi) {
+template <bool kAll>
+__device__ __forceinline__ float bsdf_pdf_world(const Bsdf &b, f3 wo, f3 wi, LayerRng &r) {
         const f3 il = to_local(b, wi), ol = to_local(b, wo);
         if (ol.z == 0.0f) return 0.0f;
-        if (!(ol.z * il.z > 0.0f)) return 0.0f;
-        return fabsf(il.z) / kPi;
+        return bx_pdf<kAll>(b.b, ol, il, r);
 }
-// Sampling.swift:3-25
-__device__ __forceinline__ f3 cosine_hemisphere(float u0, float u1) {
-        const float ox = 2.0f * u0 - 1.0f, oy = 2.0f * u1 - 1.0f;
-        float x = 0.0f, y = 0.0f;
-        if (!(ox == 0.0f && oy == 0.0f)) {
-                float theta, r;
-                if (fabsf(ox) > fabsf(oy)) {
-                        r = ox;
-                        theta = (kPi / 4.0f) * (oy / ox);
-                } else {
-                        r = oy;
-                        theta = (kPi / 2.0f) - (kPi / 4.0f) * (ox / oy);
-                }
-                x = r * cosf(theta);
-                y = r * sinf(theta);
-        }
-        const float z2 = 1.0f - x * x - y * y;
-        return mk(x, y, sqrtf(z2 > 0.0f ? z2 : 0.0f));
-}
-__device__ __forceinline__ void bsdf_sample_world(const Bsdf &b, f3 wo, float u0, float u1, f3 &f, f3 &wi, float &pdf) {
+template <bool kAll>
+__device__ __forceinline__ void bsdf_sample_world(const Bsdf &b, f3 wo, float u0, float u1, float u2, f3 &f, f3 &wi,
+                                                  float &pdf, LayerRng &r) {
         const f3 ol = to_local(b, wo);
-        f3 il = cosine_hemisphere(u0, u1);
-        if (ol.z < 0.0f) il.z = -il.z;
-        pdf = (ol.z * il.z > 0.0f) ? fabsf(il.z) / kPi : 0.0f;
-        f = bsdf_f(b);
-        wi = to_world(b, il);
+        const BSample s = bx_sample<kAll>(b.b, ol, u0, u1, u2, r);
+        f = s.f;
+        wi = to_world(b, s.wi);
+        pdf = s.pdf;
 }
-__device__ __forceinline__ float power_heuristic(float f, float g) {
-        if (f == 0.0f || g == 0.0f) return 0.0f;
-        return (f * f) / (f * f + g * g);
-}
+
+// Streams of the layered BSDF for the BSDF calls of one bounce.
+enum : int { kCallLightF = 1, kCallLightPdf = 2, kCallMisSample = 3, kCallContinue = 4 };
 
 // PowerLightSampler.chooseLight + lowerBound (PowerLightSampler.swift:4-49)
 __device__ __forceinline__ int choose_light(const MisArgs &a, float u, float &pdf) {
@@ -412,7 +356,7 @@ __device__ __forceinline__ float rr_draw(uint64_t seed, int px, int py, int si, 
 
 // One bounce of every active path. kStats: also adds the bounce's traversal work (all its rays)
 // to stats[i] and counts the bounce in stats[i] (interior, tris, bounces as a third word).
-template <bool kStats>
+template <bool kStats, bool kAll>
 __global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__restrict__ paths, int64_t n,
                                                           uint32_t *__restrict__ stats) {
         const int64_t i = (int64_t)blockIdx.x * kMisBlock + threadIdx.x;
@@ -451,9 +395,10 @@ __global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__r
                 Bsdf b;
                 bsdf_make(a, si, b);
                 if (p.bounce == 0) {
-                        p.albedo[0] = b.R.x;
-                        p.albedo[1] = b.R.y;
-                        p.albedo[2] = b.R.z;
+                        const f3 al = bx_albedo<kAll>(b.b);
+                        p.albedo[0] = al.x;
+                        p.albedo[1] = al.y;
+                        p.albedo[2] = al.z;
                         p.firstNormal[0] = si.normal.x;
                         p.firstNormal[1] = si.normal.y;
                         p.firstNormal[2] = si.normal.z;
@@ -469,19 +414,23 @@ __global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__r
                 f3 Lest = mk(0.0f, 0.0f, 0.0f), linc = mk(0.0f, 0.0f, 0.0f);
                 float lpdf = 0.0f;
                 if (!cblack(ls.L) && !isinf(ls.pdf) && !occluded<kStats>(a, si.position, ls.target, w)) {
-                        const f3 scatter = mul(bsdf_eval_world(b, si.outgoing, ls.dir), absdot(ls.dir, si.shadingNormal));
+                        LayerRng r1 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightF);
+                        const f3 scatter =
+                                mul(bsdf_eval_world<kAll>(b, si.outgoing, ls.dir, r1), absdot(ls.dir, si.shadingNormal));
                         Lest = cmul(scatter, ls.L);
                         lpdf = ls.pdf;
                         linc = ls.dir;
                 }
-                const float bpdfL = bsdf_pdf_world(b, si.outgoing, linc);
+                LayerRng r2 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightPdf);
+                const float bpdfL = bsdf_pdf_world<kAll>(b, si.outgoing, linc, r2);
                 const float wl = power_heuristic(lpdf, bpdfL);
                 const f3 lc = lpdf == 0.0f ? mk(0.0f, 0.0f, 0.0f) : divs(mul(Lest, wl), lpdf);
                 float v0, v1, v2;
                 zs_get3d(a.zs, zs, a.sobol, v0, v1, v2);
                 f3 bf, bwi;
                 float bpdf;
-                bsdf_sample_world(b, si.outgoing, v0, v1, bf, bwi, bpdf);
+                LayerRng r3 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallMisSample);
+                bsdf_sample_world<kAll>(b, si.outgoing, v0, v1, v2, bf, bwi, bpdf, r3);
                 bf = mul(bf, absdot(bwi, si.shadingNormal));
                 f3 bc = mk(0.0f, 0.0f, 0.0f);
                 const bool weighted = choice >= a.nLights || (a.flags & GZ_MIS_FIX_LIGHT_PDF);
@@ -517,9 +466,10 @@ __global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__r
                 zs_get3d(a.zs, zs, a.sobol, w0, w1, w2);
                 f3 f, wi;
                 float pdf;
-                bsdf_sample_world(b, si.outgoing, w0, w1, f, wi, pdf);
+                LayerRng r4 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallContinue);
+                bsdf_sample_world<kAll>(b, si.outgoing, w0, w1, w2, f, wi, pdf, r4);
                 if (pdf != 0.0f && !isnan(pdf)) {
-                        specular = false;
+                        specular = bx_specular<kAll>(b.b);
                         T = cmul(T, divs(mul(f, absdot(wi, si.normal)), pdf));
                         p.ray = spawn(si, wi);
                         go = true;
@@ -564,6 +514,7 @@ struct MisScene {
         float totalPower = 0.0f;
         bool hasSky = false;
         float sky[3] = {0, 0, 0};
+        bool allDiffuse = true;  // picks the kernel built with the diffuse BSDF only
 };
 
 void release_mis(MisScene *m) {
@@ -650,6 +601,8 @@ static int set_mis_scene(Scene *s, const GzMaterial *materials, int32_t material
         m->nChoices = (int)power.size();
         m->totalPower = total;
         m->hasSky = skyRadiance != nullptr;
+        for (int32_t i = 0; i < materialCount; ++i)
+                if (materials[i].type != GZ_MAT_DIFFUSE) m->allDiffuse = false;
         if (skyRadiance)
                 for (int k = 0; k < 3; ++k) m->sky[k] = skyRadiance[k];
         const bool ok = put(&m->dMats, materials, (size_t)materialCount, "upload materials") &&
@@ -681,12 +634,20 @@ extern "C" int gz_gpu_set_mis_scene(void *handle, const GzMaterial *materials, i
                 set_error("gz_gpu_set_mis_scene: the scene has no light");
                 return GZ_E_INVALID;
         }
-        for (int32_t i = 0; i < materialCount; ++i)
-                if (materials[i].type != GZ_MAT_DIFFUSE) {
-                        set_error("gz_gpu_set_mis_scene: material %d has type %d; only diffuse (%d) is supported", i,
-                                  materials[i].type, GZ_MAT_DIFFUSE);
+        for (int32_t i = 0; i < materialCount; ++i) {
+                const GzMaterial &m = materials[i];
+                if (m.type < GZ_MAT_DIFFUSE || m.type > GZ_MAT_COATED_CONDUCTOR) {
+                        set_error("gz_gpu_set_mis_scene: material %d has unknown type %d", i, m.type);
                         return GZ_E_INVALID;
                 }
+                // The layered BSDF's loops run nSamples x maxDepth steps per call.
+                if (m.type == GZ_MAT_COATED_CONDUCTOR &&
+                    (m.nSamples < 1 || m.nSamples > 64 || m.maxDepth < 0 || m.maxDepth > 256)) {
+                        set_error("gz_gpu_set_mis_scene: coated conductor %d needs 1 <= nSamples <= 64 and "
+                                  "0 <= maxDepth <= 256 (got %d, %d)", i, m.nSamples, m.maxDepth);
+                        return GZ_E_INVALID;
+                }
+        }
         Scene *s = static_cast<Scene *>(handle);
         std::lock_guard<std::mutex> lock(s->mu);
         if (!enter_device(s)) return GZ_E_HIP;
@@ -757,13 +718,11 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
         const int launches = toEnd ? params->maxDepth + 1 : 1;
         const hipStream_t st = static_cast<hipStream_t>(stream);
         uint32_t *stats = reinterpret_cast<uint32_t *>(d_stats);
+        const bool all = !s->mis->allDiffuse;
+        auto kernel = d_stats ? (all ? k_mis_bounce<true, true> : k_mis_bounce<true, false>)
+                              : (all ? k_mis_bounce<false, true> : k_mis_bounce<false, false>);
         for (int b = 0; b < launches; ++b) {
-                if (d_stats)
-                        hipLaunchKernelGGL(k_mis_bounce<true>, dim3((unsigned)blocks), dim3(kMisBlock), 0, st, a, d_paths,
-                                           count, stats);
-                else
-                        hipLaunchKernelGGL(k_mis_bounce<false>, dim3((unsigned)blocks), dim3(kMisBlock), 0, st, a,
-                                           d_paths, count, stats);
+                hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(kMisBlock), 0, st, a, d_paths, count, stats);
                 if (!hip_ok(hipGetLastError(), "MIS bounce launch")) return GZ_E_HIP;
         }
         return GZ_OK;

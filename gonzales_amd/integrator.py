@@ -17,6 +17,65 @@ from . import abi
 from .accelerator import PRIM_AREA_LIGHT, BoundingHierarchy
 
 
+# The reference's default conductor, namedSpectra "metal-Cu-eta" / "metal-Cu-k" reduced to RGB by
+# PiecewiseLinearSpectrum.asRgb (values at 630, 532 and 465 nm; Core/Spectrum.swift:23-28, 81-160).
+CU_ETA = (0.230813, 1.04, 1.155312)
+CU_K = (3.458187, 2.583, 2.469187)
+
+
+def _pair(r) -> tuple:
+    return (float(r), float(r)) if np.isscalar(r) else (float(r[0]), float(r[1]))
+
+
+def diffuse_material(reflectance) -> np.ndarray:
+    m = np.zeros(1, abi.GZ_MATERIAL)
+    m["type"] = abi.GZ_MAT_DIFFUSE
+    m["reflectance"] = reflectance
+    return m
+
+
+def conductor_material(eta=CU_ETA, k=CU_K, roughness=0.5) -> np.ndarray:
+    """Conductor.create defaults (Conductor.swift:25-40): Cu, roughness 0.5, not remapped."""
+    m = np.zeros(1, abi.GZ_MATERIAL)
+    m["type"] = abi.GZ_MAT_CONDUCTOR
+    m["reflectance"] = 1.0
+    m["roughness"] = _pair(roughness)
+    m["conductorEta"] = eta
+    m["conductorK"] = k
+    return m
+
+
+def dielectric_material(eta=1.5, roughness=0.0, remap=True) -> np.ndarray:
+    """Dielectric.create defaults (Dielectric.swift:61-78): eta 1.5, roughness 0, remapped."""
+    m = np.zeros(1, abi.GZ_MATERIAL)
+    m["type"] = abi.GZ_MAT_DIELECTRIC
+    m["flags"] = abi.GZ_MAT_REMAP_ROUGHNESS if remap else 0
+    m["eta"] = eta
+    m["roughness"] = _pair(roughness)
+    return m
+
+
+def coated_conductor_material(reflectance=(1.0, 1.0, 1.0), interface_roughness=0.5, conductor_roughness=0.5,
+                              eta=1.5, thickness=0.01, albedo=(0.0, 0.0, 0.0), g=0.0, conductor_eta=CU_ETA,
+                              conductor_k=CU_K, max_depth=10, n_samples=1, remap=True) -> np.ndarray:
+    """CoatedConductor.create defaults (CoatedConductor.swift:91-153)."""
+    m = np.zeros(1, abi.GZ_MATERIAL)
+    m["type"] = abi.GZ_MAT_COATED_CONDUCTOR
+    m["flags"] = abi.GZ_MAT_REMAP_ROUGHNESS if remap else 0
+    m["reflectance"] = reflectance
+    m["eta"] = eta
+    m["roughness"] = _pair(interface_roughness)
+    m["conductorRoughness"] = _pair(conductor_roughness)
+    m["conductorEta"] = conductor_eta
+    m["conductorK"] = conductor_k
+    m["thickness"] = thickness
+    m["albedo"] = albedo
+    m["g"] = g
+    m["maxDepth"] = max_depth
+    m["nSamples"] = n_samples
+    return m
+
+
 def materials_from_scene(bh: BoundingHierarchy) -> np.ndarray:
     """GzMaterial table for the geometric primitives' material indices (diffuse only, as the
     MIS kernel supports today). Area-light surfaces shade with entry 0, as the reference does."""

@@ -260,18 +260,33 @@ int gz_gpu_shade_device(void *handle, struct PathState_C *d_paths, int64_t count
 /* MIS path tracing: the bounce of VolumePathIntegrator (Integrator/VolumePathIntegrator.swift) */
 /* ---------------------------------------------------------------------------------------- */
 
-/* Material parameters the MIS integrator shades with (the reference's Material/ types, resolved
- * on the host: constant textures only). type: GZ_MAT_DIFFUSE (Diffuse.swift:138-157). */
+/* Material parameters the MIS integrator shades with (the reference's Material/ types with
+ * constant textures), read by the kernel as each type's getBsdf does:
+ *   GZ_MAT_DIFFUSE           reflectance (Diffuse.swift)
+ *   GZ_MAT_CONDUCTOR         conductorEta, conductorK, roughness (alpha = max(r, 1e-3), never
+ *                            remapped; Conductor.swift:3-17); reflectance is white
+ *   GZ_MAT_DIELECTRIC        eta, roughness (alpha = sqrt(r) when remapped; Dielectric.swift:44-54)
+ *   GZ_MAT_COATED_CONDUCTOR  eta and roughness of the coating, conductorRoughness, conductorEta,
+ *                            conductorK, reflectance, thickness, albedo, g, maxDepth, nSamples
+ *                            (alphas remapped when asked, then max(., 1e-3); CoatedConductor.swift)
+ * flags: GZ_MAT_REMAP_ROUGHNESS. */
 struct GzMaterial {
         int32_t type;
+        int32_t flags;
         float reflectance[3];
         float eta;
-        float roughness;
+        float roughness[2];
+        float conductorRoughness[2];
         float conductorEta[3];
         float conductorK[3];
-        int32_t remapRoughness;
-        int32_t reserved[3];
+        float thickness;
+        float albedo[3];
+        float g;
+        int32_t maxDepth;
+        int32_t nSamples;
+        int32_t reserved;
 };
+enum { GZ_MAT_REMAP_ROUGHNESS = 1 };
 
 /* One triangle area light (AreaLight.swift:3-49): constant radiance on triangle `triangle` of
  * mesh `mesh`; its index is the id1 of its type-3 PrimId. */
@@ -308,7 +323,7 @@ struct GzMisParams {
 };
 
 enum { GZ_PATH_ACTIVE = 1, GZ_PATH_SPECULAR = 2 };
-enum { GZ_MAT_DIFFUSE = 1 };
+enum { GZ_MAT_DIFFUSE = 1, GZ_MAT_CONDUCTOR = 2, GZ_MAT_DIELECTRIC = 3, GZ_MAT_COATED_CONDUCTOR = 4 };
 /* Off (the default) reproduces the reference: the light pdf of a BSDF-sampled direction toward a
  * triangle light is 0 (Sampleable.swift:57-58 passes tHit = 0), so BSDF samples of area lights
  * get MIS weight 0. On, the pdf is computed from the chosen triangle.
@@ -417,7 +432,7 @@ static_assert(offsetof(struct PathState_C, active) == 80, "PathState_C.active");
 static_assert(sizeof(struct BVH2Node) == 32, "BVH2Node layout");
 static_assert(sizeof(struct SceneDescriptor2_C) == 32, "SceneDescriptor2_C layout");
 static_assert(sizeof(struct GzHit) == 16, "GzHit layout");
-static_assert(sizeof(struct GzMaterial) == 64, "GzMaterial layout");
+static_assert(sizeof(struct GzMaterial) == 96, "GzMaterial layout");
 static_assert(sizeof(struct GzAreaLight) == 24, "GzAreaLight layout");
 static_assert(sizeof(struct GzPath) == 112, "GzPath layout");
 static_assert(sizeof(struct GzMisParams) == 32, "GzMisParams layout");

@@ -51,6 +51,16 @@ _SIGS = {
     "orc_equal_area_square_to_sphere": (None, [ctypes.c_float, ctypes.c_float, ctypes.c_int, _P]),
     "orc_rr_draw": (ctypes.c_float, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                      ctypes.c_int32]),
+    "orc_fresnel_dielectric": (ctypes.c_float, [ctypes.c_float, ctypes.c_float]),
+    "orc_fresnel_conductor": (None, [ctypes.c_float, _P, _P, _P]),
+    "orc_refract": (ctypes.c_int, [_P, _P, ctypes.c_float, _P]),
+    "orc_tr_D": (ctypes.c_float, [ctypes.c_float, ctypes.c_float, _P]),
+    "orc_tr_G1": (ctypes.c_float, [ctypes.c_float, ctypes.c_float, _P]),
+    "orc_tr_pdf": (ctypes.c_float, [ctypes.c_float, ctypes.c_float, _P, _P]),
+    "orc_tr_sample_h": (None, [ctypes.c_float, ctypes.c_float, _P, ctypes.c_float, ctypes.c_float, _P]),
+    "orc_bx_f": (None, [_P, _P, _P, ctypes.c_uint64, _P]),
+    "orc_bx_pdf": (ctypes.c_float, [_P, _P, _P, ctypes.c_uint64]),
+    "orc_bx_sample": (None, [_P, _P, _P, ctypes.c_uint64, _P]),
 }
 
 _lib = None
@@ -105,7 +115,9 @@ def traverse(desc_ptr: int, node_count: int, rays: np.ndarray, t_max: np.ndarray
     n = len(rays)
     out = np.zeros(n, isect_dtype)
     st = np.zeros(n, STATS) if with_stats else None
-    rc = lib().orc_traverse_batch(desc_ptr, node_count, _p(rays), _p(np.ascontiguousarray(t_max, np.float32)), n,
+    rays = np.ascontiguousarray(rays)
+    tm = np.ascontiguousarray(t_max, np.float32)
+    rc = lib().orc_traverse_batch(desc_ptr, node_count, _p(rays), _p(tm), n,
                                   _p(out), _p(st) if st is not None else None, nthreads)
     if rc != 0:
         raise RuntimeError("oracle traversal stack overflow")
@@ -117,7 +129,9 @@ def occluded(desc_ptr: int, node_count: int, rays: np.ndarray, t_max: np.ndarray
     n = len(rays)
     out = np.zeros(n, np.uint8)
     st = np.zeros(n, STATS) if with_stats else None
-    rc = lib().orc_occluded_batch(desc_ptr, node_count, _p(rays), _p(np.ascontiguousarray(t_max, np.float32)), n,
+    rays = np.ascontiguousarray(rays)
+    tm = np.ascontiguousarray(t_max, np.float32)
+    rc = lib().orc_occluded_batch(desc_ptr, node_count, _p(rays), _p(tm), n,
                                   _p(out), _p(st) if st is not None else None, nthreads)
     if rc != 0:
         raise RuntimeError("oracle traversal stack overflow")
@@ -128,8 +142,9 @@ def brute_closest(desc_ptr: int, prim_count: int, rays: np.ndarray, t_max: np.nd
     n = len(rays)
     hi = np.zeros(n, np.int64)
     t = np.zeros(n, np.float32)
-    lib().orc_brute_closest(desc_ptr, prim_count, _p(rays), _p(np.ascontiguousarray(t_max, np.float32)), n, _p(hi),
-                            _p(t))
+    rays = np.ascontiguousarray(rays)
+    tm = np.ascontiguousarray(t_max, np.float32)
+    lib().orc_brute_closest(desc_ptr, prim_count, _p(rays), _p(tm), n, _p(hi), _p(t))
     return hi, t
 
 
@@ -212,3 +227,70 @@ def equal_area_square_to_sphere(u: float, v: float, unit: bool = False) -> np.nd
 
 def rr_draw(seed: int, px: int, py: int, si: int, bounce: int) -> float:
     return lib().orc_rr_draw(seed, px, py, si, bounce)
+
+
+# ---- BSDFs (oracle_bsdf.c) -------------------------------------------------------------------
+
+def _f3(v) -> np.ndarray:
+    return np.ascontiguousarray(v, np.float32).reshape(3)
+
+
+def fresnel_dielectric(cos_i: float, eta: float) -> float:
+    return lib().orc_fresnel_dielectric(cos_i, eta)
+
+
+# (every array handed to C is bound to a local name first, so that it outlives the call)
+
+def fresnel_conductor(cos_t: float, eta, k) -> np.ndarray:
+    e, kk, out = _f3(eta), _f3(k), np.zeros(3, np.float32)
+    lib().orc_fresnel_conductor(cos_t, _p(e), _p(kk), _p(out))
+    return out
+
+
+def refract(wi, n, eta: float):
+    w, nn, out = _f3(wi), _f3(n), np.zeros(4, np.float32)
+    ok = lib().orc_refract(_p(w), _p(nn), eta, _p(out))
+    return bool(ok), out[:3], float(out[3])
+
+
+def tr_D(ax: float, ay: float, h) -> float:
+    hh = _f3(h)
+    return lib().orc_tr_D(ax, ay, _p(hh))
+
+
+def tr_G1(ax: float, ay: float, w) -> float:
+    ww = _f3(w)
+    return lib().orc_tr_G1(ax, ay, _p(ww))
+
+
+def tr_pdf(ax: float, ay: float, wo, h) -> float:
+    o, hh = _f3(wo), _f3(h)
+    return lib().orc_tr_pdf(ax, ay, _p(o), _p(hh))
+
+
+def tr_sample_h(ax: float, ay: float, wo, u0: float, u1: float) -> np.ndarray:
+    o, out = _f3(wo), np.zeros(3, np.float32)
+    lib().orc_tr_sample_h(ax, ay, _p(o), u0, u1, _p(out))
+    return out
+
+
+def bsdf_f(material: np.ndarray, wo, wi, seed: int = 0) -> np.ndarray:
+    """Local-frame f of one GzMaterial record (the layered BSDF draws from stream `seed`)."""
+    m = np.ascontiguousarray(material).reshape(1)
+    o, i, out = _f3(wo), _f3(wi), np.zeros(3, np.float32)
+    lib().orc_bx_f(_p(m), _p(o), _p(i), seed, _p(out))
+    return out
+
+
+def bsdf_pdf(material: np.ndarray, wo, wi, seed: int = 0) -> float:
+    m = np.ascontiguousarray(material).reshape(1)
+    o, i = _f3(wo), _f3(wi)
+    return lib().orc_bx_pdf(_p(m), _p(o), _p(i), seed)
+
+
+def bsdf_sample(material: np.ndarray, wo, u, seed: int = 0):
+    """Returns (f, wi, pdf) of the local-frame sample for uniform numbers u[0..2]."""
+    m = np.ascontiguousarray(material).reshape(1)
+    o, uu, out = _f3(wo), _f3(u), np.zeros(7, np.float32)
+    lib().orc_bx_sample(_p(m), _p(o), _p(uu), seed, _p(out))
+    return out[:3], out[3:6], float(out[6])

@@ -20,9 +20,10 @@
  *     tHit = 0) ........................................... Sampleable.swift:49-69
  *   Visibility.occluded, spawn-ray offsets ................ Core/Visibility.swift:5-24,
  *                                                          Core/Interaction.swift:15-53
- *   ShadingFrame, BsdfFrame, FramedBsdf ................... Core/ShadingFrame.swift, Reflection/
- *   DiffuseBsdf, Bsdf default cosine sampling ............. Bsdf/DiffuseBsdf.swift, Reflection/Bsdf.swift
- *   concentric disk, cosine hemisphere, power heuristic ... Sampler/Sampling.swift
+ *   ShadingFrame, BsdfFrame, FramedBsdf ................... Core/ShadingFrame.swift,
+ *                                                          Reflection/BsdfFrame.swift, FramedBsdf.swift
+ *   the BSDFs in their local frame ......................... oracle_bsdf.c
+ *   power heuristic ....................................... Sampler/Sampling.swift:27-30
  *   InfiniteLight (constant radiance map: uniform equal-area
  *     sphere sampling, pdf 1/(4 pi), power pi 50^2 avg) .... Light/InfiniteLight.swift:39-138
  * Deterministic replacement for the reference's system-random Russian roulette: u = the top 24
@@ -34,12 +35,13 @@
  * uniform in area; light pdf = d^2 / (cos A)), and the white-furnace closed form
  * L = Le (1 - a^(D+1)) / (1 - a) with the light-pdf correction switched on.
  */
-#include <math.h>
 #include <pthread.h>
 #include <stdatomic.h>
 #include <stddef.h>
 #include <stdint.h>
 #include <string.h>
+
+#include "oracle_bsdf.h"
 
 /* ---- shared layouts (match include/gonzales_amd.h byte for byte) -------------------------- */
 typedef struct { int64_t id1, id2, materialIndex; int8_t type; int8_t pad[7]; } mb_prim;
@@ -56,18 +58,6 @@ typedef struct {
 } mb_path;
 _Static_assert(sizeof(mb_path) == 112, "GzPath");
 
-typedef struct {
-        int32_t type;
-        float reflectance[3];
-        float eta;
-        float roughness;
-        float conductorEta[3];
-        float conductorK[3];
-        int32_t remapRoughness;
-        int32_t reserved[3];
-} mb_material;
-_Static_assert(sizeof(mb_material) == 64, "GzMaterial");
-
 typedef struct { float brightness[3]; int32_t mesh; int64_t triangle; } mb_light;
 _Static_assert(sizeof(mb_light) == 24, "GzAreaLight");
 
@@ -76,15 +66,16 @@ typedef struct {
         int32_t flags;          /* FIX_LIGHT_PDF | FIX_SKY_DIRECTION */
         int64_t zsSeed;
         int32_t zsLog2Spp, zsBase4Digits;
-        uint64_t rrSeed;
+        uint64_t rrSeed;        /* Russian roulette and the layered BSDF's streams */
         float sky[3];           /* constant infinite light radiance */
         int32_t hasSky;
 } mb_params;
 _Static_assert(sizeof(mb_params) == 48, "GzMisParams");
 
 enum { PATH_ACTIVE = 1, PATH_SPECULAR = 2 };
-enum { MAT_DIFFUSE = 1 };
 enum { FIX_LIGHT_PDF = 1, FIX_SKY_DIRECTION = 2 };
+/* the layered BSDF's stream of each BSDF call of a bounce */
+enum { CALL_LIGHT_F = 1, CALL_LIGHT_PDF = 2, CALL_MIS_SAMPLE = 3, CALL_CONTINUE = 4 };
 
 /* oracle_traverse.c / oracle_zsobol.c */
 typedef struct { uint32_t interior, tris; } mb_stats;
@@ -97,49 +88,9 @@ uint64_t orc_zsobol_sample_index(const mb_zs *z);
 float orc_zsobol_get1d(mb_zs *z, const uint32_t *m);
 void orc_zsobol_get2d(mb_zs *z, const uint32_t *m, float *out);
 void orc_zsobol_get3d(mb_zs *z, const uint32_t *m, float *out);
-
-/* ---- vector helpers in the reference's operation order ------------------------------------- */
-typedef struct { float x, y, z; } v3;
-static inline v3 V(float x, float y, float z) { v3 r = {x, y, z}; return r; }
-static inline v3 add(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
-static inline v3 sub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
-static inline v3 mul(v3 a, float s) { return V(a.x * s, a.y * s, a.z * s); }
-static inline v3 dvs(v3 a, float s) { return V(a.x / s, a.y / s, a.z / s); }
-static inline v3 neg(v3 a) { return V(-a.x, -a.y, -a.z); }
-static inline float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }   /* Three.swift:131-133 */
-static inline float absdot(v3 a, v3 b) { return fabsf(dot(a, b)); }
-static inline v3 cross(v3 a, v3 b) {                                                  /* Three.swift:119-129 */
-        return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-static inline float len2(v3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }        /* Three.swift:15-17 */
-static inline v3 normalized(v3 a) {                                                   /* Three.swift:108-117 */
-        float l = sqrtf(len2(a));
-        return l == 0.0f ? a : dvs(a, l);
-}
-/* Geometry.swift:63-77 */
-static void coordinate_system(v3 v1, v3 *v2, v3 *v3o) {
-        if (fabsf(v1.x) > fabsf(v1.y))
-                *v2 = dvs(V(-v1.z, 0.0f, v1.x), sqrtf(v1.x * v1.x + v1.z * v1.z));
-        else
-                *v2 = dvs(V(0.0f, v1.z, -v1.y), sqrtf(v1.y * v1.y + v1.z * v1.z));
-        *v3o = cross(v1, *v2);
-}
-
-typedef struct { float r, g, b; } rgb;
-static inline rgb C(float r, float g, float b) { rgb c = {r, g, b}; return c; }
-static inline rgb cadd(rgb a, rgb b) { return C(a.r + b.r, a.g + b.g, a.b + b.b); }
-static inline rgb cmul(rgb a, rgb b) { return C(a.r * b.r, a.g * b.g, a.b * b.b); }
-static inline rgb cscale(rgb a, float s) { return C(a.r * s, a.g * s, a.b * s); }
-static inline rgb cdiv(rgb a, float s) { return C(a.r / s, a.g / s, a.b / s); }
-static inline int cblack(rgb a) { return a.r == 0.0f && a.g == 0.0f && a.b == 0.0f; }
-static inline float cmax(rgb a) {                                                      /* Spectrum.swift:357-359 */
-        /* max(red, max(green, blue)) with Swift's max(x, y) = y >= x ? y : x */
-        float gb = a.b >= a.g ? a.b : a.g;
-        return gb >= a.r ? gb : a.r;
-}
-static inline float cavg(rgb a) { return (a.r + a.g + a.b) / 3.0f; }                  /* Spectrum.swift:353-355 */
-
-static const float PI_F = 3.14159265358979323846f;
+/* oracle_bsdf.c */
+void orc_concentric_disk_xy(float u0, float u1, float *x, float *y);
+v3 orc_cosine_hemisphere_v(float u0, float u1);
 
 /* ---- scene access --------------------------------------------------------------------------- */
 typedef struct {
@@ -213,59 +164,39 @@ static int surface_interaction(const mb_ctx *c, const mb_ray *ray, const mb_isec
         return 1;
 }
 
-/* ---- BSDF (diffuse) in the shading frame -------------------------------------------------- */
-typedef struct { v3 x, y, z, ng; rgb R; int type; } mb_bsdf;
+/* ---- BSDF in the shading frame (BsdfFrame / FramedBsdf) ------------------------------------ */
+typedef struct { v3 x, y, z, ng; bx b; } mb_bsdf;
 
 static void bsdf_make(const mb_ctx *c, const mb_si *si, mb_bsdf *b) {
         b->y = normalized(si->dpdu);            /* BsdfFrame(interaction:) */
         b->z = si->shadingNormal;
         b->x = cross(b->y, b->z);                /* ShadingFrame(y:z:) */
         b->ng = si->normal;
-        const mb_material *m = &c->mats[si->material];
-        b->type = m->type;
-        b->R = C(m->reflectance[0], m->reflectance[1], m->reflectance[2]);
+        bx_from_material(&c->mats[si->material], &b->b);
 }
 static v3 to_local(const mb_bsdf *b, v3 w) { return normalized(V(dot(w, b->x), dot(w, b->y), dot(w, b->z))); }
 static v3 to_world(const mb_bsdf *b, v3 l) { return add(add(mul(b->x, l.x), mul(b->y, l.y)), mul(b->z, l.z)); }
 
-static rgb bsdf_f(const mb_bsdf *b) { return cdiv(b->R, PI_F); }  /* DiffuseBsdf.swift:15-17 */
-
-/* FramedBsdf.evaluateWorldSpace (FramedBsdf.swift:18-34): reflection only for diffuse */
-static rgb bsdf_eval_world(const mb_bsdf *b, v3 wo, v3 wi) {
+/* FramedBsdf.evaluateWorldSpace (FramedBsdf.swift:18-34): every BSDF keeps the protocol's
+ * isReflective = true, isTransmissive = false, so only the reflection side is evaluated. */
+static rgb bsdf_eval_world(const mb_bsdf *b, v3 wo, v3 wi, lrng *r) {
+        v3 ol = to_local(b, wo), il = to_local(b, wi);
         int reflect = dot(wi, b->ng) * dot(wo, b->ng) > 0.0f;
-        return reflect ? bsdf_f(b) : C(0, 0, 0);
+        return reflect ? bx_f(&b->b, ol, il, r) : C(0, 0, 0);
 }
-/* Bsdf.probabilityDensity (Bsdf.swift:147-151) via probabilityDensityWorldSpace */
-static float bsdf_pdf_world(const mb_bsdf *b, v3 wo, v3 wi) {
+/* FramedBsdf.probabilityDensityWorldSpace (FramedBsdf.swift:36-44) */
+static float bsdf_pdf_world(const mb_bsdf *b, v3 wo, v3 wi, lrng *r) {
         v3 il = to_local(b, wi), ol = to_local(b, wo);
         if (ol.z == 0.0f) return 0.0f;
-        if (!(ol.z * il.z > 0.0f)) return 0.0f;
-        return fabsf(il.z) / PI_F;
+        return bx_pdf(&b->b, ol, il, r);
 }
-/* Sampling.swift:3-25 */
-static void concentric_disk(float u0, float u1, float *x, float *y) {
-        float ox = 2.0f * u0 - 1.0f, oy = 2.0f * u1 - 1.0f;
-        if (ox == 0.0f && oy == 0.0f) { *x = 0.0f; *y = 0.0f; return; }
-        float theta, r;
-        if (fabsf(ox) > fabsf(oy)) { r = ox; theta = (PI_F / 4.0f) * (oy / ox); }
-        else { r = oy; theta = (PI_F / 2.0f) - (PI_F / 4.0f) * (ox / oy); }
-        *x = r * cosf(theta);
-        *y = r * sinf(theta);
-}
-static v3 cosine_hemisphere(float u0, float u1) {
-        float x, y;
-        concentric_disk(u0, u1, &x, &y);
-        float z2 = 1.0f - x * x - y * y;
-        return V(x, y, sqrtf(z2 > 0.0f ? z2 : 0.0f));
-}
-/* FramedBsdf.sampleWorldSpace + Bsdf.sample (Bsdf.swift:123-133) */
-static void bsdf_sample_world(const mb_bsdf *b, v3 wo, const float *u, rgb *f, v3 *wi, float *pdf) {
+/* FramedBsdf.sampleWorldSpace (FramedBsdf.swift:46-57) */
+static void bsdf_sample_world(const mb_bsdf *b, v3 wo, const float *u, rgb *f, v3 *wi, float *pdf, lrng *r) {
         v3 ol = to_local(b, wo);
-        v3 il = cosine_hemisphere(u[0], u[1]);
-        if (ol.z < 0.0f) il.z = -il.z;
-        *pdf = (ol.z * il.z > 0.0f) ? fabsf(il.z) / PI_F : 0.0f;
-        *f = bsdf_f(b);
-        *wi = to_world(b, il);
+        bsample s = bx_sample(&b->b, ol, u[0], u[1], u[2], r);
+        *f = s.f;
+        *wi = to_world(b, s.wi);
+        *pdf = s.pdf;
 }
 
 static float power_heuristic(float f, float g) {  /* Sampling.swift:27-30 */
@@ -452,8 +383,10 @@ static int one_bounce(const mb_ctx *c, mb_path *p, mb_zs *zs, mb_zs *lzs) {
         }
         mb_bsdf b;
         bsdf_make(c, &si, &b);
+#define RNG(call) lrng_make(c->p.rrSeed, p->pixelX, p->pixelY, p->sampleIndex, p->bounce, (call))
         if (p->bounce == 0) {
-                p->albedo[0] = b.R.r; p->albedo[1] = b.R.g; p->albedo[2] = b.R.b;
+                rgb al = bx_albedo(&b.b);
+                p->albedo[0] = al.r; p->albedo[1] = al.g; p->albedo[2] = al.b;
                 p->firstNormal[0] = si.normal.x; p->firstNormal[1] = si.normal.y; p->firstNormal[2] = si.normal.z;
         }
         {
@@ -473,12 +406,14 @@ static int one_bounce(const mb_ctx *c, mb_path *p, mb_zs *zs, mb_zs *lzs) {
                 float lpdf = 0.0f;
                 v3 linc = V(0, 0, 0);
                 if (!cblack(ls.L) && !isinf(ls.pdf) && !occluded(c, ls.from, ls.target)) {
-                        rgb scatter = cscale(bsdf_eval_world(&b, si.outgoing, ls.dir), absdot(ls.dir, si.shadingNormal));
+                        lrng r1 = RNG(CALL_LIGHT_F);
+                        rgb scatter = cscale(bsdf_eval_world(&b, si.outgoing, ls.dir, &r1), absdot(ls.dir, si.shadingNormal));
                         Lest = cmul(scatter, ls.L);
                         lpdf = ls.pdf;
                         linc = ls.dir;
                 }
-                float bpdfL = bsdf_pdf_world(&b, si.outgoing, linc);
+                lrng r2 = RNG(CALL_LIGHT_PDF);
+                float bpdfL = bsdf_pdf_world(&b, si.outgoing, linc, &r2);
                 float wl = power_heuristic(lpdf, bpdfL);
                 rgb lc = lpdf == 0.0f ? C(0, 0, 0) : cdiv(cscale(Lest, wl), lpdf);
                 /* sampleMultipleImportance: BSDF sample */
@@ -487,7 +422,8 @@ static int one_bounce(const mb_ctx *c, mb_path *p, mb_zs *zs, mb_zs *lzs) {
                 rgb bf;
                 v3 bwi;
                 float bpdf;
-                bsdf_sample_world(&b, si.outgoing, u3, &bf, &bwi, &bpdf);
+                lrng r3 = RNG(CALL_MIS_SAMPLE);
+                bsdf_sample_world(&b, si.outgoing, u3, &bf, &bwi, &bpdf, &r3);
                 bf = cscale(bf, absdot(bwi, si.shadingNormal));
                 rgb bc = C(0, 0, 0);
                 if (!cblack(bf) && bpdf > 0.0f) {
@@ -526,9 +462,10 @@ static int one_bounce(const mb_ctx *c, mb_path *p, mb_zs *zs, mb_zs *lzs) {
                 rgb f;
                 v3 wi;
                 float pdf;
-                bsdf_sample_world(&b, si.outgoing, u3, &f, &wi, &pdf);
+                lrng r4 = RNG(CALL_CONTINUE);
+                bsdf_sample_world(&b, si.outgoing, u3, &f, &wi, &pdf, &r4);
                 if (pdf == 0.0f || isnan(pdf)) goto done;
-                specular = 0;  /* diffuse */
+                specular = bx_specular(&b.b);
                 T = cmul(T, cdiv(cscale(f, absdot(wi, si.normal)), pdf));
                 p->ray = spawn(&si, wi);
         }
@@ -606,9 +543,9 @@ int orc_mis_trace_paths(const mb_scene *s, int64_t nodeCount, const mb_material 
 
 /* Exposed for the known-answer tests. */
 float orc_power_heuristic(float f, float g) { return power_heuristic(f, g); }
-void orc_concentric_disk(float u0, float u1, float *out) { concentric_disk(u0, u1, &out[0], &out[1]); }
+void orc_concentric_disk(float u0, float u1, float *out) { orc_concentric_disk_xy(u0, u1, &out[0], &out[1]); }
 void orc_cosine_hemisphere(float u0, float u1, float *out) {
-        v3 v = cosine_hemisphere(u0, u1);
+        v3 v = orc_cosine_hemisphere_v(u0, u1);
         out[0] = v.x; out[1] = v.y; out[2] = v.z;
 }
 void orc_equal_area_square_to_sphere(float u, float v, int unit, float *out) {

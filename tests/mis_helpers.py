@@ -46,6 +46,33 @@ def box_on_plane_scene():
     return build_bounding_hierarchy(Scene(meshes, _materials([(0.6, 0.5, 0.4), (0.3, 0.7, 0.5)])), prims)
 
 
+def mixed_material_scene():
+    """Diffuse plane, a rotated box whose 10 triangles cycle through a conductor, smooth and rough
+    dielectrics and two coated conductors (clear and scattering coat), and one triangle area light
+    above. Returns (hierarchy, GzMaterial table)."""
+    from gonzales_amd.integrator import (coated_conductor_material, conductor_material, dielectric_material,
+                                         diffuse_material)
+    from gonzales_amd.scenes import _box
+    gz = abi.concat([diffuse_material((0.6, 0.5, 0.4)), conductor_material(roughness=0.2),
+                     dielectric_material(eta=1.5, roughness=0.0), dielectric_material(eta=1.33, roughness=0.1),
+                     coated_conductor_material(interface_roughness=0.05, conductor_roughness=0.3),
+                     coated_conductor_material(reflectance=(0.9, 0.6, 0.3), albedo=(0.5, 0.7, 0.9), g=0.3,
+                                               thickness=0.2, interface_roughness=0.2, conductor_roughness=0.1)],
+                    abi.GZ_MATERIAL)
+    v, i = quad([-50, 0, -50], [-50, 0, 50], [50, 0, 50], [50, 0, -50])
+    bv, bi = _box(0.1, -0.2, 0.5, 1.0, 30.0, with_floor=False)
+    tri = np.array([[-0.6, 2.2, -0.4], [0.9, 2.2, -0.5], [0.1, 2.2, 0.8]], np.float32)  # normal down
+    meshes = [TriangleMesh.from_xyz(v, i),
+              TriangleMesh.from_xyz(np.asarray(bv, np.float32), np.asarray(bi, np.int64)),
+              TriangleMesh.from_xyz(tri, np.arange(3, dtype=np.int64))]
+    box_mats = 1 + (np.arange(len(bi) // 3) % 5)
+    prims = Primitives.concat([Primitives.geometric(0, 2, 0), Primitives.geometric(1, len(bi) // 3, box_mats),
+                               Primitives.area_lights(2, 1, 0)])
+    scene = Scene(meshes, _materials([(0.5, 0.5, 0.5)] * len(gz)),
+                  light_emission=np.array([[6.0, 5.0, 4.0]], np.float32))
+    return build_bounding_hierarchy(scene, prims), gz
+
+
 # The light triangle of light_over_plane: corners above the plane at height 1.
 LIGHT_TRI = np.array([[-0.3, 1.0, -0.2], [0.4, 1.0, -0.1], [0.1, 1.0, 0.5]], np.float64)
 

@@ -187,3 +187,20 @@ def test_errors_are_reported(gpu):
     integ.trace(d)
     got = _to_host(d)
     assert not np.any(got["flags"] & abi.GZ_PATH_ACTIVE)
+
+
+@pytest.mark.parametrize("sky", [None, (0.6, 0.7, 0.8)])
+def test_mixed_materials_match_oracle(gpu, sky):
+    """Conductor, smooth and rough dielectrics, clear and scattering coated conductors (the
+    layered BSDF's random walk draws from the same deterministic streams on both sides)."""
+    from mis_helpers import mixed_material_scene
+    bh, gz = mixed_material_scene()
+    cam = {"eye": (0.0, 1.4, 3.5), "look": (0.1, 0.5, -0.2), "up": (0.0, 1.0, 0.0), "fov": 35.0}
+    w = h = 32
+    spp = 8
+    rays, px, py = pinhole_rays(cam, w, h)
+    paths = abi.concat([new_paths(rays, px, py, s) for s in range(spp)], abi.GZ_PATH)
+    opt = MisOptions.for_image(w, h, spp, max_depth=5)
+    got, ref = _run_both(bh, paths, opt, sky=sky, materials=gz)
+    assert np.any(ref["flags"] == 0)
+    _check(got, ref, min_frac=0.98)
