@@ -100,6 +100,19 @@ class GzMisParams(ctypes.Structure):
                 ("zsLog2Spp", ctypes.c_int32), ("zsBase4Digits", ctypes.c_int32), ("rrSeed", ctypes.c_uint64)]
 
 
+class GzCamera(ctypes.Structure):
+    _fields_ = [("rasterToCamera", ctypes.c_float * 16), ("cameraToWorld", ctypes.c_float * 16),
+                ("resX", ctypes.c_int32), ("resY", ctypes.c_int32), ("x0", ctypes.c_int32), ("y0", ctypes.c_int32),
+                ("x1", ctypes.c_int32), ("y1", ctypes.c_int32), ("spp", ctypes.c_int32),
+                ("filterType", ctypes.c_int32), ("filterRadius", ctypes.c_float * 2), ("filterSigma", ctypes.c_float),
+                ("lensRadius", ctypes.c_float), ("focalDistance", ctypes.c_float), ("pad", ctypes.c_int32),
+                ("zsSeed", ctypes.c_int64), ("zsLog2Spp", ctypes.c_int32), ("zsBase4Digits", ctypes.c_int32)]
+
+
+GZ_FILTER_GAUSSIAN = 0
+GZ_FILTER_BOX = 1
+
+
 class GzError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"{message} (code {code})")
@@ -138,6 +151,9 @@ SIGNATURES = {
     "gz_gpu_mis_bounce_device": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
     "gz_gpu_mis_trace_device": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
     "gz_gpu_mis_trace_stats_device": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
+    "gz_gpu_camera_paths_device": (ctypes.c_int, [_P, _I64, _I64, _P, _P]),
+    "gz_gpu_film_add_device": (ctypes.c_int, [_P, _P, _I64, ctypes.c_float, ctypes.c_float, _P, _P, _P, _P, _P]),
+    "gz_gpu_film_resolve_device": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
 }
 
 _lib = None

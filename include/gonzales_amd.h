@@ -366,6 +366,54 @@ int gz_gpu_mis_trace_stats_device(void *handle, struct GzPath *d_paths, int64_t 
                                   const struct GzMisParams *params, struct GzPathStats *d_stats, void *stream);
 
 /* ---------------------------------------------------------------------------------------- */
+/* Camera samples, primary rays and film (Core/Tile.swift:27-59, :170-219; Core/Film.swift).  */
+/* ---------------------------------------------------------------------------------------- */
+
+/* A pinhole PerspectiveCamera with its film and pixel filter, as the host builds it
+ * (Camera/PerspectiveCamera.swift:3-23, CameraTransform.swift:3-27): rasterToCamera and
+ * cameraToWorld are row-major 4x4 matrices. Paths are numbered pixel-major over the crop
+ * [x0, x1) x [y0, y1): path i is sample i % spp of pixel i / spp (row-major in the crop). */
+struct GzCamera {
+        float rasterToCamera[16];
+        float cameraToWorld[16];
+        int32_t resX, resY;
+        int32_t x0, y0, x1, y1;
+        int32_t spp;
+        int32_t filterType;      /* GZ_FILTER_GAUSSIAN or GZ_FILTER_BOX */
+        float filterRadius[2];   /* the filter's support */
+        float filterSigma;
+        float lensRadius;        /* must be 0: thin-lens depth of field is not supported */
+        float focalDistance;
+        int32_t pad;
+        int64_t zsSeed;          /* the Z-Sobol constants of the film (gz_zsobol_params) */
+        int32_t zsLog2Spp;
+        int32_t zsBase4Digits;
+};
+enum { GZ_FILTER_GAUSSIAN = 0, GZ_FILTER_BOX = 1 };
+
+/* Fresh GzPath records for paths first .. first+count-1 of the crop: startPixelSample (dim 0),
+ * getCameraSample (Sampler.swift:69-83), generateRay, filterWeight = filter value / filter pdf
+ * (Tile.swift:27-59); dim = 4, lightDim = 0, GZ_PATH_ACTIVE. Asynchronous on `stream`.
+ * Replaces the reference's primary-ray loop (Tile.swift:27-59), which has no C-ABI form. */
+int gz_gpu_camera_paths_device(const struct GzCamera *camera, int64_t first, int64_t count,
+                               struct GzPath *d_paths, void *stream);
+
+/* Film.add(sample:) for `count` finished paths in the layout gz_gpu_camera_paths_device writes
+ * (whole pixels: count is a multiple of spp), each pixel's samples added in sample order to
+ * the resX*resY images: d_light RGB (estimate x iso/100, clamped to maxComponentValue when it is
+ * > 0), d_weight (filterWeight), d_albedo RGB, d_normal RGB (|normalized first normal|)
+ * (Core/Film.swift:107-136, Image/Image.swift:21-30). Asynchronous on `stream`. */
+int gz_gpu_film_add_device(const struct GzCamera *camera, const struct GzPath *d_paths, int64_t count, float iso,
+                           float maxComponentValue, float *d_light, float *d_weight, float *d_albedo,
+                           float *d_normal, void *stream);
+
+/* Pixel.normalized for `pixels` pixels (Core/Pixel.swift:12-16): black stays black, otherwise
+ * light / weight into d_out (RGB); pixels with light but zero weight (the reference throws) are
+ * set to NaN and counted into *d_zeroWeight. Asynchronous on `stream`. */
+int gz_gpu_film_resolve_device(const float *d_light, const float *d_weight, int64_t pixels, float *d_out,
+                               int32_t *d_zeroWeight, void *stream);
+
+/* ---------------------------------------------------------------------------------------- */
 /* Z-Sobol sampler (replaces the Swift ZSobolSampler, Sampler/ZSobolSampler.swift).            */
 /* ---------------------------------------------------------------------------------------- */
 
@@ -437,6 +485,7 @@ static_assert(sizeof(struct GzAreaLight) == 24, "GzAreaLight layout");
 static_assert(sizeof(struct GzPath) == 112, "GzPath layout");
 static_assert(sizeof(struct GzMisParams) == 32, "GzMisParams layout");
 static_assert(sizeof(struct GzPathStats) == 12, "GzPathStats layout");
+static_assert(sizeof(struct GzCamera) == 200, "GzCamera layout");
 #endif
 
 #endif /* GONZALES_AMD_H */

@@ -61,6 +61,13 @@ _SIGS = {
     "orc_bx_f": (None, [_P, _P, _P, ctypes.c_uint64, _P]),
     "orc_bx_pdf": (ctypes.c_float, [_P, _P, _P, ctypes.c_uint64]),
     "orc_bx_sample": (None, [_P, _P, _P, ctypes.c_uint64, _P]),
+    "orc_erfinv": (ctypes.c_float, [ctypes.c_float]),
+    "orc_gaussian_evaluate": (ctypes.c_float, [ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    "orc_gaussian_sample": (None, [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                   ctypes.c_float, _P]),
+    "orc_camera_paths": (ctypes.c_int, [_P, _P, _I64, _I64, _P]),
+    "orc_film_add": (ctypes.c_int, [_P, _P, _I64, _I64, ctypes.c_float, ctypes.c_float, _P, _P, _P, _P]),
+    "orc_film_resolve": (_I64, [_P, _P, _I64, _P]),
 }
 
 _lib = None
@@ -294,3 +301,41 @@ def bsdf_sample(material: np.ndarray, wo, u, seed: int = 0):
     o, uu, out = _f3(wo), _f3(u), np.zeros(7, np.float32)
     lib().orc_bx_sample(_p(m), _p(o), _p(uu), seed, _p(out))
     return out[:3], out[3:6], float(out[6])
+
+
+# ---- camera, filter, film (oracle_film.c) ------------------------------------------------------
+
+def erfinv(y: float) -> float:
+    return lib().orc_erfinv(y)
+
+
+def gaussian_evaluate(x: float, y: float, sigma: float) -> float:
+    return lib().orc_gaussian_evaluate(x, y, sigma)
+
+
+def gaussian_sample(u0: float, u1: float, rx: float, ry: float, sigma: float):
+    """(dx, dy, pdf) of GaussianFilter.sample."""
+    out = np.zeros(3, np.float32)
+    lib().orc_gaussian_sample(u0, u1, rx, ry, sigma, _p(out))
+    return float(out[0]), float(out[1]), float(out[2])
+
+
+def camera_paths(camera, first: int, count: int, path_dtype, table=None) -> np.ndarray:
+    """GzPath records of paths first..first+count-1 for a GzCamera (ctypes structure)."""
+    table = sobol_matrices() if table is None else table
+    out = np.zeros(count, path_dtype)
+    if lib().orc_camera_paths(ctypes.byref(camera), _p(table), first, count, _p(out)) != 0:
+        raise RuntimeError("oracle camera: unsupported camera")
+    return out
+
+
+def film_add(camera, paths: np.ndarray, iso: float, max_component: float, light, weight, albedo, normal) -> None:
+    paths = np.ascontiguousarray(paths)
+    lib().orc_film_add(ctypes.byref(camera), _p(paths), 0, len(paths), iso, max_component, _p(light), _p(weight),
+                       _p(albedo), _p(normal))
+
+
+def film_resolve(light: np.ndarray, weight: np.ndarray):
+    out = np.zeros_like(light)
+    bad = lib().orc_film_resolve(_p(light), _p(weight), len(weight), _p(out))
+    return out, int(bad)
