@@ -65,9 +65,28 @@ def parse():
     ap.add_argument("--path-res", type=int, default=512)
     ap.add_argument("--path-spp", type=int, default=16)
     ap.add_argument("--path-steps", type=int, default=5)
+    ap.add_argument("--no-full-path", action="store_true", help="skip the 5M-triangle 1080p path-tracing leg")
+    ap.add_argument("--full-res", type=lambda s: tuple(int(x) for x in s.split("x")), default=(1920, 1080))
+    ap.add_argument("--full-spp", type=int, default=64)
+    ap.add_argument("--full-subdiv", type=int, default=9)
+    ap.add_argument("--full-frames", type=int, default=1)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device copy (reference dispatch for HBM byte counters)")
     return ap.parse_args()
+
+
+def spread_sample(cam, n_paths: int, table, block_pixels: int = 16):
+    """CPU-baseline sample: about n_paths path samples in blocks of whole pixels spread evenly
+    over the crop (so the sample sees sky, walls and objects in the image's proportions)."""
+    import oracle
+    from gonzales_amd import abi
+    spp = cam.spp
+    npix = cam.path_count // spp
+    blocks = max(1, min(npix // block_pixels, n_paths // (block_pixels * spp)))
+    stride = npix // blocks
+    parts = [oracle.camera_paths(cam.c, j * stride * spp, block_pixels * spp, abi.GZ_PATH, table)
+             for j in range(blocks)]
+    return abi.concat(parts, abi.GZ_PATH), [(j * stride * spp, block_pixels * spp) for j in range(blocks)]
 
 
 PATH_RECORD_BYTES = 224  # GzPath read + write per bounce
@@ -75,44 +94,50 @@ SHADING_FETCH_BYTES = 160  # PrimId 32 + mesh view 40 + 3 vertex indices 24 + 3 
 
 
 def path_leg(args, dev, stream, cores: int, run_cpu: bool) -> dict:
-    """Cornell box, 512 x 512 x 16 spp, max depth 5: the MIS integrator's path samples/s.
+    """Cornell box, 512 x 512 x 16 spp, max depth 5: a whole frame on the device.
 
-    One step = reset the 4,194,304 path records from the camera batch resident in HBM (a device
-    copy) + every bounce of every path (shadow rays and MIS BSDF rays included). Algorithmic bytes
-    per path = sum over its bounces of (224 path record + 160 shading fetch) + 64 x interior visits
-    + 40 x triangle tests of all its rays, from the counting variant of the same kernel."""
+    One step = one frame: camera paths for all 4,194,304 path samples (Z-Sobol camera sample,
+    Gaussian filter r 1.5 sigma 0.5, pinhole rays) + every MIS bounce of every path (shadow rays
+    and MIS BSDF rays included) + the film sums. Algorithmic bytes per path = 112 (record written
+    by the camera) + sum over its bounces of (224 path record + 160 shading fetch) + 64 x interior
+    visits + 40 x triangle tests of all its rays + 112 (record read by the film), the work
+    counts from the counting variant of the same kernel."""
     import ctypes
 
     import torch
 
     from gonzales_amd import abi, scenes
-    from gonzales_amd.integrator import MisOptions, VolumePathIntegrator, new_paths, pinhole_rays
+    from gonzales_amd.camera import DeviceFilm, cornell_camera
+    from gonzales_amd.integrator import MisOptions, VolumePathIntegrator
 
-    w = h = args.path_res
-    spp = args.path_spp
-    bh, cam = scenes.cornell_hierarchy()
+    res, spp = args.path_res, args.path_spp
+    cam = cornell_camera(res=res, spp=spp, seed=0)
+    bh, _ = scenes.cornell_hierarchy()
     bh.upload_to_gpu()
-    opt = MisOptions.for_image(w, h, spp, seed=0, max_depth=5)
+    opt = MisOptions(max_depth=5, zsobol_seed=cam.c.zsSeed, log2_spp=cam.c.zsLog2Spp,
+                     base4_digits=cam.c.zsBase4Digits)
     integ = VolumePathIntegrator(bh, opt)
-    rays, px, py = pinhole_rays(cam, w, h)
-    host = abi.concat([new_paths(rays, px, py, s, start_dim=4) for s in range(spp)], abi.GZ_PATH)
-    n = len(host)
-    d_init = torch.from_numpy(host.view(np.uint8)).to(dev)
-    d_paths = torch.empty_like(d_init)
+    n = cam.path_count
+    d_paths = torch.empty(n * abi.GZ_PATH.itemsize, dtype=torch.uint8, device=dev)
     d_stats = torch.zeros(n * 3, dtype=torch.int32, device=dev)
-    d_paths.copy_(d_init)
+    film = DeviceFilm(cam, dev)
+    cam.paths_device(d_paths)
+    start = d_paths.cpu().numpy().view(abi.GZ_PATH).copy() if run_cpu else None
     integ.trace_stats(d_paths, d_stats)
     bh.check_traversal()
     st = d_stats.view(n, 3).to(torch.int64).sum(0).tolist()
-    algo = (PATH_RECORD_BYTES + SHADING_FETCH_BYTES) * st[2] + 64 * st[0] + 40 * st[1]
+    algo = 2 * abi.GZ_PATH.itemsize * n + (PATH_RECORD_BYTES + SHADING_FETCH_BYTES) * st[2] + 64 * st[0] + 40 * st[1]
 
     def step():
-        d_paths.copy_(d_init)
+        cam.paths_device(d_paths)
         integ.trace(d_paths)
+        film.add(d_paths, n)
 
+    film.clear()
     for _ in range(2):
         step()
     torch.cuda.synchronize()
+    film.clear()
     a = torch.cuda.Event(enable_timing=True)
     b = torch.cuda.Event(enable_timing=True)
     steps = args.path_steps
@@ -123,18 +148,19 @@ def path_leg(args, dev, stream, cores: int, run_cpu: bool) -> dict:
     torch.cuda.synchronize()
     bh.check_traversal()
     ms = a.elapsed_time(b) / steps
-    got = d_paths.cpu().numpy().view(abi.GZ_PATH)
-    img = got["estimate"].astype(np.float64).reshape(spp, h, w, 3).mean(0)
-    res = {"workload": f"Cornell box (34 triangles, 2 area-light triangles), {w}x{h}, {spp} spp, max depth 5, "
-                       "Z-Sobol seed 0, RR after bounce 1, diffuse",
-           "path_samples_per_s": round(n / (ms * 1e-3), 1), "ms_per_frame": round(ms, 3),
-           "bounces_per_path": round(st[2] / n, 3), "interior_per_path": round(st[0] / n, 2),
-           "tris_per_path": round(st[1] / n, 2), "bytes_per_path": round(algo / n, 1),
-           "roofline": {"bound": "hbm", "achieved": round(algo / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-           "image_mean_rgb": [round(float(x), 5) for x in img.reshape(-1, 3).mean(0)]}
-    log(f"path leg: {res['path_samples_per_s'] / 1e6:.2f} M path samples/s, {ms:.2f} ms/frame, "
-        f"{res['bounces_per_path']} bounces/path, {res['bytes_per_path']} B/path")
+    img = film.resolve().cpu().numpy().reshape(res, res, 3)
+    res_d = {"workload": f"Cornell box (34 triangles, 2 area-light triangles), {res}x{res}, {spp} spp, max depth 5, "
+                         "Z-Sobol seed 0, RR after bounce 1, diffuse, Gaussian filter r 1.5 sigma 0.5; "
+                         "camera + bounces + film per frame",
+             "path_samples_per_s": round(n / (ms * 1e-3), 1), "ms_per_frame": round(ms, 3),
+             "bounces_per_path": round(st[2] / n, 3), "interior_per_path": round(st[0] / n, 2),
+             "tris_per_path": round(st[1] / n, 2), "bytes_per_path": round(algo / n, 1),
+             "roofline": {"bound": "hbm", "achieved": round(algo / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+             "image_mean_rgb": [round(float(x), 5) for x in img.reshape(-1, 3).mean(0)],
+             "film_zero_weight_pixels": int(film.bad.item())}
+    log(f"path leg: {res_d['path_samples_per_s'] / 1e6:.2f} M path samples/s, {ms:.2f} ms/frame, "
+        f"{res_d['bounces_per_path']} bounces/path, {res_d['bytes_per_path']} B/path")
     if run_cpu:
         import oracle
 
@@ -142,24 +168,130 @@ def path_leg(args, dev, stream, cores: int, run_cpu: bool) -> dict:
         table = oracle.sobol_matrices()
         prm = oracle.MisParams(opt.max_depth, 0, opt.zsobol_seed, opt.log2_spp, opt.base4_digits, opt.rr_seed,
                                (ctypes.c_float * 3)(0, 0, 0), 0)
-        calib = host[:1 << 14].copy()
+        calib = start[:1 << 14].copy()
         t1 = time.perf_counter()
         oracle.mis_trace(ctypes.addressof(desc), len(bh.nodes), integ.materials, integ.lights, prm, calib, cores, table)
         rate = len(calib) / (time.perf_counter() - t1)
-        m = int(min(n, max(1 << 14, rate * args.cpu_seconds / 2)))
-        sample = host[:m].copy()
+        want = int(min(n, max(1 << 14, rate * args.cpu_seconds / 2)))
         t1 = time.perf_counter()
+        sample, ranges = spread_sample(cam, want, table)
+        oracle.mis_trace(ctypes.addressof(desc), len(bh.nodes), integ.materials, integ.lights, prm, sample, cores,
+                         table)
+        px = res * res
+        light, albedo, normal = (np.zeros(3 * px, np.float32) for _ in range(3))
+        weight = np.zeros(px, np.float32)
+        oracle.film_add(cam.c, sample, 100.0, 0.0, light, weight, albedo, normal)
+        dt = time.perf_counter() - t1
+        m = len(sample)
+        allp = d_paths.cpu().numpy().view(abi.GZ_PATH)
+        got = abi.concat([allp[a:a + c] for a, c in ranges], abi.GZ_PATH)
+        e_g, e_r = got["estimate"].astype(np.float64), sample["estimate"].astype(np.float64)
+        agree = ((np.abs(e_g - e_r) <= 1e-4 * np.abs(e_r) + 1e-6).all(1) & (got["bounce"] == sample["bounce"]))
+        res_d["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "path samples/s", "cores": cores,
+                                 "kind": "port",
+                                 "sample": f"{m} of the {n} path samples ({len(ranges)} blocks of 16 pixels spread "
+                                           f"over the image), camera + bounces (the oracle's traversal per ray, "
+                                           f"{cores} threads) + film, {dt:.1f} s"}
+        res_d["cpu_sample_paths_agreeing"] = round(float(agree.mean()), 5)
+        log(f"path leg cpu oracle {m / dt:.0f} path samples/s on {cores} threads, agreeing {agree.mean():.5f}")
+    bh.free()
+    return res_d
+
+
+def full_path_leg(args, dev, stream, cores: int, run_cpu: bool) -> dict:
+    """Displaced icosphere (20 x 4^9 = 5,242,880 triangles, fBm amplitude 0.05) on a ground plane,
+    diffuse / dielectric / coated-conductor patches of 64 faces, one quad area light and a
+    constant sky; 1920 x 1080, 64 spp, max depth 8, RR after bounce 1. One step = one frame of
+    132,710,400 path samples rendered in batches of whole pixels (camera -> bounces -> film).
+    Work counts (bytes per path) come from the counting kernel over the first batch."""
+    import ctypes
+
+    import torch
+
+    from gonzales_amd import abi, scenes
+    from gonzales_amd.camera import DeviceFilm, Film, PerspectiveCamera, look_at
+    from gonzales_amd.integrator import MisOptions, VolumePathIntegrator
+    from gonzales_amd.render import FrameRenderer
+
+    t0 = time.perf_counter()
+    bh, gz = scenes.displaced_sphere_scene(args.full_subdiv)
+    t_build = time.perf_counter() - t0
+    bh.upload_to_gpu()
+    w, h = args.full_res
+    cam = PerspectiveCamera(look_at((0.0, 0.45, 3.4), (0.0, -0.15, 0.0), (0, 1, 0)), 40.0, Film(w, h),
+                            args.full_spp, seed=0)
+    opt = MisOptions(max_depth=8, zsobol_seed=cam.c.zsSeed, log2_spp=cam.c.zsLog2Spp,
+                     base4_digits=cam.c.zsBase4Digits)
+    sky = (0.4, 0.45, 0.5)
+    integ = VolumePathIntegrator(bh, opt, sky=sky, materials=gz)
+    film = DeviceFilm(cam, dev)
+    rnd = FrameRenderer(cam, integ, film, batch_paths=1 << 23)
+    n = cam.path_count
+    # counting pass over a whole frame (untimed; it also warms the kernels up)
+    m = rnd.batch
+    d_stats = torch.zeros(m * 3, dtype=torch.int32, device=dev)
+    sums = torch.zeros(3, dtype=torch.int64, device=dev)
+    first = 0
+    while first < n:
+        c = min(m, n - first)
+        cam.paths_device(rnd.d_paths, first, c)
+        d_stats.zero_()
+        integ.trace_stats(rnd.d_paths, d_stats, c)
+        sums += d_stats[:3 * c].view(c, 3).to(torch.int64).sum(0)
+        first += c
+    bh.check_traversal()
+    st = sums.tolist()
+    del d_stats
+    per_path = (2 * abi.GZ_PATH.itemsize * n + (PATH_RECORD_BYTES + SHADING_FETCH_BYTES) * st[2] + 64 * st[0]
+                + 40 * st[1]) / n
+    torch.cuda.synchronize()
+    film.clear()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(args.full_frames):
+        rnd.render(stream)
+    b.record(stream)
+    torch.cuda.synchronize()
+    bh.check_traversal()
+    ms = a.elapsed_time(b) / args.full_frames
+    img = film.resolve().cpu().numpy().reshape(h, w, 3)
+    achieved = per_path * n / (ms * 1e-3) / 1e9
+    res = {"workload": f"displaced icosphere {len(bh.prim_ids) - 4} triangles + ground plane, patch materials "
+                       f"(diffuse / dielectric / coated conductor, GGX roughness U[0.05,0.6]), quad area light + "
+                       f"constant sky, {w}x{h}, {args.full_spp} spp, max depth 8; camera + bounces + film",
+           "path_samples_per_s": round(n / (ms * 1e-3), 1), "ms_per_frame": round(ms, 2),
+           "host_scene_build_s": round(t_build, 2),
+           "bounces_per_path": round(st[2] / n, 3), "interior_per_path": round(st[0] / n, 2),
+           "tris_per_path": round(st[1] / n, 2), "bytes_per_path": round(per_path, 1),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4)},
+           "image_mean_rgb": [round(float(x), 5) for x in img.reshape(-1, 3).mean(0)],
+           "film_zero_weight_pixels": int(film.bad.item())}
+    log(f"full path leg: {res['path_samples_per_s'] / 1e6:.2f} M path samples/s, {ms:.1f} ms/frame, "
+        f"{res['bounces_per_path']} bounces/path")
+    if run_cpu:
+        import oracle
+
+        desc, keep = bh.descriptor()
+        table = oracle.sobol_matrices()
+        prm = oracle.MisParams(opt.max_depth, 0, opt.zsobol_seed, opt.log2_spp, opt.base4_digits, opt.rr_seed,
+                               (ctypes.c_float * 3)(*sky), 1)
+        calib, _ = spread_sample(cam, 1 << 13, table)
+        t1 = time.perf_counter()
+        oracle.mis_trace(ctypes.addressof(desc), len(bh.nodes), integ.materials, integ.lights, prm, calib, cores,
+                         table)
+        rate = len(calib) / (time.perf_counter() - t1)
+        t1 = time.perf_counter()
+        sample, ranges = spread_sample(cam, int(max(1 << 13, rate * args.cpu_seconds / 2)), table)
         oracle.mis_trace(ctypes.addressof(desc), len(bh.nodes), integ.materials, integ.lights, prm, sample, cores,
                          table)
         dt = time.perf_counter() - t1
-        g = got[:m]
-        e_g, e_r = g["estimate"].astype(np.float64), sample["estimate"].astype(np.float64)
-        agree = ((np.abs(e_g - e_r) <= 1e-4 * np.abs(e_r) + 1e-6).all(1) & (g["bounce"] == sample["bounce"]))
-        res["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "path samples/s", "cores": cores, "kind": "port",
-                               "sample": f"first {m} of the {n} path samples (sample-major order), same scene, "
-                                         f"{cores} threads, {dt:.1f} s"}
-        res["cpu_sample_paths_agreeing"] = round(float(agree.mean()), 5)
-        log(f"path leg cpu oracle {m / dt:.0f} path samples/s on {cores} threads, agreeing {agree.mean():.5f}")
+        k = len(sample)
+        res["cpu_baseline"] = {"value": round(k / dt, 1), "unit": "path samples/s", "cores": cores, "kind": "port",
+                               "sample": f"{k} path samples ({len(ranges)} blocks of 16 pixels spread over the "
+                                         f"frame): camera + bounces, {cores} threads, {dt:.1f} s"}
+        log(f"full path leg cpu oracle {k / dt:.0f} path samples/s on {cores} threads")
     bh.free()
     return res
 
@@ -330,6 +462,9 @@ def main() -> int:
     if not args.no_path:
         pcores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         extra["path_trace"] = path_leg(args, dev, stream, max(1, min(pcores, 16)), not args.no_cpu)
+    if not args.no_full_path:
+        pcores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        extra["path_trace_full"] = full_path_leg(args, dev, stream, max(1, min(pcores, 16)), not args.no_cpu)
 
     out = {
         "metric": METRIC,

@@ -179,3 +179,109 @@ def cornell_hierarchy():
 
 def total_triangles(scene: Scene) -> int:
     return int(sum(m.number_triangles for m in scene.meshes))
+
+
+# ---------------------------------------------------------------------------------------------
+# Displaced icosphere (the full path-trace workload)
+# ---------------------------------------------------------------------------------------------
+
+def icosphere(subdivisions: int) -> tuple[np.ndarray, np.ndarray]:
+    """Unit icosphere: the icosahedron with every triangle split into 4, `subdivisions` times
+    (20 * 4^s faces, 10 * 4^s + 2 vertices), midpoints pushed to the sphere. Float64 vertices,
+    int64 (F, 3) faces, counter-clockwise seen from outside."""
+    t = (1 + 5 ** 0.5) / 2
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    for _ in range(subdivisions):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        key = np.sort(e, axis=1)
+        uniq, inv = np.unique(key[:, 0] * len(v) + key[:, 1], return_inverse=True)
+        a, b = uniq // len(v), uniq % len(v)
+        mid = v[a] + v[b]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        m = (len(v) + inv).reshape(3, -1)  # midpoint ids of edges (0,1), (1,2), (2,0) per face
+        v = np.concatenate([v, mid])
+        m01, m12, m20 = m
+        f = np.concatenate([np.stack([f[:, 0], m01, m20], 1), np.stack([m01, f[:, 1], m12], 1),
+                            np.stack([m20, m12, f[:, 2]], 1), np.stack([m01, m12, m20], 1)])
+    return v, f
+
+
+def _value_noise(p: np.ndarray, seed: int, table: int = 256) -> np.ndarray:
+    """3-D value noise in [-1, 1]: lattice values from a seeded permutation table, smoothstep
+    trilinear interpolation."""
+    g = _rng(seed)
+    perm = g.permutation(table)
+    vals = g.uniform(-1, 1, table)
+    i = np.floor(p).astype(np.int64)
+    fr = p - i
+    s = fr * fr * (3 - 2 * fr)
+
+    def lat(dx, dy, dz):
+        h = perm[(perm[(perm[(i[:, 0] + dx) % table] + i[:, 1] + dy) % table] + i[:, 2] + dz) % table]
+        return vals[h]
+
+    x0 = lat(0, 0, 0) * (1 - s[:, 0]) + lat(1, 0, 0) * s[:, 0]
+    x1 = lat(0, 1, 0) * (1 - s[:, 0]) + lat(1, 1, 0) * s[:, 0]
+    x2 = lat(0, 0, 1) * (1 - s[:, 0]) + lat(1, 0, 1) * s[:, 0]
+    x3 = lat(0, 1, 1) * (1 - s[:, 0]) + lat(1, 1, 1) * s[:, 0]
+    y0 = x0 * (1 - s[:, 1]) + x1 * s[:, 1]
+    y1 = x2 * (1 - s[:, 1]) + x3 * s[:, 1]
+    return y0 * (1 - s[:, 2]) + y1 * s[:, 2]
+
+
+def fbm(p: np.ndarray, seed: int, octaves: int = 4, lacunarity: float = 2.0, gain: float = 0.5,
+        frequency: float = 4.0) -> np.ndarray:
+    out = np.zeros(len(p))
+    amp, freq = 1.0, frequency
+    for _ in range(octaves):
+        out += amp * _value_noise(p * freq, seed)
+        amp *= gain
+        freq *= lacunarity
+    return out
+
+
+def displaced_sphere_scene(subdivisions: int = 9, amplitude: float = 0.05, seed_noise: int = 5,
+                           seed_mat: int = 6, patch: int = 64):
+    """Icosphere (r = 1, `subdivisions` levels) displaced along its normal by amplitude x fBm
+    (value noise, 4 octaves, lacunarity 2, gain 0.5), on a 2-triangle ground plane at y = -1.05,
+    with one quad area light (radiance 8) above. One material per `patch` consecutive faces,
+    drawn with seed_mat: diffuse (albedo U[0.1, 0.8]^3), dielectric (eta 1.5) or coated
+    conductor (Cu), GGX roughness U[0.05, 0.6]. Returns (hierarchy, GzMaterial table)."""
+    from .integrator import coated_conductor_material, dielectric_material, diffuse_material
+    v, f = icosphere(subdivisions)
+    v = v * (1 + amplitude * fbm(v, seed_noise))[:, None]
+    sphere = TriangleMesh.from_xyz(v.astype(np.float32), f.reshape(-1))
+    half = 20.0
+    gv, gi = _quad([-half, -1.05, -half], [-half, -1.05, half], [half, -1.05, half], [half, -1.05, -half])
+    ground = TriangleMesh.from_xyz(np.asarray(gv, np.float32), np.asarray(gi, np.int64))
+    lv, li = _quad([-0.75, 2.5, -0.75], [0.75, 2.5, -0.75], [0.75, 2.5, 0.75], [-0.75, 2.5, 0.75])  # faces down
+    light = TriangleMesh.from_xyz(np.asarray(lv, np.float32), np.asarray(li, np.int64))
+    n_faces = len(f)
+    n_patch = (n_faces + patch - 1) // patch
+    g = _rng(seed_mat)
+    kind = g.integers(0, 3, n_patch)
+    albedo = g.uniform(0.1, 0.8, (n_patch, 3))
+    rough = g.uniform(0.05, 0.6, n_patch)
+    mats = [diffuse_material((0.5, 0.5, 0.5))]  # ground
+    for k in range(n_patch):
+        if kind[k] == 0:
+            mats.append(diffuse_material(albedo[k]))
+        elif kind[k] == 1:
+            mats.append(dielectric_material(eta=1.5, roughness=rough[k]))
+        else:
+            mats.append(coated_conductor_material(interface_roughness=rough[k], conductor_roughness=rough[k]))
+    gz = abi.concat(mats, abi.GZ_MATERIAL)
+    face_mat = 1 + np.arange(n_faces) // patch
+    prims = Primitives.concat([Primitives.geometric(0, n_faces, face_mat), Primitives.geometric(1, 2, 0),
+                               Primitives.area_lights(2, 2, 0)])
+    placeholder = np.zeros(len(gz), abi.MATERIAL)
+    placeholder["type"] = 1
+    placeholder["albedo"] = 0.5
+    scene = Scene([sphere, ground, light], placeholder,
+                  light_emission=np.array([[8.0, 8.0, 8.0], [8.0, 8.0, 8.0]], np.float32))
+    return build_bounding_hierarchy(scene, prims), gz
