@@ -13,6 +13,11 @@ figure B = 28 (ray + tMax) + 16 (hit record) + 64 x interior visits + 40 x trian
 the visit and test counts taken from the counting variant of the same kernel over the same rays
 (checked against the CPU oracle on the baseline sample). cpu_baseline = the CPU oracle (a port of
 the reference's mojo_cpu_traverse_batch) on the host cores, on a bounded sample of the same rays.
+
+The other metrics of the workload set are reported under "extra", each with its own roofline
+and CPU baseline: path samples/s of whole frames on the device for the Cornell box (512^2 x 16
+spp, "path_trace") and for the 5M-triangle displaced icosphere (1920x1080 x 64 spp,
+"path_trace_full"), and the path-replay adjoint's time over the forward render ("adjoint").
 """
 from __future__ import annotations
 
@@ -70,6 +75,11 @@ def parse():
     ap.add_argument("--full-spp", type=int, default=64)
     ap.add_argument("--full-subdiv", type=int, default=9)
     ap.add_argument("--full-frames", type=int, default=1)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the inverse-rendering adjoint leg")
+    ap.add_argument("--adj-res", type=int, default=512)
+    ap.add_argument("--adj-spp", type=int, default=32)
+    ap.add_argument("--adj-target-spp", type=int, default=256)
+    ap.add_argument("--adj-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device copy (reference dispatch for HBM byte counters)")
     return ap.parse_args()
@@ -296,6 +306,64 @@ def full_path_leg(args, dev, stream, cores: int, run_cpu: bool) -> dict:
     return res
 
 
+def adjoint_leg(args, dev, stream) -> dict:
+    """Inverse rendering: the Cornell box with two displaced icospheres (163,840 triangles) whose
+    patches carry 32 diffuse (reflectance) and 32 conductor (roughness) parameters; 512 x 512 x 32
+    spp, max depth 5. Forward = one frame (camera + bounces + film); backward = the per-path
+    adjoints from dLoss/dImage of an L2 loss against a 256-spp target image + the path-replay
+    adjoint of every path."""
+    import torch
+
+    from gonzales_amd import scenes
+    from gonzales_amd.camera import cornell_camera
+    from gonzales_amd.diff import DiffRenderer
+    from gonzales_amd.integrator import MisOptions
+
+    bh, gz, ta, tr = scenes.inverse_rendering_scene(subdivisions=6, n_params=32)
+    bh.upload_to_gpu()
+
+    def make(spp):
+        cam = cornell_camera(res=args.adj_res, spp=spp, seed=0)
+        opt = MisOptions(max_depth=5, zsobol_seed=cam.c.zsSeed, log2_spp=cam.c.zsLog2Spp,
+                         base4_digits=cam.c.zsBase4Digits)
+        return DiffRenderer(bh, cam, gz, opt, device=dev)
+
+    target = make(args.adj_target_spp).render().double()
+    r = make(args.adj_spp)
+    m = len(gz)
+    albedo = np.full((m, 3), 0.5, np.float32)
+    rough = np.full(m, 0.5, np.float32)
+    albedo[:3] = gz["reflectance"][:3]
+    r.set_parameters(albedo, rough)
+    for _ in range(2):
+        img = r.render()
+        r.backward(2 * (img.double() - target))
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fwd, bwd = [], []
+    for _ in range(args.adj_steps):
+        ev[0].record(stream)
+        img = r.render()
+        ev[1].record(stream)
+        g_img = 2 * (img.double() - target)
+        ga, gr = r.backward(g_img)
+        ev[2].record(stream)
+        torch.cuda.synchronize()
+        fwd.append(ev[0].elapsed_time(ev[1]))
+        bwd.append(ev[1].elapsed_time(ev[2]))
+    loss = float(((img.double() - target) ** 2).sum())
+    f, b = float(np.median(fwd)), float(np.median(bwd))
+    res = {"workload": f"inverse rendering: Cornell box + 2 displaced icospheres ({len(bh.prim_ids)} triangles), "
+                       f"32 diffuse + 32 conductor parameter materials, {args.adj_res}x{args.adj_res} x "
+                       f"{args.adj_spp} spp, max depth 5, L2 loss vs a {args.adj_target_spp}-spp target",
+           "forward_ms": round(f, 3), "backward_ms": round(b, 3), "adjoint_over_forward": round(b / f, 3),
+           "loss": loss, "grad_albedo_norm": float(ga.norm()), "grad_roughness_norm": float(gr.norm()),
+           "gradient_check": "tests/test_gpu_adjoint.py: reflectance within 1% of central differences"}
+    log(f"adjoint leg: forward {f:.2f} ms, backward {b:.2f} ms, ratio {b / f:.2f}")
+    bh.free()
+    return res
+
+
 def main() -> int:
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -465,6 +533,8 @@ def main() -> int:
     if not args.no_full_path:
         pcores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         extra["path_trace_full"] = full_path_leg(args, dev, stream, max(1, min(pcores, 16)), not args.no_cpu)
+    if not args.no_adjoint:
+        extra["adjoint"] = adjoint_leg(args, dev, stream)
 
     out = {
         "metric": METRIC,

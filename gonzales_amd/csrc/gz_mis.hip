@@ -354,13 +354,92 @@ __device__ __forceinline__ float rr_draw(uint64_t seed, int px, int py, int si, 
         return (float)(uint32_t)(z >> 40) * (1.0f / 16777216.0f);
 }
 
-// One bounce of every active path. kStats: also adds the bounce's traversal work (all its rays)
-// to stats[i] and counts the bounce in stats[i] (interior, tris, bounces as a third word).
-template <bool kStats, bool kAll>
-__global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__restrict__ paths, int64_t n,
-                                                          uint32_t *__restrict__ stats) {
-        const int64_t i = (int64_t)blockIdx.x * kMisBlock + threadIdx.x;
-        if (i >= n) return;
+// Path-replay adjoint state (gz_gpu_mis_adjoint_device): per path its forward radiance and the
+// adjoint of its radiance; gradients accumulate per material as (dL/dR r, g, b, dL/droughness).
+struct AdjArgs {
+        const GzPathAdjoint *paths;
+        double *grad;  // [materials][4] in global memory
+};
+constexpr int kAdjLdsMats = 128;
+
+// Log-derivatives in alpha of an isotropic Trowbridge-Reitz conductor (MicrofacetReflection.swift:
+// 15-26, TrowbridgeReitzDistribution.swift:9-26) for local directions wo, wi:
+//   f:   d ln D(h) + d ln G(wo, wi)    (F and the cosines do not depend on alpha)
+//   pdf: d ln D(h) + d ln G1(wo)       (pdf = D G1(wo) / (4 |cos wo|))
+// D = 1 / (pi a^2 cos^4 (1 + t/a^2)^2), t = tan^2(theta_h); Lambda = (sqrt(1 + a^2 tan^2) - 1) / 2.
+struct DlogAlpha {
+        float f, pdf;
+};
+__device__ __forceinline__ DlogAlpha dlog_dalpha(float al, f3 wo, f3 wi) {
+        f3 h = add(wo, wi);
+        if (h.x == 0.0f && h.y == 0.0f && h.z == 0.0f) return DlogAlpha{0.0f, 0.0f};
+        h = normalized(h);
+        const float t = tan2_theta(h);
+        float dD = -2.0f / al;
+        if (!isinf(t)) dD += 4.0f * t / (al * (al * al + t));
+        float lam[2], dlam[2];
+        const f3 ws[2] = {wo, wi};
+        for (int k = 0; k < 2; ++k) {
+                const float t2 = tan2_theta(ws[k]);
+                if (isinf(t2)) {
+                        lam[k] = 0.0f;
+                        dlam[k] = 0.0f;
+                        continue;
+                }
+                const float s = sqrtf(1.0f + al * al * t2);
+                lam[k] = (s - 1.0f) / 2.0f;
+                dlam[k] = al * t2 / (2.0f * s);
+        }
+        return DlogAlpha{dD - (dlam[0] + dlam[1]) / (1.0f + lam[0] + lam[1]), dD - dlam[0] / (1.0f + lam[0])};
+}
+
+// Adjoint of one bounce's BSDF at a vertex of material m: the light-strategy term, the
+// BSDF-strategy term and everything after the vertex (cont) each carry the BSDF once; the
+// derivative of the path's radiance is each term times d ln f / d theta at its direction, plus,
+// for the two MIS terms, the derivative of their power-heuristic weights through the BSDF pdf
+// (the reference's weights need not sum to one: its light pdf of a BSDF direction toward a
+// triangle is 0). Sampled directions and Russian roulette are held fixed (detached sampling).
+__device__ __forceinline__ void adjoint_vertex(const MisArgs &a, const Bsdf &b, int m, f3 adj, f3 termL, f3 wl,
+                                               float misL, f3 termB, f3 wb, float misB, f3 cont, f3 wc, f3 wo,
+                                               double *lds, double *grad) {
+        double g[4] = {0.0, 0.0, 0.0, 0.0};
+        bool any = false;
+        const GzMaterial &mat = a.mats[m];
+        if (b.b.type == GZ_MAT_DIFFUSE) {
+                const f3 s = add(add(termL, termB), cont);
+                const float R[3] = {b.b.R.x, b.b.R.y, b.b.R.z};
+                const float S[3] = {s.x, s.y, s.z}, A[3] = {adj.x, adj.y, adj.z};
+                for (int c = 0; c < 3; ++c)
+                        if (R[c] != 0.0f) g[c] = (double)A[c] * (double)S[c] / (double)R[c];
+                any = true;
+        } else if (b.b.type == GZ_MAT_CONDUCTOR && mat.roughness[0] == mat.roughness[1] && mat.roughness[0] > 1e-3f) {
+                const float al = b.b.d.ax;
+                float rl = 0.0f, rb = 0.0f, rc = 0.0f;
+                if (!cblack(termL)) {  // w_l = pl^2 / (pl^2 + pb^2): d ln w_l = -2 (1 - w_l) d ln pb
+                        const DlogAlpha d = dlog_dalpha(al, wo, wl);
+                        rl = d.f - 2.0f * (1.0f - misL) * d.pdf;
+                }
+                if (!cblack(termB)) {  // w_b = pb^2 / (pb^2 + pl^2): d ln w_b = 2 (1 - w_b) d ln pb
+                        const DlogAlpha d = dlog_dalpha(al, wo, wb);
+                        rb = d.f + 2.0f * (1.0f - misB) * d.pdf;
+                }
+                if (!cblack(cont)) rc = dlog_dalpha(al, wo, wc).f;
+                const f3 s = add(add(mul(termL, rl), mul(termB, rb)), mul(cont, rc));
+                g[3] = (double)adj.x * s.x + (double)adj.y * s.y + (double)adj.z * s.z;
+                any = true;
+        }
+        if (!any) return;
+        double *dst = lds ? lds : grad;
+        for (int k = 0; k < 4; ++k)
+                if (g[k] != 0.0) atomicAdd(&dst[4 * m + k], g[k]);
+}
+
+// One bounce of path i. kStats: also adds the bounce's traversal work (all its rays) to
+// stats[i] and counts the bounce (interior, tris, bounces). kAdj: replays the bounce of the
+// forward pass and adds the gradient of the path's radiance (lds: per-block accumulator).
+template <bool kStats, bool kAll, bool kAdj>
+__device__ __forceinline__ void bounce_one(const MisArgs &a, GzPath *__restrict__ paths, int64_t i,
+                                           uint32_t *__restrict__ stats, const AdjArgs &adj, double *lds) {
         GzPath p = paths[i];
         if (!(p.flags & GZ_PATH_ACTIVE)) return;
         if (p.bounce > a.maxDepth) {
@@ -433,6 +512,7 @@ __global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__r
                 bsdf_sample_world<kAll>(b, si.outgoing, v0, v1, v2, bf, bwi, bpdf, r3);
                 bf = mul(bf, absdot(bwi, si.shadingNormal));
                 f3 bc = mk(0.0f, 0.0f, 0.0f);
+                float wbMis = 0.0f;
                 const bool weighted = choice >= a.nLights || (a.flags & GZ_MIS_FIX_LIGHT_PDF);
                 if (weighted && !cblack(bf) && bpdf > 0.0f) {
                         const Ray_C r = spawn(si, bwi);
@@ -456,6 +536,7 @@ __global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__r
                                 const f3 best = cmul(bf, bL);
                                 const float lden = light_pdf(a, choice, si, bwi);
                                 const float wb = power_heuristic(bpdf, lden);
+                                wbMis = wb;
                                 bc = divs(mul(best, wb), bpdf);
                         }
                 }
@@ -468,6 +549,15 @@ __global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__r
                 float pdf;
                 LayerRng r4 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallContinue);
                 bsdf_sample_world<kAll>(b, si.outgoing, w0, w1, w2, f, wi, pdf, r4);
+                if (kAdj) {
+                        const GzPathAdjoint &pa = adj.paths[i];
+                        const f3 Ltot = mk(pa.radiance[0], pa.radiance[1], pa.radiance[2]);
+                        const f3 av = mk(pa.adjoint[0], pa.adjoint[1], pa.adjoint[2]);
+                        const int mi = si.material >= 0 && si.material < a.nMats ? si.material : 0;
+                        adjoint_vertex(a, b, mi, av, cmul(T, divs(lc, choicePdf)), to_local(b, linc), wl,
+                                       cmul(T, divs(bc, choicePdf)), to_local(b, bwi), wbMis, sub(Ltot, E),
+                                       to_local(b, wi), to_local(b, si.outgoing), lds, adj.grad);
+                }
                 if (pdf != 0.0f && !isnan(pdf)) {
                         specular = bx_specular<kAll>(b.b);
                         T = cmul(T, divs(mul(f, absdot(wi, si.normal)), pdf));
@@ -499,6 +589,35 @@ __global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__r
                 stats[3 * i] += w.interior;
                 stats[3 * i + 1] += w.tris;
                 stats[3 * i + 2] += 1;
+        }
+}
+
+template <bool kStats, bool kAll>
+__global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__restrict__ paths, int64_t n,
+                                                          uint32_t *__restrict__ stats) {
+        const int64_t i = (int64_t)blockIdx.x * kMisBlock + threadIdx.x;
+        if (i < n) bounce_one<kStats, kAll, false>(a, paths, i, stats, AdjArgs{nullptr, nullptr}, nullptr);
+}
+
+// The replayed bounce with gradients; per-block LDS accumulators when the material table fits
+// (one global atomic per touched entry per block instead of one per vertex).
+template <bool kAll>
+__global__ __launch_bounds__(kMisBlock) void k_mis_adjoint(MisArgs a, GzPath *__restrict__ paths, int64_t n,
+                                                           AdjArgs adj) {
+        __shared__ double sg[kAdjLdsMats * 4];
+        const bool useLds = a.nMats <= kAdjLdsMats;
+        if (useLds) {
+                for (int k = threadIdx.x; k < a.nMats * 4; k += kMisBlock) sg[k] = 0.0;
+                __syncthreads();
+        }
+        const int64_t i = (int64_t)blockIdx.x * kMisBlock + threadIdx.x;
+        if (i < n) bounce_one<false, kAll, true>(a, paths, i, nullptr, adj, useLds ? sg : nullptr);
+        if (useLds) {
+                __syncthreads();
+                for (int k = threadIdx.x; k < a.nMats * 4; k += kMisBlock) {
+                        const double v = sg[k];
+                        if (v != 0.0) atomicAdd(&adj.grad[k], v);
+                }
         }
 }
 
@@ -745,4 +864,32 @@ extern "C" int gz_gpu_mis_trace_stats_device(void *handle, GzPath *d_paths, int6
                 return GZ_E_INVALID;
         }
         return mis_launch(handle, d_paths, count, params, d_stats, stream, true, "gz_gpu_mis_trace_stats_device");
+}
+
+extern "C" int gz_gpu_mis_adjoint_device(void *handle, GzPath *d_paths, const GzPathAdjoint *d_adjoint,
+                                         int64_t count, const GzMisParams *params, double *d_grad, void *stream) {
+        clear_error();
+        if (!handle || count < 0 || (count > 0 && (!d_paths || !d_adjoint || !d_grad))) {
+                set_error("gz_gpu_mis_adjoint_device: bad arguments");
+                return GZ_E_INVALID;
+        }
+        Scene *s = static_cast<Scene *>(handle);
+        std::lock_guard<std::mutex> lock(s->mu);
+        if (!enter_device(s)) return GZ_E_HIP;
+        MisArgs a;
+        if (!mis_args(s, params, a)) return GZ_E_INVALID;
+        if (count == 0) return GZ_OK;
+        const int64_t blocks = (count + kMisBlock - 1) / kMisBlock;
+        if (blocks > 0x7fffffffLL) {
+                set_error("gz_gpu_mis_adjoint_device: %lld paths exceed one launch", (long long)count);
+                return GZ_E_INVALID;
+        }
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        const AdjArgs adj{d_adjoint, d_grad};
+        auto kernel = s->mis->allDiffuse ? k_mis_adjoint<false> : k_mis_adjoint<true>;
+        for (int b = 0; b <= params->maxDepth; ++b) {
+                hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(kMisBlock), 0, st, a, d_paths, count, adj);
+                if (!hip_ok(hipGetLastError(), "MIS adjoint launch")) return GZ_E_HIP;
+        }
+        return GZ_OK;
 }

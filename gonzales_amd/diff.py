@@ -1,0 +1,111 @@
+"""Differentiable rendering: a torch.autograd.Function over the GPU MIS integrator whose backward
+pass is the path-replay adjoint kernel (gz_gpu_mis_adjoint_device).
+
+Parameters are the reflectance of diffuse materials and the (isotropic) roughness of conductor
+materials of the scene's GzMaterial table. forward renders the image (camera -> bounces -> film)
+and keeps the paths' starting records and radiance; backward turns dLoss/dImage into each path
+sample's radiance adjoint (the film divides the sum of a pixel's samples, scaled by iso/100, by
+the sum of their filter weights) and replays every path to accumulate dLoss/dtheta in float64.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import abi
+from .camera import DeviceFilm, PerspectiveCamera
+from .integrator import MisOptions, VolumePathIntegrator
+
+
+class DiffRenderer:
+    """Renders a full-frame camera (no crop) with the parameters written into the material table."""
+
+    def __init__(self, bh, camera: PerspectiveCamera, materials: np.ndarray, options: MisOptions, sky=None,
+                 device="cuda"):
+        import torch
+        c = camera.c
+        if (c.x0, c.y0, c.x1, c.y1) != (0, 0, c.resX, c.resY):
+            raise ValueError("DiffRenderer renders whole frames (no crop)")
+        self.bh, self.camera, self.options, self.sky = bh, camera, options, sky
+        self.materials = np.ascontiguousarray(materials, abi.GZ_MATERIAL).copy()
+        self.integrator = VolumePathIntegrator(bh, options, sky=sky, materials=self.materials)
+        self.film = DeviceFilm(camera, device)
+        n = camera.path_count
+        self.n = n
+        self.d_start = torch.empty(n * abi.GZ_PATH.itemsize, dtype=torch.uint8, device=device)
+        self.d_paths = torch.empty_like(self.d_start)
+        self.device = device
+
+    def set_parameters(self, albedo=None, roughness=None) -> None:
+        """albedo: (M, 3) reflectances (used where the material is diffuse); roughness: (M,)
+        (used where the material is a conductor)."""
+        m = self.materials
+        if albedo is not None:
+            a = np.asarray(albedo, np.float32).reshape(len(m), 3)
+            sel = m["type"] == abi.GZ_MAT_DIFFUSE
+            m["reflectance"][sel] = a[sel]
+        if roughness is not None:
+            r = np.asarray(roughness, np.float32).reshape(len(m))
+            sel = m["type"] == abi.GZ_MAT_CONDUCTOR
+            m["roughness"][sel] = np.stack([r[sel], r[sel]], 1)
+        self.integrator = VolumePathIntegrator(self.bh, self.options, sky=self.sky, materials=m)
+
+    def render(self):
+        """Forward pass: returns the (H, W, 3) image (a torch tensor on the device)."""
+        import torch
+        self.camera.paths_device(self.d_start)
+        self.d_paths.copy_(self.d_start)
+        self.integrator.trace(self.d_paths)
+        self.film.clear()
+        self.film.add(self.d_paths, self.n)
+        img = self.film.resolve()
+        c = self.camera.c
+        return img.view(c.resY, c.resX, 3).clone()
+
+    def path_radiance(self):
+        """(n, 3) radiance of each path sample of the last render (a strided view of the records)."""
+        import torch
+        rec = self.d_paths.view(self.n, abi.GZ_PATH.itemsize)
+        off = abi.GZ_PATH.fields["estimate"][1]
+        return rec[:, off:off + 12].contiguous().view(torch.float32).view(self.n, 3)
+
+    def backward(self, grad_image):
+        """dLoss/dtheta for dLoss/dImage (H, W, 3) of the last render: (grad_albedo (M, 3),
+        grad_roughness (M,)) as float64 torch tensors."""
+        import torch
+        spp = self.camera.spp
+        k = self.camera.film.iso / 100.0
+        g = grad_image.reshape(-1, 3).to(torch.float32)
+        w = self.film.weight
+        per_pixel = g * (k / w)[:, None]
+        adj = torch.empty(self.n, 6, dtype=torch.float32, device=self.device)
+        adj[:, :3] = self.path_radiance()
+        adj[:, 3:] = per_pixel.repeat_interleave(spp, dim=0)
+        grad = torch.zeros(len(self.materials) * 4, dtype=torch.float64, device=self.device)
+        self.d_paths.copy_(self.d_start)
+        prm = self.options.params()
+        abi.check(abi.load().gz_gpu_mis_adjoint_device(self.bh.handle, abi.ptr(self.d_paths), abi.ptr(adj), self.n,
+                                                        ctypes.byref(prm), abi.ptr(grad), None),
+                  "gz_gpu_mis_adjoint_device")
+        grad = grad.view(-1, 4)
+        return grad[:, :3].clone(), grad[:, 3].clone()
+
+
+def render_function():
+    """torch.autograd.Function: image = f(albedo (M, 3), roughness (M,)) for a DiffRenderer."""
+    import torch
+
+    class RenderFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, albedo, roughness, renderer: DiffRenderer):
+            renderer.set_parameters(albedo.detach().cpu().numpy(), roughness.detach().cpu().numpy())
+            ctx.renderer = renderer
+            return renderer.render()
+
+        @staticmethod
+        def backward(ctx, grad_image):
+            ga, gr = ctx.renderer.backward(grad_image)
+            return ga.to(torch.float32), gr.to(torch.float32), None
+
+    return RenderFunction

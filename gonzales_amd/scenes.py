@@ -245,6 +245,39 @@ def fbm(p: np.ndarray, seed: int, octaves: int = 4, lacunarity: float = 2.0, gai
     return out
 
 
+def inverse_rendering_scene(subdivisions: int = 6, n_params: int = 32, seed: int = 8, patch: int = 64):
+    """The Cornell box plus two displaced icospheres (occluders; 2 x 20 x 4^s triangles, 163,840
+    at s = 6) whose 64-face patches cycle through n_params diffuse materials (parameter: RGB
+    reflectance) and n_params conductor materials (parameter: GGX roughness). Material table:
+    the 3 walls (white, red, green), then the diffuse, then the conductor materials. Returns
+    (hierarchy, GzMaterial table, target reflectances (n_params, 3), target roughness (n_params,)),
+    targets drawn with `seed`: reflectance U[0.1, 0.9], roughness U[0.05, 0.5]."""
+    from .integrator import conductor_material, diffuse_material
+    scene, prims, _ = cornell_box()
+    meshes = list(scene.meshes)
+    parts = [prims]
+    for k, (c, r) in enumerate((((-0.42, 0.45, -0.25), 0.36), ((0.40, 0.36, 0.30), 0.3))):
+        v, f = icosphere(subdivisions)
+        v = v * (1 + 0.03 * fbm(v, 5 + k))[:, None]
+        v = v * r + np.asarray(c)
+        mi = len(meshes)
+        meshes.append(TriangleMesh.from_xyz(v.astype(np.float32), f.reshape(-1)))
+        patch_id = np.arange(len(f)) // patch + k * 7
+        parts.append(Primitives.geometric(mi, len(f), 3 + (patch_id % (2 * n_params))))
+    g = _rng(seed)
+    target_albedo = g.uniform(0.1, 0.9, (n_params, 3)).astype(np.float32)
+    target_rough = g.uniform(0.05, 0.5, n_params).astype(np.float32)
+    walls = [diffuse_material(tuple(scene.materials["albedo"][i])) for i in range(3)]
+    mats = walls + [diffuse_material(a) for a in target_albedo] + [conductor_material(roughness=r)
+                                                                    for r in target_rough]
+    gz = abi.concat(mats, abi.GZ_MATERIAL)
+    placeholder = np.zeros(len(gz), abi.MATERIAL)
+    placeholder["type"] = 1
+    placeholder["albedo"] = 0.5
+    new_scene = Scene(meshes, placeholder, light_emission=scene.light_emission)
+    return build_bounding_hierarchy(new_scene, Primitives.concat(parts)), gz, target_albedo, target_rough
+
+
 def displaced_sphere_scene(subdivisions: int = 9, amplitude: float = 0.05, seed_noise: int = 5,
                            seed_mat: int = 6, patch: int = 64):
     """Icosphere (r = 1, `subdivisions` levels) displaced along its normal by amplitude x fBm

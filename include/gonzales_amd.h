@@ -365,6 +365,24 @@ struct GzPathStats {
 int gz_gpu_mis_trace_stats_device(void *handle, struct GzPath *d_paths, int64_t count,
                                   const struct GzMisParams *params, struct GzPathStats *d_stats, void *stream);
 
+/* Path-replay adjoint input of one path: its forward radiance (the estimate the forward trace
+ * left) and dLoss/d(radiance) of that path sample. */
+struct GzPathAdjoint {
+        float radiance[3];
+        float adjoint[3];
+};
+
+/* Path-replay backward pass (no counterpart in the reference, which has no derivatives):
+ * replays every bounce of the `count` paths from the same starting records the forward trace
+ * used (same samples, streams and decisions) and adds dLoss/dtheta to d_grad, a
+ * [materialCount][4] double array (zero it first): entries 0..2 the reflectance of
+ * GZ_MAT_DIFFUSE materials, entry 3 the (isotropic) roughness of GZ_MAT_CONDUCTOR materials
+ * with roughness > 1e-3. Directions, MIS weights and Russian-roulette decisions are held fixed
+ * (detached sampling). The path records are overwritten as by gz_gpu_mis_trace_device.
+ * Asynchronous on `stream`. */
+int gz_gpu_mis_adjoint_device(void *handle, struct GzPath *d_paths, const struct GzPathAdjoint *d_adjoint,
+                              int64_t count, const struct GzMisParams *params, double *d_grad, void *stream);
+
 /* ---------------------------------------------------------------------------------------- */
 /* Camera samples, primary rays and film (Core/Tile.swift:27-59, :170-219; Core/Film.swift).  */
 /* ---------------------------------------------------------------------------------------- */
@@ -486,6 +504,7 @@ static_assert(sizeof(struct GzPath) == 112, "GzPath layout");
 static_assert(sizeof(struct GzMisParams) == 32, "GzMisParams layout");
 static_assert(sizeof(struct GzPathStats) == 12, "GzPathStats layout");
 static_assert(sizeof(struct GzCamera) == 200, "GzCamera layout");
+static_assert(sizeof(struct GzPathAdjoint) == 24, "GzPathAdjoint layout");
 #endif
 
 #endif /* GONZALES_AMD_H */
