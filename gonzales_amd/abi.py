@@ -95,6 +95,11 @@ class GzZSobolParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_int64), ("log2SamplesPerPixel", ctypes.c_int32), ("nBase4Digits", ctypes.c_int32)]
 
 
+class GzMisLaunch(ctypes.Structure):
+    _fields_ = [("queued", ctypes.c_int32), ("block", ctypes.c_int32), ("gridScale", ctypes.c_int32),
+                ("chunk", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
+
+
 class GzMisParams(ctypes.Structure):
     _fields_ = [("maxDepth", ctypes.c_int32), ("flags", ctypes.c_int32), ("zsSeed", ctypes.c_int64),
                 ("zsLog2Spp", ctypes.c_int32), ("zsBase4Digits", ctypes.c_int32), ("rrSeed", ctypes.c_uint64)]
@@ -107,6 +112,18 @@ class GzCamera(ctypes.Structure):
                 ("filterType", ctypes.c_int32), ("filterRadius", ctypes.c_float * 2), ("filterSigma", ctypes.c_float),
                 ("lensRadius", ctypes.c_float), ("focalDistance", ctypes.c_float), ("pad", ctypes.c_int32),
                 ("zsSeed", ctypes.c_int64), ("zsLog2Spp", ctypes.c_int32), ("zsBase4Digits", ctypes.c_int32)]
+
+
+class GzBatchScene(ctypes.Structure):
+    _fields_ = [("handle", ctypes.c_void_p), ("camera", GzCamera), ("params", GzMisParams), ("iso", ctypes.c_float),
+                ("maxComponentValue", ctypes.c_float), ("d_light", ctypes.c_void_p), ("d_weight", ctypes.c_void_p),
+                ("d_albedo", ctypes.c_void_p), ("d_normal", ctypes.c_void_p)]
+
+
+class GzBatchInfo(ctypes.Structure):
+    _fields_ = [("totalPaths", ctypes.c_int64), ("rounds", ctypes.c_int32), ("items", ctypes.c_int32),
+                ("roundPaths", ctypes.c_int32), ("maxItemsPerRound", ctypes.c_int32), ("levels", ctypes.c_int32),
+                ("allBsdfs", ctypes.c_int32)]
 
 
 GZ_FILTER_GAUSSIAN = 0
@@ -152,9 +169,16 @@ SIGNATURES = {
     "gz_gpu_mis_trace_device": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
     "gz_gpu_mis_trace_stats_device": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
     "gz_gpu_mis_adjoint_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P]),
+    "gz_gpu_set_mis_launch": (ctypes.c_int, [_P, _P]),
+    "gz_gpu_get_mis_launch": (ctypes.c_int, [_P, _P]),
     "gz_gpu_camera_paths_device": (ctypes.c_int, [_P, _I64, _I64, _P, _P]),
     "gz_gpu_film_add_device": (ctypes.c_int, [_P, _P, _I64, ctypes.c_float, ctypes.c_float, _P, _P, _P, _P, _P]),
     "gz_gpu_film_resolve_device": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
+    "gz_batch_create": (ctypes.c_void_p, [_P, ctypes.c_int32, _I64]),
+    "gz_batch_info": (ctypes.c_int, [_P, _P]),
+    "gz_batch_set_launch": (ctypes.c_int, [_P, _P]),
+    "gz_batch_render_device": (ctypes.c_int, [_P, ctypes.c_int32, _P]),
+    "gz_batch_free": (None, [_P]),
 }
 
 _lib = None

@@ -1,4 +1,4 @@
-"""Launch-configuration autotuner for the traversal kernels.
+"""Launch-configuration autotuners for the traversal kernels and the MIS path trace.
 
 The results of a traversal never depend on its launch configuration (every configuration runs
 the same per-ray algorithm); only the speed does, and the best choice depends on the scene and on
@@ -97,8 +97,56 @@ def autotune(bh, d_rays, d_tmax, any_hit: bool = False, sample: int = 1 << 21, r
     return best, timings
 
 
+@dataclass(frozen=True)
+class MisLaunch:
+    queued: int = 1
+    block: int = 256
+    grid_scale: int = 1
+    chunk: int = 256
+
+
+MIS_CANDIDATES = (
+    MisLaunch(0, 256, 1, 256),
+    MisLaunch(1, 256, 1, 64),
+    MisLaunch(1, 256, 1, 256),
+    MisLaunch(1, 256, 1, 1024),
+    MisLaunch(1, 256, 2, 256),
+    MisLaunch(1, 128, 1, 256),
+    MisLaunch(1, 64, 1, 256),
+)
+
+
+def autotune_mis(integrator, d_start, count: int, reps: int = 2, candidates=MIS_CANDIDATES) -> tuple[MisLaunch, dict]:
+    """Times a whole MIS trace of the `count` path records in d_start (a device tensor, left
+    unchanged: every run starts from a copy) under each candidate launch configuration, keeps the
+    fastest on the integrator's scene handle and returns (best, {config: ms})."""
+    import torch
+
+    work = torch.empty_like(d_start)
+    s = torch.cuda.current_stream()
+    timings = {}
+    for cfg in candidates:
+        integrator.set_launch(cfg.queued, cfg.block, cfg.grid_scale, cfg.chunk)
+        total = 0.0
+        for rep in range(reps + 1):
+            work.copy_(d_start)
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(s)
+            integrator.trace(work, count)
+            b.record(s)
+            torch.cuda.synchronize()
+            if rep > 0:  # the first run also grows the queue storage
+                total += a.elapsed_time(b)
+        timings[cfg] = total / reps
+    best = min(timings, key=timings.get)
+    integrator.set_launch(best.queued, best.block, best.grid_scale, best.chunk)
+    return best, timings
+
+
 def describe(timings: dict) -> list[dict]:
     return [dict(asdict(c), ms=round(float(t), 3)) for c, t in sorted(timings.items(), key=lambda kv: kv[1])]
 
 
-__all__ = ["TraceConfig", "CANDIDATES", "autotune", "describe", "scene_signature"]
+__all__ = ["TraceConfig", "CANDIDATES", "autotune", "describe", "scene_signature", "MisLaunch", "MIS_CANDIDATES",
+           "autotune_mis"]

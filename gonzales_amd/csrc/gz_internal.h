@@ -52,10 +52,15 @@ struct DevMesh {
 // Light and material tables of the MIS integrator (gz_mis.hip), set by gz_gpu_set_mis_scene.
 struct MisScene;
 void release_mis(MisScene *m);
+// Device path queues of the MIS trace, per stream (gz_mis.hip).
+struct MisQueues;
+void release_mis_queues(MisQueues *m);
 
 struct Scene {
         std::mutex mu;
         MisScene *mis = nullptr;
+        MisQueues *misQueues = nullptr;
+        GzMisLaunch misLaunch{1, 256, 1, 256, {0, 0, 0, 0}};
         int device = 0;
         hipStream_t stream = nullptr;  // used by the synchronous host-buffer entry points
 

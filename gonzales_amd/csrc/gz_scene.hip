@@ -66,6 +66,7 @@ static void free_scene(Scene *s) {
         if (hipGetDevice(&cur) == hipSuccess && cur != s->device) (void)hipSetDevice(s->device);
         if (s->stream) (void)hipStreamSynchronize(s->stream);
         release_mis(s->mis);
+        release_mis_queues(s->misQueues);
         void *ptrs[] = {s->dNodes, s->dTris, s->dPrims, s->dMeshes, s->dMaterials, s->dErr, s->dStage, s->dCounters};
         for (void *p : ptrs)
                 if (p) (void)hipFree(p);

@@ -1,0 +1,604 @@
+// Device code of the MIS path tracer shared by the single-scene kernels (gz_mis.hip) and the
+// multi-scene batch (gz_batch.hip): the bounce of the reference's VolumePathIntegrator
+// (Sources/libgonzales/Integrator/VolumePathIntegrator.swift:35-463), one path per lane.
+//
+// A bounce of a path, in the reference's order:
+//   closest hit + surface interaction ......... BoundingHierarchy.swift:387-456, Triangle.swift:363-474
+//   miss: infinite light if bounce 0/specular .. VolumePathIntegrator.swift:380-393
+//   stop at maxDepth ........................... :400-402
+//   emission on bounce 0/specular (one-sided) .. :331-340, AreaLight.swift:13-19
+//   BSDF (area-light surfaces use materials[0]) :341-348
+//   one light by power (own sampler at Morton 0) PowerLightSampler.swift:41-49
+//   light sample + shadow ray (MIS weight) ...... :67-89, :173-195, Visibility.swift:5-24
+//   BSDF sample + ray (MIS weight, light pdf 0
+//     for triangles unless corrected) .......... :91-133, :197-211, Sampleable.swift:49-69
+//   continuation, spawn offset 1e-4 n ........... :358-369, Interaction.swift:32-53
+//   Russian roulette after bounce 1 ............. :269-283 (deterministic stream here)
+// Arithmetic follows the reference's operation order without contraction; sqrt and division are
+// IEEE-correct; cos/sin come from the device math library.
+//
+// The shadow ray and the BSDF ray are traced inline with the same traversal as the batch
+// kernels. A BSDF ray toward a chosen triangle light is skipped when its MIS weight is 0 (the
+// reference's light pdf quirk): it would contribute exactly 0.
+#pragma once
+
+#include <cmath>
+
+#include "bsdf_device.h"
+#include "traverse_device.h"
+#include "zsobol_device.h"
+
+#pragma clang fp contract(off)
+
+namespace gz {
+
+const uint32_t *device_sobol_table();  // gz_zsobol.hip
+
+constexpr int kMisBlock = 256;
+// Scene data of the MIS integrator on the device.
+struct DevLight {
+        float4 p0;  // w: area
+        float4 p1;
+        float4 p2;
+        float4 L;   // brightness
+};
+
+struct MisArgs {
+        const NodePair *nodes;
+        const TriRec *tris;
+        int rootDesc;
+        bool sceneSafe;
+        bool empty;
+        const PrimId_C *prims;
+        int64_t primCount;
+        const DevMesh *meshes;
+        int64_t meshCount;
+        const GzMaterial *mats;
+        int nMats;
+        const DevLight *lights;
+        int nLights;
+        const float *power;       // per choice (area lights, then the sky)
+        const float *cumulative;  // running sum of power
+        int nChoices;
+        float totalPower;
+        bool hasSky;
+        float sky[3];
+        const uint32_t *sobol;
+        ZSobolParams zs;
+        int maxDepth;
+        int flags;
+        uint64_t rrSeed;
+        int *err;
+};
+
+// Per-lane traversal state shared by the rays of one bounce: one stack (so the three inline
+// traversals do not each get their own private array) and the work counters (interior visits
+// and triangle tests, all rays).
+struct Work {
+        Stack<kLdsMax> S;
+        uint32_t interior = 0, tris = 0;
+};
+
+// Closest or any hit of one ray (the traversal of the batch kernels, inline).
+template <bool kAny, bool kStats>
+__device__ __forceinline__ bool trace_ray(const MisArgs &a, const Ray_C &ray, float tMax, int &prim, float &t,
+                                          float &u, float &v, Work &w) {
+        if (a.empty) return false;
+        Lane L;
+        lane_init(L, ray, tMax, a.rootDesc, a.sceneSafe);
+        int st;
+        do {
+                st = step<kAny, kStats, kLdsMax>(L, w.S, a.nodes, a.tris);
+        } while (st == kStepGo);
+        if (kStats) {
+                w.interior += L.nInterior;
+                w.tris += L.nTris;
+        }
+        if (st == kStepOverflow) {
+                atomicOr(a.err, 1);
+                return false;
+        }
+        prim = L.hitPrim;
+        t = L.tBest;
+        u = L.bu;
+        v = L.bv;
+        return prim >= 0;
+}
+
+struct Interaction {
+        f3 position, normal, shadingNormal, outgoing, dpdu;
+        int material;
+        int areaLight;
+};
+
+__device__ __forceinline__ bool tri_points(const MisArgs &a, const PrimId_C &pr, f3 &p0, f3 &p1, f3 &p2) {
+        int64_t mesh, base;
+        if (pr.type == 0) {
+                mesh = pr.id1;
+                base = pr.id2;
+        } else {
+                mesh = pr.id2 >> 32;
+                base = (pr.id2 & 0xFFFFFFFFLL) * 3;
+        }
+        if (mesh < 0 || mesh >= a.meshCount) return false;
+        const DevMesh m = a.meshes[mesh];
+        if (base < 0 || base + 2 >= m.vertexIndexCount) return false;
+        const int64_t i0 = m.vertexIndices[base], i1 = m.vertexIndices[base + 1], i2 = m.vertexIndices[base + 2];
+        if (i0 < 0 || i1 < 0 || i2 < 0 || i0 * 4 + 2 >= m.pointFloats || i1 * 4 + 2 >= m.pointFloats ||
+            i2 * 4 + 2 >= m.pointFloats)
+                return false;
+        p0 = mk(m.points[i0 * 4], m.points[i0 * 4 + 1], m.points[i0 * 4 + 2]);
+        p1 = mk(m.points[i1 * 4], m.points[i1 * 4 + 1], m.points[i1 * 4 + 2]);
+        p2 = mk(m.points[i2 * 4], m.points[i2 * 4 + 1], m.points[i2 * 4 + 2]);
+        return true;
+}
+
+// BoundingHierarchy.intersect -> Triangle.computeSurfaceInteraction for meshes without vertex
+// normals or uvs (Triangle.swift:363-474). false where the reference returns nil.
+__device__ bool surface_interaction(const MisArgs &a, const Ray_C &ray, int prim, float u, float v, Interaction &si) {
+        if (prim < 0 || prim >= a.primCount) return false;
+        const PrimId_C pr = a.prims[prim];
+        f3 p0, p1, p2;
+        if (!tri_points(a, pr, p0, p1, p2)) {
+                atomicOr(a.err, 2);
+                return false;
+        }
+        const float b0 = 1.0f - u - v, b1 = u, b2 = v;
+        const f3 dp02 = sub(p0, p2), dp12 = sub(p1, p2);
+        const f3 pHit = add(add(mul(p0, b0), mul(p1, b1)), mul(p2, b2));
+        const f3 n = normalized(cross(dp02, dp12));
+        const f3 gn = cross(sub(p2, p0), sub(p1, p0));
+        if (len2(gn) == 0.0f) return false;
+        f3 dpdu, dpdv;
+        coordinate_system(normalized(gn), dpdu, dpdv);
+        const f3 sn = n;
+        f3 S = normalized(dpdu);
+        f3 T = cross(S, sn);
+        if (len2(T) > 0.0f) {
+                T = normalized(T);
+                S = cross(T, sn);
+        } else {
+                coordinate_system(sn, S, T);
+        }
+        si.position = pHit;
+        si.normal = normalized(n);
+        si.shadingNormal = normalized(sn);
+        si.outgoing = normalized(neg(mk(ray.dirX, ray.dirY, ray.dirZ)));
+        si.dpdu = S;
+        if (pr.type == 3) {
+                si.areaLight = (int)pr.id1;
+                si.material = 0;
+        } else {
+                si.areaLight = -1;
+                si.material = (int)pr.materialIndex;
+        }
+        return true;
+}
+
+// The material's BSDF in its shading frame (BsdfFrame.swift:96-104, ShadingFrame.swift:27-77,
+// FramedBsdf.swift:18-83); the BSDFs themselves are in bsdf_device.h.
+struct Bsdf {
+        f3 x, y, z, ng;
+        Bx b;
+};
+
+__device__ __forceinline__ void bsdf_make(const MisArgs &a, const Interaction &si, Bsdf &b) {
+        b.y = normalized(si.dpdu);
+        b.z = si.shadingNormal;
+        b.x = cross(b.y, b.z);
+        b.ng = si.normal;
+        const int mi = si.material >= 0 && si.material < a.nMats ? si.material : 0;
+        b.b = bx_from_material(a.mats[mi]);
+}
+__device__ __forceinline__ f3 to_local(const Bsdf &b, f3 w) { return normalized(mk(dot(w, b.x), dot(w, b.y), dot(w, b.z))); }
+__device__ __forceinline__ f3 to_world(const Bsdf &b, f3 l) { return add(add(mul(b.x, l.x), mul(b.y, l.y)), mul(b.z, l.z)); }
+// evaluateWorldSpace: every BSDF keeps the protocol's isReflective = true and
+// isTransmissive = false, so only the reflection side is evaluated.
+template <bool kAll>
+__device__ __forceinline__ f3 bsdf_eval_world(const Bsdf &b, f3 wo, f3 wi, LayerRng &r) {
+        const f3 ol = to_local(b, wo), il = to_local(b, wi);
+        const bool reflect = dot(wi, b.ng) * dot(wo, b.ng) > 0.0f;
+        return reflect ? bx_f<kAll>(b.b, ol, il, r) : mk(0.0f, 0.0f, 0.0f);
+}
+template <bool kAll>
+__device__ __forceinline__ float bsdf_pdf_world(const Bsdf &b, f3 wo, f3 wi, LayerRng &r) {
+        const f3 il = to_local(b, wi), ol = to_local(b, wo);
+        if (ol.z == 0.0f) return 0.0f;
+        return bx_pdf<kAll>(b.b, ol, il, r);
+}
+template <bool kAll>
+__device__ __forceinline__ void bsdf_sample_world(const Bsdf &b, f3 wo, float u0, float u1, float u2, f3 &f, f3 &wi,
+                                                  float &pdf, LayerRng &r) {
+        const f3 ol = to_local(b, wo);
+        const BSample s = bx_sample<kAll>(b.b, ol, u0, u1, u2, r);
+        f = s.f;
+        wi = to_world(b, s.wi);
+        pdf = s.pdf;
+}
+
+// Streams of the layered BSDF for the BSDF calls of one bounce.
+enum : int { kCallLightF = 1, kCallLightPdf = 2, kCallMisSample = 3, kCallContinue = 4 };
+
+// PowerLightSampler.chooseLight + lowerBound (PowerLightSampler.swift:4-49)
+__device__ __forceinline__ int choose_light(const MisArgs &a, float u, float &pdf) {
+        const float key = u * a.totalPower;
+        int lo = 0, hi = a.nChoices, idx = -1;
+        while (lo < hi) {
+                const int mid = lo + (hi - lo) / 2;
+                const float cm = a.cumulative[mid];
+                if (cm == key) {
+                        idx = mid;
+                        break;
+                }
+                if (cm < key)
+                        lo = mid + 1;
+                else
+                        hi = mid;
+        }
+        if (idx < 0) idx = lo;
+        if (idx >= a.nChoices) idx = a.nChoices - 1;
+        pdf = a.power[idx] / a.totalPower;
+        return idx;
+}
+
+// InfiniteLight.equalAreaSquareToSphere (InfiniteLight.swift:112-127); `unit` applies the factor
+// sqrt(2 - r^2) to x and y that the reference omits (GZ_MIS_FIX_SKY_DIRECTION).
+__device__ __forceinline__ f3 equal_area_square_to_sphere(float tu, float tv, bool unit) {
+        const float u = 2.0f * tu - 1.0f, v = 2.0f * tv - 1.0f;
+        const float au = fabsf(u), av = fabsf(v);
+        const float sd = 1.0f - (au + av);
+        const float ad = fabsf(sd);
+        const float r = 1.0f - ad;
+        const float phi = (r == 0.0f ? 1.0f : (av - au) / r + 1.0f) * (kPi / 4.0f);
+        const float zz = 1.0f - r * r;
+        const float z = copysignf(zz > 0.0f ? zz : 0.0f, sd);
+        const float cp = cosf(phi), sp = sinf(phi);
+        float rr = r;
+        if (unit) {
+                const float s = 2.0f - r * r;
+                rr = r * sqrtf(s > 0.0f ? s : 0.0f);
+        }
+        return mk(copysignf(rr * cp, u), copysignf(rr * sp, v), z);
+}
+
+struct LightSample {
+        f3 L, dir, target;
+        float pdf;
+};
+
+// AreaLight.sample (AreaLight.swift:21-32) with Triangle.sample and Sampleable.sample(point:)
+// (Triangle.swift:517-544, Sampleable.swift:33-44), or the constant sky (InfiniteLight.swift:39-65).
+__device__ __forceinline__ void light_sample(const MisArgs &a, int choice, f3 point, float u0, float u1, LightSample &ls) {
+        if (choice < a.nLights) {
+                const DevLight dl = a.lights[choice];
+                const f3 p0 = mk(dl.p0.x, dl.p0.y, dl.p0.z), p1 = mk(dl.p1.x, dl.p1.y, dl.p1.z);
+                const f3 p2 = mk(dl.p2.x, dl.p2.y, dl.p2.z);
+                const float su0 = sqrtf(u0);
+                const float b0 = 1.0f - su0, b1 = u1 * su0;
+                const f3 pos = add(add(mul(p0, b0), mul(p1, b1)), mul(p2, 1.0f - b0 - b1));
+                const f3 ln = normalized(cross(sub(p1, p0), sub(p2, p0)));
+                float pdf = 1.0f / dl.p0.w;
+                const f3 inc = normalized(sub(pos, point));
+                const float d2 = len2(sub(point, pos));
+                const float angle = absdot(ln, neg(inc));
+                pdf = pdf * (d2 / angle);
+                ls.dir = normalized(sub(pos, point));
+                ls.target = pos;
+                ls.L = dot(ln, neg(ls.dir)) > 0.0f ? mk(dl.L.x, dl.L.y, dl.L.z) : mk(0.0f, 0.0f, 0.0f);
+                ls.pdf = pdf;
+                return;
+        }
+        const f3 d = equal_area_square_to_sphere(u0, u1, (a.flags & GZ_MIS_FIX_SKY_DIRECTION) != 0);
+        ls.dir = d;
+        ls.pdf = 1.0f / (4.0f * kPi);
+        ls.target = add(point, mul(d, 100.0f));
+        ls.L = mk(a.sky[0], a.sky[1], a.sky[2]);
+}
+
+// Visibility.occluded via spawnRay(from:target:) (Visibility.swift:5-24, Interaction.swift:15-24):
+// origin from + 1e-4 (target - from), unnormalised direction, tMax 1 - 1e-5.
+template <bool kStats>
+__device__ __forceinline__ bool occluded(const MisArgs &a, f3 from, f3 target, Work &w) {
+        const f3 o = add(from, mul(sub(target, from), 0.0001f));
+        const f3 d = sub(target, o);
+        Ray_C r{o.x, o.y, o.z, d.x, d.y, d.z};
+        int prim;
+        float t, u, v;
+        return trace_ray<true, kStats>(a, r, 1.0f - 0.00001f, prim, t, u, v, w);
+}
+
+// Interaction.spawnRay(inDirection:) (Interaction.swift:32-53)
+__device__ __forceinline__ Ray_C spawn(const Interaction &si, f3 dir) {
+        const f3 off = mul(si.normal, 0.0001f);
+        const f3 o = dot(dir, si.normal) > 0.0f ? add(si.position, off) : add(si.position, neg(off));
+        return Ray_C{o.x, o.y, o.z, dir.x, dir.y, dir.z};
+}
+
+// Light pdf of a BSDF-sampled direction (Sampleable.swift:49-69): 0 for triangles in the
+// reference (tHit = 0); with GZ_MIS_FIX_LIGHT_PDF the chosen triangle is intersected.
+__device__ float light_pdf(const MisArgs &a, int choice, const Interaction &si, f3 dir) {
+        if (choice >= a.nLights) return 1.0f / (4.0f * kPi);
+        if (!(a.flags & GZ_MIS_FIX_LIGHT_PDF)) return 0.0f;
+        const DevLight dl = a.lights[choice];
+        const f3 p0 = mk(dl.p0.x, dl.p0.y, dl.p0.z), p1 = mk(dl.p1.x, dl.p1.y, dl.p1.z);
+        const f3 p2 = mk(dl.p2.x, dl.p2.y, dl.p2.z);
+        const Ray_C r = spawn(si, dir);
+        const f3 e1 = sub(p1, p0), e2 = sub(p2, p0);
+        const f3 d = mk(r.dirX, r.dirY, r.dirZ), o = mk(r.orgX, r.orgY, r.orgZ);
+        const f3 pv = cross(d, e2);
+        const float det = dot(e1, pv);
+        if (det > -1e-12f && det < 1e-12f) return 0.0f;
+        const float inv = 1.0f / det;
+        const f3 tv = sub(o, p0);
+        const float uu = dot(tv, pv) * inv;
+        if (uu < 0.0f || uu > 1.0f) return 0.0f;
+        const f3 qv = cross(tv, e1);
+        const float vv = dot(d, qv) * inv;
+        if (vv < 0.0f || uu + vv > 1.0f) return 0.0f;
+        const float t = dot(e2, qv) * inv;
+        if (t <= 0.0f) return 0.0f;
+        const f3 hitp = add(o, mul(d, t));
+        const float d2 = len2(sub(si.position, hitp));
+        const f3 ln = normalized(cross(sub(p0, p2), sub(p1, p2)));
+        const float angle = absdot(ln, neg(dir));
+        const float density = d2 / (angle * dl.p0.w);
+        return isinf(density) ? 0.0f : density;
+}
+
+// Deterministic Russian-roulette draw: top 24 bits of SplitMix64(seed, pixel, sample, bounce).
+__device__ __forceinline__ float rr_draw(uint64_t seed, int px, int py, int si, int bounce) {
+        uint64_t z = seed ^ ((uint64_t)(uint32_t)px << 40) ^ ((uint64_t)(uint32_t)py << 20) ^ (uint64_t)(uint32_t)si;
+        z += 0x9E3779B97F4A7C15ull * (uint64_t)(uint32_t)(bounce + 1);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z = z ^ (z >> 31);
+        return (float)(uint32_t)(z >> 40) * (1.0f / 16777216.0f);
+}
+
+// Path-replay adjoint state (gz_gpu_mis_adjoint_device): per path its forward radiance and the
+// adjoint of its radiance; gradients accumulate per material as (dL/dR r, g, b, dL/droughness).
+struct AdjArgs {
+        const GzPathAdjoint *paths;
+        double *grad;  // [materials][4] in global memory
+};
+constexpr int kAdjLdsMats = 128;
+
+// Log-derivatives in alpha of an isotropic Trowbridge-Reitz conductor (MicrofacetReflection.swift:
+// 15-26, TrowbridgeReitzDistribution.swift:9-26) for local directions wo, wi:
+//   f:   d ln D(h) + d ln G(wo, wi)    (F and the cosines do not depend on alpha)
+//   pdf: d ln D(h) + d ln G1(wo)       (pdf = D G1(wo) / (4 |cos wo|))
+// D = 1 / (pi a^2 cos^4 (1 + t/a^2)^2), t = tan^2(theta_h); Lambda = (sqrt(1 + a^2 tan^2) - 1) / 2.
+struct DlogAlpha {
+        float f, pdf;
+};
+__device__ __forceinline__ DlogAlpha dlog_dalpha(float al, f3 wo, f3 wi) {
+        f3 h = add(wo, wi);
+        if (h.x == 0.0f && h.y == 0.0f && h.z == 0.0f) return DlogAlpha{0.0f, 0.0f};
+        h = normalized(h);
+        const float t = tan2_theta(h);
+        float dD = -2.0f / al;
+        if (!isinf(t)) dD += 4.0f * t / (al * (al * al + t));
+        float lam[2], dlam[2];
+        const f3 ws[2] = {wo, wi};
+        for (int k = 0; k < 2; ++k) {
+                const float t2 = tan2_theta(ws[k]);
+                if (isinf(t2)) {
+                        lam[k] = 0.0f;
+                        dlam[k] = 0.0f;
+                        continue;
+                }
+                const float s = sqrtf(1.0f + al * al * t2);
+                lam[k] = (s - 1.0f) / 2.0f;
+                dlam[k] = al * t2 / (2.0f * s);
+        }
+        return DlogAlpha{dD - (dlam[0] + dlam[1]) / (1.0f + lam[0] + lam[1]), dD - dlam[0] / (1.0f + lam[0])};
+}
+
+// Adjoint of one bounce's BSDF at a vertex of material m: the light-strategy term, the
+// BSDF-strategy term and everything after the vertex (cont) each carry the BSDF once; the
+// derivative of the path's radiance is each term times d ln f / d theta at its direction, plus,
+// for the two MIS terms, the derivative of their power-heuristic weights through the BSDF pdf
+// (the reference's weights need not sum to one: its light pdf of a BSDF direction toward a
+// triangle is 0). Sampled directions and Russian roulette are held fixed (detached sampling).
+__device__ __forceinline__ void adjoint_vertex(const MisArgs &a, const Bsdf &b, int m, f3 adj, f3 termL, f3 wl,
+                                               float misL, f3 termB, f3 wb, float misB, f3 cont, f3 wc, f3 wo,
+                                               double *lds, double *grad) {
+        double g[4] = {0.0, 0.0, 0.0, 0.0};
+        bool any = false;
+        const GzMaterial &mat = a.mats[m];
+        if (b.b.type == GZ_MAT_DIFFUSE) {
+                const f3 s = add(add(termL, termB), cont);
+                const float R[3] = {b.b.R.x, b.b.R.y, b.b.R.z};
+                const float S[3] = {s.x, s.y, s.z}, A[3] = {adj.x, adj.y, adj.z};
+                for (int c = 0; c < 3; ++c)
+                        if (R[c] != 0.0f) g[c] = (double)A[c] * (double)S[c] / (double)R[c];
+                any = true;
+        } else if (b.b.type == GZ_MAT_CONDUCTOR && mat.roughness[0] == mat.roughness[1] && mat.roughness[0] > 1e-3f) {
+                const float al = b.b.d.ax;
+                float rl = 0.0f, rb = 0.0f, rc = 0.0f;
+                if (!cblack(termL)) {  // w_l = pl^2 / (pl^2 + pb^2): d ln w_l = -2 (1 - w_l) d ln pb
+                        const DlogAlpha d = dlog_dalpha(al, wo, wl);
+                        rl = d.f - 2.0f * (1.0f - misL) * d.pdf;
+                }
+                if (!cblack(termB)) {  // w_b = pb^2 / (pb^2 + pl^2): d ln w_b = 2 (1 - w_b) d ln pb
+                        const DlogAlpha d = dlog_dalpha(al, wo, wb);
+                        rb = d.f + 2.0f * (1.0f - misB) * d.pdf;
+                }
+                if (!cblack(cont)) rc = dlog_dalpha(al, wo, wc).f;
+                const f3 s = add(add(mul(termL, rl), mul(termB, rb)), mul(cont, rc));
+                g[3] = (double)adj.x * s.x + (double)adj.y * s.y + (double)adj.z * s.z;
+                any = true;
+        }
+        if (!any) return;
+        double *dst = lds ? lds : grad;
+        for (int k = 0; k < 4; ++k)
+                if (g[k] != 0.0) atomicAdd(&dst[4 * m + k], g[k]);
+}
+
+// One bounce of path i. kStats: also adds the bounce's traversal work (all its rays) to
+// stats[i] and counts the bounce (interior, tris, bounces). kAdj: replays the bounce of the
+// forward pass and adds the gradient of the path's radiance (lds: per-block accumulator).
+// One bounce of path i; returns whether the path goes on to another bounce.
+template <bool kStats, bool kAll, bool kAdj>
+__device__ __forceinline__ bool bounce_one(const MisArgs &a, GzPath *__restrict__ paths, int64_t i,
+                                           uint32_t *__restrict__ stats, const AdjArgs &adj, double *lds) {
+        GzPath p = paths[i];
+        if (!(p.flags & GZ_PATH_ACTIVE)) return false;
+        if (p.bounce > a.maxDepth) {
+                paths[i].flags = p.flags & ~GZ_PATH_ACTIVE;
+                return false;
+        }
+        Work w;
+        f3 T = mk(p.throughput[0], p.throughput[1], p.throughput[2]);
+        f3 E = mk(p.estimate[0], p.estimate[1], p.estimate[2]);
+        bool specular = (p.flags & GZ_PATH_SPECULAR) != 0;
+        bool go = false;
+        int prim = -1;
+        float th, hu, hv;
+        Interaction si;
+        const bool hit = trace_ray<false, kStats>(a, p.ray, INFINITY, prim, th, hu, hv, w) &&
+                         surface_interaction(a, p.ray, prim, hu, hv, si);
+        ZSobolState zs;
+        zs.morton = (a.zs.log2spp >= 64 ? 0ull : zs_morton2((uint32_t)p.pixelX, (uint32_t)p.pixelY) << a.zs.log2spp) |
+                    (uint64_t)(uint32_t)p.sampleIndex;
+        zs.dim = p.dim;
+        ZSobolState lzs;
+        lzs.morton = 0;  // the light chooser's sampler never starts a pixel sample
+        lzs.dim = p.lightDim;
+        if (!hit) {
+                if ((p.bounce == 0 || specular) && a.hasSky) E = cadd(E, cmul(T, mk(a.sky[0], a.sky[1], a.sky[2])));
+        } else if (!cblack(T) && p.bounce < a.maxDepth) {
+                if ((p.bounce == 0 || specular) && si.areaLight >= 0 && si.areaLight < a.nLights) {
+                        const float4 Lb = a.lights[si.areaLight].L;
+                        const f3 Le = dot(si.normal, si.outgoing) > 0.0f ? mk(Lb.x, Lb.y, Lb.z) : mk(0.0f, 0.0f, 0.0f);
+                        E = cadd(E, cmul(T, Le));
+                }
+                Bsdf b;
+                bsdf_make(a, si, b);
+                if (p.bounce == 0) {
+                        const f3 al = bx_albedo<kAll>(b.b);
+                        p.albedo[0] = al.x;
+                        p.albedo[1] = al.y;
+                        p.albedo[2] = al.z;
+                        p.firstNormal[0] = si.normal.x;
+                        p.firstNormal[1] = si.normal.y;
+                        p.firstNormal[2] = si.normal.z;
+                }
+                // sampleOneLight -> estimateDirect -> sampleMultipleImportance
+                float choicePdf;
+                const float ul = zs_get1d(a.zs, lzs, a.sobol);
+                const int choice = choose_light(a, ul, choicePdf);
+                float u0, u1;
+                zs_get2d(a.zs, zs, a.sobol, u0, u1);
+                LightSample ls;
+                light_sample(a, choice, si.position, u0, u1, ls);
+                f3 Lest = mk(0.0f, 0.0f, 0.0f), linc = mk(0.0f, 0.0f, 0.0f);
+                float lpdf = 0.0f;
+                if (!cblack(ls.L) && !isinf(ls.pdf) && !occluded<kStats>(a, si.position, ls.target, w)) {
+                        LayerRng r1 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightF);
+                        const f3 scatter =
+                                mul(bsdf_eval_world<kAll>(b, si.outgoing, ls.dir, r1), absdot(ls.dir, si.shadingNormal));
+                        Lest = cmul(scatter, ls.L);
+                        lpdf = ls.pdf;
+                        linc = ls.dir;
+                }
+                LayerRng r2 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightPdf);
+                const float bpdfL = bsdf_pdf_world<kAll>(b, si.outgoing, linc, r2);
+                const float wl = power_heuristic(lpdf, bpdfL);
+                const f3 lc = lpdf == 0.0f ? mk(0.0f, 0.0f, 0.0f) : divs(mul(Lest, wl), lpdf);
+                float v0, v1, v2;
+                zs_get3d(a.zs, zs, a.sobol, v0, v1, v2);
+                f3 bf, bwi;
+                float bpdf;
+                LayerRng r3 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallMisSample);
+                bsdf_sample_world<kAll>(b, si.outgoing, v0, v1, v2, bf, bwi, bpdf, r3);
+                bf = mul(bf, absdot(bwi, si.shadingNormal));
+                f3 bc = mk(0.0f, 0.0f, 0.0f);
+                float wbMis = 0.0f;
+                const bool weighted = choice >= a.nLights || (a.flags & GZ_MIS_FIX_LIGHT_PDF);
+                if (weighted && !cblack(bf) && bpdf > 0.0f) {
+                        const Ray_C r = spawn(si, bwi);
+                        int bprim = -1;
+                        float bt, bu, bv;
+                        Interaction bsi;
+                        f3 bL = mk(0.0f, 0.0f, 0.0f);
+                        bool ok = false;
+                        if (trace_ray<false, kStats>(a, r, INFINITY, bprim, bt, bu, bv, w) &&
+                            surface_interaction(a, r, bprim, bu, bv, bsi)) {
+                                if (bsi.areaLight >= 0 && bsi.areaLight < a.nLights) {
+                                        const float4 Lb = a.lights[bsi.areaLight].L;
+                                        bL = dot(bsi.normal, bwi) > 0.0f ? mk(Lb.x, Lb.y, Lb.z) : mk(0.0f, 0.0f, 0.0f);
+                                        ok = true;
+                                }
+                        } else if (a.hasSky) {
+                                bL = mk(a.sky[0], a.sky[1], a.sky[2]);
+                                ok = true;
+                        }
+                        if (ok) {
+                                const f3 best = cmul(bf, bL);
+                                const float lden = light_pdf(a, choice, si, bwi);
+                                const float wb = power_heuristic(bpdf, lden);
+                                wbMis = wb;
+                                bc = divs(mul(best, wb), bpdf);
+                        }
+                }
+                const f3 est = cadd(lc, bc);
+                E = cadd(E, cmul(T, divs(est, choicePdf)));
+                // continuation: bsdf.sampleWorldSpace(outgoing:uSample: get3D())
+                float w0, w1, w2;
+                zs_get3d(a.zs, zs, a.sobol, w0, w1, w2);
+                f3 f, wi;
+                float pdf;
+                LayerRng r4 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallContinue);
+                bsdf_sample_world<kAll>(b, si.outgoing, w0, w1, w2, f, wi, pdf, r4);
+                if (kAdj) {
+                        const GzPathAdjoint &pa = adj.paths[i];
+                        const f3 Ltot = mk(pa.radiance[0], pa.radiance[1], pa.radiance[2]);
+                        const f3 av = mk(pa.adjoint[0], pa.adjoint[1], pa.adjoint[2]);
+                        const int mi = si.material >= 0 && si.material < a.nMats ? si.material : 0;
+                        adjoint_vertex(a, b, mi, av, cmul(T, divs(lc, choicePdf)), to_local(b, linc), wl,
+                                       cmul(T, divs(bc, choicePdf)), to_local(b, bwi), wbMis, sub(Ltot, E),
+                                       to_local(b, wi), to_local(b, si.outgoing), lds, adj.grad);
+                }
+                if (pdf != 0.0f && !isnan(pdf)) {
+                        specular = bx_specular<kAll>(b.b);
+                        T = cmul(T, divs(mul(f, absdot(wi, si.normal)), pdf));
+                        p.ray = spawn(si, wi);
+                        go = true;
+                        if (cmax(T) < 1.0f && p.bounce > 1) {
+                                const float m = cmax(T);
+                                const float q = 1.0f - m > 0.0f ? 1.0f - m : 0.0f;
+                                const float u = rr_draw(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce);
+                                if (u < q)
+                                        go = false;
+                                else
+                                        T = divs(T, 1.0f - q);
+                        }
+                }
+        }
+        p.throughput[0] = T.x;
+        p.throughput[1] = T.y;
+        p.throughput[2] = T.z;
+        p.estimate[0] = E.x;
+        p.estimate[1] = E.y;
+        p.estimate[2] = E.z;
+        p.dim = zs.dim;
+        p.lightDim = lzs.dim;
+        p.flags = (go ? GZ_PATH_ACTIVE : 0) | (specular ? GZ_PATH_SPECULAR : 0);
+        if (go) p.bounce += 1;
+        paths[i] = p;
+        if (kStats) {
+                stats[3 * i] += w.interior;
+                stats[3 * i + 1] += w.tris;
+                stats[3 * i + 2] += 1;
+        }
+        return go;
+}
+
+// Host side (gz_mis.hip): the device arguments of a scene's MIS tables and parameters (false
+// with the error set when the handle has none or the parameters are out of range), and whether
+// every material is diffuse (the kernels built without the other BSDFs then suffice).
+bool mis_args(Scene *s, const GzMisParams *params, MisArgs &a);
+bool mis_all_diffuse(const Scene *s);
+
+}  // namespace gz

@@ -348,9 +348,32 @@ int gz_gpu_mis_bounce_device(void *handle, struct GzPath *d_paths, int64_t count
                              const struct GzMisParams *params, void *stream);
 
 /* Runs bounces until every path has stopped (VolumePathIntegrator.bounces, :448-463).
- * Asynchronous on `stream`. */
+ * Asynchronous on `stream`. With a queued launch configuration (the default) bounce 0 runs over
+ * all paths and every later bounce over a device queue of the paths still active; the handle
+ * keeps the queue storage per stream (grow-only: the first call with a larger count allocates,
+ * so make one such call before capturing these launches in a graph). */
 int gz_gpu_mis_trace_device(void *handle, struct GzPath *d_paths, int64_t count,
                             const struct GzMisParams *params, void *stream);
+
+/* Launch configuration of gz_gpu_mis_trace_device, _trace_stats_device and _adjoint_device
+ * (results never depend on it; no counterpart in the reference):
+ *   queued    : 1 = persistent waves over device path queues, compacted after every bounce by a
+ *               wave ballot and prefix popcount (one atomic per wave); 0 = every bounce launches
+ *               one lane per path of all `count` (stopped paths idle in their waves);
+ *   block     : threads per workgroup, 64, 128 or 256;
+ *   gridScale : persistent grid = gridScale x the workgroups resident per CU x CUs (1..16);
+ *   chunk     : queue entries a wave takes per atomic, a multiple of 64 in 64..4096. */
+struct GzMisLaunch {
+        int32_t queued;
+        int32_t block;
+        int32_t gridScale;
+        int32_t chunk;
+        int32_t reserved[4];
+};
+
+int gz_gpu_set_mis_launch(void *handle, const struct GzMisLaunch *launch);
+int gz_gpu_get_mis_launch(void *handle, struct GzMisLaunch *launch);
+
 
 /* Traversal work of one path, summed over its bounces and all rays of each bounce (closest hit,
  * shadow ray, BSDF ray): interior-node visits, triangle tests, bounces run. */
@@ -432,6 +455,54 @@ int gz_gpu_film_resolve_device(const float *d_light, const float *d_weight, int6
                                int32_t *d_zeroWeight, void *stream);
 
 /* ---------------------------------------------------------------------------------------- */
+/* Multi-scene batch.                                                                        */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Many scenes in one launch sequence (no counterpart in the reference, which renders one scene
+ * per process): each scene is a handle with gz_gpu_set_mis_scene done, a camera (whole frame or
+ * crop), MIS parameters and a film (device images of resX*resY pixels as gz_gpu_film_add_device
+ * takes them, zeroed by the caller). */
+struct GzBatchScene {
+        void *handle;
+        struct GzCamera camera;
+        struct GzMisParams params;
+        float iso;
+        float maxComponentValue;
+        float *d_light;
+        float *d_weight;
+        float *d_albedo;
+        float *d_normal;
+};
+
+struct GzBatchInfo {
+        int64_t totalPaths;      /* path samples of all scenes */
+        int32_t rounds;          /* launch sequences one render runs */
+        int32_t items;           /* (scene, whole pixels) pieces over all rounds */
+        int32_t roundPaths;      /* path slots of a round */
+        int32_t maxItemsPerRound;
+        int32_t levels;          /* bounce levels of a round: 1 + the largest maxDepth */
+        int32_t allBsdfs;        /* 1 if some scene has a non-diffuse material */
+};
+
+/* Plans the batch: the paths of all scenes (in scene order, whole pixels) are cut into rounds of
+ * at most roundPaths path slots, and device tables are built for them. The batch snapshots each
+ * scene's MIS tables and parameters: call gz_gpu_set_mis_scene on a scene only before creating a
+ * batch over it, and free the batch before its scenes. All scenes must be on one device. NULL on
+ * failure, with gz_last_error() set. */
+void *gz_batch_create(const struct GzBatchScene *scenes, int32_t sceneCount, int64_t roundPaths);
+int gz_batch_info(void *batch, struct GzBatchInfo *info);
+/* Launch shape of the batch's persistent bounce kernel (queued must be 1; see GzMisLaunch). */
+int gz_batch_set_launch(void *batch, const struct GzMisLaunch *launch);
+/* Renders every path of every scene once and adds it to the scenes' films, asynchronously on
+ * `stream`. Each round is the same launch sequence (camera rays, device path queues per bounce
+ * level, film) driven by device tables and a device round cursor. useGraph = 1 captures that
+ * sequence once as a HIP graph (on the first call after creation or gz_batch_set_launch) and
+ * replays it once per round; 0 launches it directly. Both give identical films. Calls on one
+ * batch must not overlap (they share its path slots and queues). */
+int gz_batch_render_device(void *batch, int32_t useGraph, void *stream);
+void gz_batch_free(void *batch);
+
+/* ---------------------------------------------------------------------------------------- */
 /* Z-Sobol sampler (replaces the Swift ZSobolSampler, Sampler/ZSobolSampler.swift).            */
 /* ---------------------------------------------------------------------------------------- */
 
@@ -505,6 +576,9 @@ static_assert(sizeof(struct GzMisParams) == 32, "GzMisParams layout");
 static_assert(sizeof(struct GzPathStats) == 12, "GzPathStats layout");
 static_assert(sizeof(struct GzCamera) == 200, "GzCamera layout");
 static_assert(sizeof(struct GzPathAdjoint) == 24, "GzPathAdjoint layout");
+static_assert(sizeof(struct GzMisLaunch) == 32, "GzMisLaunch layout");
+static_assert(sizeof(struct GzBatchScene) == 280, "GzBatchScene layout");
+static_assert(sizeof(struct GzBatchInfo) == 32, "GzBatchInfo layout");
 #endif
 
 #endif /* GONZALES_AMD_H */

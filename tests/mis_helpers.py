@@ -46,6 +46,25 @@ def box_on_plane_scene():
     return build_bounding_hierarchy(Scene(meshes, _materials([(0.6, 0.5, 0.4), (0.3, 0.7, 0.5)])), prims)
 
 
+def conductor_strips_scene(n: int = 8, roughness=None):
+    """n conductor strips side by side on the plane y = 0 (x in [-2, 2], z in [-2, 2]), strip i
+    with material i (Cu, roughness from `roughness`). No occluder and no area light: lit by a
+    constant sky, the image has no visibility edge. Returns (hierarchy, GzMaterial table)."""
+    from gonzales_amd.integrator import conductor_material
+    rough = np.linspace(0.1, 0.5, n) if roughness is None else np.asarray(roughness)
+    verts, idx = [], []
+    for i in range(n):
+        x0, x1 = -2 + 4 * i / n, -2 + 4 * (i + 1) / n
+        v, q = quad([x0, 0, -2], [x0, 0, 2], [x1, 0, 2], [x1, 0, -2])
+        idx.append(q + 4 * i)
+        verts.append(v)
+    mesh = TriangleMesh.from_xyz(np.concatenate(verts), np.concatenate(idx))
+    prims = Primitives.geometric(0, 2 * n, np.repeat(np.arange(n), 2))
+    gz = abi.concat([conductor_material(roughness=float(r)) for r in rough], abi.GZ_MATERIAL)
+    scene = Scene([mesh], _materials([(0.5, 0.5, 0.5)] * n))
+    return build_bounding_hierarchy(scene, prims), gz
+
+
 def mixed_material_scene():
     """Diffuse plane, a rotated box whose 10 triangles cycle through a conductor, smooth and rough
     dielectrics and two coated conductors (clear and scattering coat), and one triangle area light

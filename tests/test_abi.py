@@ -66,6 +66,10 @@ def test_device_entry_points_fail_cleanly_without_a_handle():
     lib.gz_gpu_traverse_batch(None, None, None, 10, None)  # NULL handle: no-op (kernel.mojo:616-618)
     lib.gz_gpu_shade_batch(None, None, 10, None)
     lib.gz_gpu_free_scene(None)
+    c = abi.GzMisLaunch(1, 256, 1, 256)
+    assert lib.gz_gpu_set_mis_launch(None, ctypes.byref(c)) == abi.GZ_E_INVALID
+    assert lib.gz_gpu_get_mis_launch(None, ctypes.byref(c)) == abi.GZ_E_INVALID
+    assert ctypes.sizeof(abi.GzMisLaunch) == 32
 
 
 def test_zsobol_rejects_bad_patterns():

@@ -7,8 +7,10 @@ directions that do not depend on them, so with Russian roulette out of play (max
 replayed derivative and the finite difference agree per parameter to the finite-difference
 truncation and float32 image rounding: bar 1% relative (parameters with a gradient above the
 noise floor). Roughness: the sampled directions move with roughness in the finite difference
-(attached) but not in the adjoint (detached), so the two agree in expectation: bar 10% relative on
-the gradient vector's norm and matching signs where the gradient is large.
+(attached) but not in the adjoint (detached), so the two agree in expectation only. Without
+visibility edges (conductor strips under a sky) they agree within 2% per parameter at 4M paths;
+with occluders, shadow edges that the moving directions cross add a term the adjoint leaves out,
+so there the bar is the gradient's direction (cosine > 0.8) and the signs of its large entries.
 """
 import numpy as np
 import pytest
@@ -88,7 +90,54 @@ def test_reflectance_gradient_matches_finite_differences(gpu):
     assert checked >= 5
 
 
-def test_roughness_gradient_matches_finite_differences(gpu):
+def _fd_roughness(r, albedo, rough, ids, target, h=1e-3):
+    fd = np.zeros(len(ids))
+    for j, m in enumerate(ids):
+        vals = []
+        for s in (1, -1):
+            rr = rough.copy()
+            rr[m] += s * h
+            r.set_parameters(albedo, rr)
+            vals.append(_loss(r.render(), target))
+        fd[j] = (vals[0] - vals[1]) / (2 * h)
+    r.set_parameters(albedo, rough)
+    return fd
+
+
+def test_roughness_gradient_matches_finite_differences_without_edges(gpu):
+    """Conductor strips under a constant sky: no visibility edge, so finite differences with
+    common random numbers (whose sampled directions move with the roughness) and the detached
+    adjoint estimate the same derivative; they differ per sample, so the image is rendered at 1024
+    spp (4M paths) and the bar is 2% per parameter."""
+    import torch
+    from mis_helpers import conductor_strips_scene
+    from gonzales_amd.camera import Film, PerspectiveCamera, look_at
+    n = 8
+    bh, gz = conductor_strips_scene(n)
+    bh.upload_to_gpu()
+    cam = PerspectiveCamera(look_at((0.0, 2.5, 2.5), (0.0, 0.0, 0.0), (0, 1, 0)), 55.0, Film(64, 64), 1024)
+    opt = MisOptions(max_depth=1, zsobol_seed=cam.c.zsSeed, log2_spp=cam.c.zsLog2Spp,
+                     base4_digits=cam.c.zsBase4Digits)
+    r = DiffRenderer(bh, cam, gz, opt, sky=(1.0, 0.9, 0.8))
+    albedo = np.full((n, 3), 0.5, np.float32)
+    rough = gz["roughness"][:, 0].copy()
+    g = np.random.default_rng(3)
+    target = torch.from_numpy(g.uniform(0, 0.3, (64, 64, 3))).cuda()
+    img = r.render()
+    _, gr = r.backward(2 * (img.double() - target))
+    gr = gr.cpu().numpy()
+    fd = _fd_roughness(r, albedo, rough, range(n), target)
+    print("roughness adjoint", np.array2string(gr, precision=5), flush=True)
+    print("roughness fd     ", np.array2string(fd, precision=5), flush=True)
+    rel = np.abs(gr - fd) / np.abs(fd)
+    print(f"relative errors {np.array2string(rel, precision=4)}", flush=True)
+    assert np.all(rel <= 0.02)
+
+
+def test_roughness_gradient_with_occluders(gpu):
+    """Cornell box with occluders: visibility edges (shadows) move with the sampled directions in
+    the finite difference but not in the adjoint, so the two differ by the edges' share; the
+    gradient's direction agrees (cosine > 0.8) and the large components agree in sign."""
     import torch
     r, ta, tr = _setup(64, 64, 2)
     g = np.random.default_rng(3)
@@ -97,22 +146,14 @@ def test_roughness_gradient_matches_finite_differences(gpu):
     target = torch.from_numpy(g.uniform(0, 0.5, (64, 64, 3))).cuda()
     img = r.render()
     _, gr = r.backward(2 * (img.double() - target))
-    gr = gr.cpu().numpy()[3 + N_PARAMS:]
-    h = 1e-3
-    fd = np.zeros(N_PARAMS)
-    for j in range(N_PARAMS):
-        vals = []
-        for s in (1, -1):
-            rr = rough.copy()
-            rr[3 + N_PARAMS + j] += s * h
-            r.set_parameters(albedo, rr)
-            vals.append(_loss(r.render(), target))
-        fd[j] = (vals[0] - vals[1]) / (2 * h)
+    ids = list(range(3 + N_PARAMS, 3 + 2 * N_PARAMS))
+    gr = gr.cpu().numpy()[ids]
+    fd = _fd_roughness(r, albedo, rough, ids, target)
     print("roughness adjoint", np.array2string(gr, precision=4), flush=True)
     print("roughness fd     ", np.array2string(fd, precision=4), flush=True)
-    err = np.linalg.norm(gr - fd) / np.linalg.norm(fd)
-    print(f"roughness gradient: vector relative error {err:.4f}", flush=True)
-    assert err <= 0.10
+    cos = float(gr @ fd / (np.linalg.norm(gr) * np.linalg.norm(fd)))
+    print(f"cosine {cos:.4f}, vector relative error {np.linalg.norm(gr - fd) / np.linalg.norm(fd):.4f}", flush=True)
+    assert cos > 0.8
     big = np.abs(fd) > 0.3 * np.abs(fd).max()
     assert np.all(np.sign(gr[big]) == np.sign(fd[big]))
 
@@ -144,3 +185,26 @@ def test_autograd_descent_reduces_the_loss(gpu):
     print("losses", [round(x, 4) for x in losses], flush=True)
     assert losses[-1] < 0.7 * losses[0]
     assert rg.grad is not None and a.grad is not None
+
+
+def test_queued_adjoint_equals_the_full_relaunch(gpu):
+    """The replay over device path queues and the one-lane-per-path replay give the same
+    gradient (to the order of the double atomics) and the same path records."""
+    import torch
+    r, ta, tr = _setup(32, 8, 4)
+    albedo, rough = _params(ta, tr, np.random.default_rng(5))
+    r.set_parameters(albedo, rough)
+    target = torch.zeros(32, 32, 3, dtype=torch.float64, device="cuda")
+    out = []
+    for queued in (1, 0):
+        r.integrator.set_launch(queued, 256, 1, 256)
+        img = r.render()
+        ga, gr = r.backward(2 * (img.double() - target))
+        out.append((img.cpu().numpy(), ga.cpu().numpy(), gr.cpu().numpy(), r.d_paths.cpu().numpy().tobytes()))
+    r.integrator.set_launch()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][3] == out[1][3]
+    for k in (1, 2):
+        a, b = out[0][k], out[1][k]
+        assert np.all(np.abs(a - b) <= 1e-9 * np.abs(b).max()), k
+    assert np.abs(out[0][1]).max() > 0 and np.abs(out[0][2]).max() > 0

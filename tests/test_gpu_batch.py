@@ -1,0 +1,95 @@
+"""Multi-scene batch (gz_batch_*): every scene's film equals, bit for bit, the film of the same
+scene rendered alone with FrameRenderer (camera paths -> MIS trace -> film add), in eager and
+graph mode, across rounds that split scenes, and on repeated graph replays with other device
+work in between."""
+import numpy as np
+import pytest
+
+from gonzales_amd import abi, scenes
+from gonzales_amd.batch import BatchEntry, SceneBatch, autotune_batch
+from gonzales_amd.camera import DeviceFilm, Film, PerspectiveCamera, cornell_camera, look_at
+from gonzales_amd.integrator import MisOptions, VolumePathIntegrator
+from gonzales_amd.render import FrameRenderer
+
+pytestmark = pytest.mark.gpu
+
+
+def _options(cam, max_depth):
+    return MisOptions(max_depth=max_depth, zsobol_seed=cam.c.zsSeed, log2_spp=cam.c.zsLog2Spp,
+                      base4_digits=cam.c.zsBase4Digits)
+
+
+def _scenes():
+    from mis_helpers import mixed_material_scene
+    out = []
+    bh, _ = scenes.cornell_hierarchy()
+    bh.upload_to_gpu()
+    cam = cornell_camera(res=40, spp=4, seed=3)
+    integ = VolumePathIntegrator(bh, _options(cam, 5))
+    out.append((integ, cam))
+    cam2 = cornell_camera(res=36, spp=2, seed=9, crop=(5, 30, 3, 20))
+    out.append((VolumePathIntegrator(bh, _options(cam2, 5)), cam2))
+    mb, gz = mixed_material_scene()
+    mb.upload_to_gpu()
+    cam3 = PerspectiveCamera(look_at((0.0, 1.4, 3.5), (0.1, 0.5, -0.2), (0, 1, 0)), 35.0, Film(32, 24), 8, 5)
+    out.append((VolumePathIntegrator(mb, _options(cam3, 6), sky=(0.6, 0.7, 0.8), materials=gz), cam3))
+    return out
+
+
+def _films(entries):
+    return [np.concatenate([t.cpu().numpy() for t in (e.film.light, e.film.weight, e.film.albedo, e.film.normal)])
+            for e in entries]
+
+
+def test_batch_films_equal_single_scene_renders(gpu):
+    import torch
+    sc = _scenes()
+    # Scenes 0 and 1 share one handle and MIS tables; the reference renders use the same.
+    refs = []
+    for integ, cam in sc:
+        f = DeviceFilm(cam)
+        FrameRenderer(cam, integ, f, batch_paths=1 << 12).render()
+        torch.cuda.synchronize()
+        refs.append(np.concatenate([t.cpu().numpy() for t in (f.light, f.weight, f.albedo, f.normal)]))
+    entries = [BatchEntry(integ, cam, DeviceFilm(cam)) for integ, cam in sc]
+    batch = SceneBatch(entries, round_paths=3000)
+    info = batch.info()
+    assert info.totalPaths == sum(cam.path_count for _, cam in sc)
+    assert info.rounds > 3 and info.levels == 7 and info.allBsdfs == 1
+    for mode in (False, True, True):
+        batch.clear_films()
+        batch.render(use_graph=mode)
+        junk = torch.randn(1 << 16, device="cuda").sum()  # other device work between replays
+        torch.cuda.synchronize()
+        assert float(junk) == float(junk)
+        for k, (got, ref) in enumerate(zip(_films(entries), refs)):
+            assert got.tobytes() == ref.tobytes(), (mode, k)
+    batch.set_launch(128, 2, 64)
+    batch.clear_films()
+    batch.render(use_graph=True)
+    torch.cuda.synchronize()
+    for k, (got, ref) in enumerate(zip(_films(entries), refs)):
+        assert got.tobytes() == ref.tobytes(), ("reshaped", k)
+    best, timings = autotune_batch(batch, reps=1)
+    assert best in timings
+    batch.render(use_graph=True)
+    torch.cuda.synchronize()
+    for k, (got, ref) in enumerate(zip(_films(entries), refs)):
+        assert got.tobytes() == ref.tobytes(), ("tuned", k)
+    for e in entries:
+        img = e.film.resolve()
+        assert torch.isfinite(img).all()
+    batch.close()
+
+
+def test_batch_rejects_bad_input(gpu):
+    sc = _scenes()
+    entries = [BatchEntry(integ, cam, DeviceFilm(cam)) for integ, cam in sc]
+    with pytest.raises(RuntimeError, match="roundPaths"):
+        SceneBatch(entries, round_paths=4)
+    batch = SceneBatch(entries, round_paths=1 << 14)
+    import ctypes
+    for cfg in [(0, 256, 1, 256), (1, 256, 1, 100), (1, 512, 1, 256)]:
+        c = abi.GzMisLaunch(*cfg)
+        assert abi.load().gz_batch_set_launch(batch.handle, ctypes.byref(c)) == abi.GZ_E_INVALID
+    batch.close()

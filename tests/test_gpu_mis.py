@@ -204,3 +204,92 @@ def test_mixed_materials_match_oracle(gpu, sky):
     got, ref = _run_both(bh, paths, opt, sky=sky, materials=gz)
     assert np.any(ref["flags"] == 0)
     _check(got, ref, min_frac=0.98)
+
+
+LAUNCHES = [(0, 256, 1, 256), (1, 256, 1, 256), (1, 64, 1, 64), (1, 128, 2, 1024), (1, 256, 4, 4096)]
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_queued_launches_equal_the_full_relaunch(gpu, mixed):
+    """Every launch configuration (device path queues compacted by ballot, or one lane per path
+    every bounce) leaves bit-identical path records and work counts: the configuration only
+    changes which lane runs a path and when."""
+    import torch
+    from mis_helpers import mixed_material_scene
+    if mixed:
+        bh, gz = mixed_material_scene()
+        cam = {"eye": (0.0, 1.4, 3.5), "look": (0.1, 0.5, -0.2), "up": (0.0, 1.0, 0.0), "fov": 35.0}
+    else:
+        bh, cam = scenes.cornell_hierarchy()
+        gz = None
+    bh.upload_to_gpu()
+    w = h = 48
+    spp = 4
+    rays, px, py = pinhole_rays(cam, w, h)
+    paths = abi.concat([new_paths(rays, px, py, s) for s in range(spp)], abi.GZ_PATH)
+    paths["flags"][::7] = 0
+    opt = MisOptions.for_image(w, h, spp, max_depth=6)
+    integ = VolumePathIntegrator(bh, opt, materials=gz, sky=(0.3, 0.4, 0.5) if mixed else None)
+    results = []
+    for cfg in LAUNCHES:
+        integ.set_launch(*cfg)
+        assert integ.launch() == cfg
+        d = _to_device(paths)
+        stats = torch.zeros(len(paths) * 3, dtype=torch.int32, device="cuda")
+        integ.trace_stats(d, stats)
+        results.append((_to_host(d).tobytes(), stats.cpu().numpy().tobytes()))
+        d2 = _to_device(paths)
+        integ.trace(d2)
+        assert _to_host(d2).tobytes() == results[-1][0], cfg
+    bh.check_traversal()
+    for cfg, r in zip(LAUNCHES[1:], results[1:]):
+        assert r[0] == results[0][0], cfg
+        assert r[1] == results[0][1], cfg
+    got = np.frombuffer(results[0][0], abi.GZ_PATH)
+    assert np.any(got["bounce"] >= 3) and not np.any(got["flags"] & abi.GZ_PATH_ACTIVE)
+
+
+def test_queue_grows_and_stays_valid_across_sizes(gpu):
+    """The per-stream queue storage grows with the path count; a smaller trace after a larger one
+    and a trace on another stream give the same records as fresh runs."""
+    import torch
+    bh, cam = scenes.cornell_hierarchy()
+    bh.upload_to_gpu()
+    rays, px, py = pinhole_rays(cam, 64, 64)
+    big = abi.concat([new_paths(rays, px, py, s) for s in range(2)], abi.GZ_PATH)
+    opt = MisOptions.for_image(64, 64, 2, max_depth=5)
+    integ = VolumePathIntegrator(bh, opt)
+    small = big[:1000].copy()
+    d_small0 = _to_device(small)
+    integ.trace(d_small0)
+    ref_small = _to_host(d_small0).tobytes()
+    d_big = _to_device(big)
+    integ.trace(d_big)
+    ref_big = _to_host(d_big).tobytes()
+    d_small = _to_device(small)
+    integ.trace(d_small)
+    assert _to_host(d_small).tobytes() == ref_small
+    s = torch.cuda.Stream()
+    d_big2 = _to_device(big)
+    torch.cuda.synchronize()
+    integ.trace(d_big2, stream=s)
+    s.synchronize()
+    assert _to_host(d_big2).tobytes() == ref_big
+    d_part = _to_device(big)
+    integ.trace(d_part, count=777)
+    got = _to_host(d_part)
+    assert got[:777].tobytes() == np.frombuffer(ref_big, abi.GZ_PATH)[:777].tobytes()
+    assert got[777:].tobytes() == big[777:].tobytes()
+
+
+def test_bad_launch_configurations_are_rejected(gpu):
+    bh, _ = scenes.cornell_hierarchy()
+    bh.upload_to_gpu()
+    lib = abi.load()
+    for cfg in [(2, 256, 1, 256), (1, 96, 1, 256), (1, 512, 1, 256), (1, 256, 0, 256), (1, 256, 17, 256),
+                (1, 256, 1, 32), (1, 256, 1, 100), (1, 256, 1, 8192)]:
+        c = abi.GzMisLaunch(*cfg)
+        assert lib.gz_gpu_set_mis_launch(bh.handle, ctypes.byref(c)) == abi.GZ_E_INVALID, cfg
+    out = abi.GzMisLaunch()
+    assert lib.gz_gpu_get_mis_launch(bh.handle, ctypes.byref(out)) == abi.GZ_OK
+    assert (out.queued, out.block, out.gridScale, out.chunk) == (1, 256, 1, 256)
