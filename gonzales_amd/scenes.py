@@ -246,8 +246,9 @@ def fbm(p: np.ndarray, seed: int, octaves: int = 4, lacunarity: float = 2.0, gai
 
 
 def inverse_rendering_scene(subdivisions: int = 6, n_params: int = 32, seed: int = 8, patch: int = 64):
-    """The Cornell box plus two displaced icospheres (occluders; 2 x 20 x 4^s triangles, 163,840
-    at s = 6) whose 64-face patches cycle through n_params diffuse materials (parameter: RGB
+    """The Cornell box plus three displaced occluders: two icospheres (2 x 20 x 4^s triangles) and a
+    lat-long sphere of about 36,160 x 4^(s-6) triangles, 200,320 in all at s = 6, whose 64-face
+    patches cycle through n_params diffuse materials (parameter: RGB
     reflectance) and n_params conductor materials (parameter: GGX roughness). Material table:
     the 3 walls (white, red, green), then the diffuse, then the conductor materials. Returns
     (hierarchy, GzMaterial table, target reflectances (n_params, 3), target roughness (n_params,)),
@@ -256,8 +257,8 @@ def inverse_rendering_scene(subdivisions: int = 6, n_params: int = 32, seed: int
     scene, prims, _ = cornell_box()
     meshes = list(scene.meshes)
     parts = [prims]
-    for k, (c, r) in enumerate((((-0.42, 0.45, -0.25), 0.36), ((0.40, 0.36, 0.30), 0.3))):
-        v, f = icosphere(subdivisions)
+    for k, (c, r) in enumerate((((-0.42, 0.45, -0.25), 0.36), ((0.40, 0.36, 0.30), 0.3), ((0.05, 1.3, -0.35), 0.2))):
+        v, f = icosphere(subdivisions) if k < 2 else latlong_sphere(int(round(36160 * 4.0 ** (subdivisions - 6))))
         v = v * (1 + 0.03 * fbm(v, 5 + k))[:, None]
         v = v * r + np.asarray(c)
         mi = len(meshes)
@@ -318,3 +319,94 @@ def displaced_sphere_scene(subdivisions: int = 9, amplitude: float = 0.05, seed_
     scene = Scene([sphere, ground, light], placeholder,
                   light_emission=np.array([[8.0, 8.0, 8.0], [8.0, 8.0, 8.0]], np.float32))
     return build_bounding_hierarchy(scene, prims), gz
+
+
+# ---------------------------------------------------------------------------------------------
+# Many-scene batch: 64 scenes of uneven size
+# ---------------------------------------------------------------------------------------------
+
+def latlong_sphere(target_tris: int) -> tuple[np.ndarray, np.ndarray]:
+    """Unit sphere of m - 1 rings of n = 2m vertices between the poles (two pole fans and m - 2
+    bands of quads): 4 m (m - 1) triangles, m chosen to bring that count nearest `target_tris`.
+    Float64 vertices, int64 (F, 3) faces, counter-clockwise seen from outside."""
+    m = max(3, int(round(0.5 + np.sqrt(0.25 + target_tris / 4.0))))
+    n = 2 * m
+    theta = np.pi * np.arange(1, m) / m
+    phi = 2 * np.pi * np.arange(n) / n
+    st, ct = np.sin(theta), np.cos(theta)
+    ring = np.stack([st[:, None] * np.cos(phi)[None, :], np.broadcast_to(ct[:, None], (m - 1, n)),
+                     st[:, None] * np.sin(phi)[None, :]], -1).reshape(-1, 3)
+    v = np.concatenate([[[0.0, 1.0, 0.0]], ring, [[0.0, -1.0, 0.0]]])
+    south = len(v) - 1
+    j = np.arange(n)
+    jn = (j + 1) % n
+    faces = [np.stack([np.zeros(n, np.int64), 1 + jn, 1 + j], 1)]
+    for r in range(m - 2):
+        a, b = 1 + r * n + j, 1 + r * n + jn
+        c, d = 1 + (r + 1) * n + j, 1 + (r + 1) * n + jn
+        faces += [np.stack([a, b, d], 1), np.stack([a, d, c], 1)]
+    last = 1 + (m - 2) * n
+    faces.append(np.stack([np.full(n, south), last + j, last + jn], 1))
+    return v, np.concatenate(faces).astype(np.int64)
+
+
+def batch_scene_specs(count: int = 64, seed: int = 7) -> list[dict]:
+    """Parameters of `count` scenes: triangle count log-uniform in [1e4, 2e6], kind alternating
+    displaced sphere / triangle soup, resolution in {256, 512, 1024} squared and spp in {4, 16, 64}
+    uniformly, max depth 6; per-scene seeds for geometry and materials."""
+    g = _rng(seed)
+    tris = np.exp(g.uniform(np.log(1e4), np.log(2e6), count)).astype(np.int64)
+    res = g.choice([256, 512, 1024], count)
+    spp = g.choice([4, 16, 64], count)
+    return [{"index": k, "kind": "sphere" if k % 2 == 0 else "soup", "tris": int(tris[k]), "res": int(res[k]),
+             "spp": int(spp[k]), "max_depth": 6, "seed": int(seed * 1000 + k)} for k in range(count)]
+
+
+def batch_scene(spec: dict, patch: int = 64):
+    """One batch scene: a displaced lat-long sphere (r = 1, fBm amplitude 0.05) on a ground plane,
+    or a triangle soup in the unit cube (the traversal benchmark's generator), with 64-face
+    patches of diffuse (albedo U[0.1, 0.8]^3), dielectric (eta 1.5, GGX roughness U[0.05, 0.6])
+    and conductor (Cu, GGX roughness U[0.05, 0.6]) materials, a quad area light above and a
+    constant sky. Returns (hierarchy, GzMaterial table, camera look-at (eye, target, up, fov))."""
+    from .integrator import conductor_material, dielectric_material, diffuse_material
+    g = _rng(spec["seed"])
+    meshes, parts = [], []
+    if spec["kind"] == "sphere":
+        v, f = latlong_sphere(spec["tris"])
+        v = v * (1 + 0.05 * fbm(v, spec["seed"]))[:, None]
+        meshes.append(TriangleMesh.from_xyz(v.astype(np.float32), f.reshape(-1)))
+        n_faces = len(f)
+        gv, gi = _quad([-20, -1.05, -20], [-20, -1.05, 20], [20, -1.05, 20], [20, -1.05, -20])
+        meshes.append(TriangleMesh.from_xyz(np.asarray(gv, np.float32), np.asarray(gi, np.int64)))
+        lv, li = _quad([-0.75, 2.5, -0.75], [0.75, 2.5, -0.75], [0.75, 2.5, 0.75], [-0.75, 2.5, 0.75])
+        cam = ((0.0, 0.45, 3.4), (0.0, -0.15, 0.0), (0.0, 1.0, 0.0), 40.0)
+    else:
+        soup = triangle_soup(spec["tris"], spec["seed"])
+        meshes.append(soup)
+        n_faces = soup.number_triangles
+        lv, li = _quad([0.1, 1.8, 0.1], [0.9, 1.8, 0.1], [0.9, 1.8, 0.9], [0.1, 1.8, 0.9])
+        cam = ((0.5, 0.7, -1.4), (0.5, 0.45, 0.5), (0.0, 1.0, 0.0), 45.0)
+    n_patch = (n_faces + patch - 1) // patch
+    kind = g.integers(0, 3, n_patch)
+    albedo = g.uniform(0.1, 0.8, (n_patch, 3))
+    rough = g.uniform(0.05, 0.6, n_patch)
+    mats = [diffuse_material((0.5, 0.5, 0.5))]
+    for k in range(n_patch):
+        if kind[k] == 0:
+            mats.append(diffuse_material(albedo[k]))
+        elif kind[k] == 1:
+            mats.append(dielectric_material(eta=1.5, roughness=rough[k]))
+        else:
+            mats.append(conductor_material(roughness=rough[k]))
+    gz = abi.concat(mats, abi.GZ_MATERIAL)
+    parts.append(Primitives.geometric(0, n_faces, 1 + np.arange(n_faces) // patch))
+    if spec["kind"] == "sphere":
+        parts.append(Primitives.geometric(1, 2, 0))
+    light_mesh = len(meshes)
+    meshes.append(TriangleMesh.from_xyz(np.asarray(lv, np.float32), np.asarray(li, np.int64)))
+    parts.append(Primitives.area_lights(light_mesh, 2, 0))
+    placeholder = np.zeros(len(gz), abi.MATERIAL)
+    placeholder["type"] = 1
+    placeholder["albedo"] = 0.5
+    scene = Scene(meshes, placeholder, light_emission=np.array([[8.0, 8.0, 8.0], [8.0, 8.0, 8.0]], np.float32))
+    return build_bounding_hierarchy(scene, Primitives.concat(parts)), gz, cam
