@@ -321,6 +321,56 @@ def full_path_leg(args, dev, stream, cores: int, run_cpu: bool) -> dict:
     return res
 
 
+def device_build_leg(args, host_bh, d_rays, d_tmax, n, stream, host_mrays) -> dict:
+    """The same 1M-triangle soup built on the device (gz_gpu_build_scene: Morton codes, radix
+    sort, Karras hierarchy, bottom-up boxes, leaves of <= 4) against the host SAH build: build
+    time (upload of primIds and meshes included), SAH cost of the exported tree in the host
+    builder's model, and incoherent closest-hit Mrays/s over the same rays with its own autotuned
+    launch configuration; the hit distances of both trees are compared bit for bit."""
+    import torch
+
+    from gonzales_amd import scenes
+    from gonzales_amd.accelerator import build_bounding_hierarchy_gpu, sah_cost
+    from gonzales_amd.autotune import autotune
+
+    scene, prims = scenes.soup_scene(args.tris, 1)
+    out = {}
+    for max_leaf in (4, 2):
+        build_bounding_hierarchy_gpu(scene, prims, max_leaf).free()  # warm-up (module load, allocator)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dbh = build_bounding_hierarchy_gpu(scene, prims, max_leaf)
+        t_dev = time.perf_counter() - t0
+        best, _ = autotune(dbh, d_rays, d_tmax, any_hit=False, sample=n, reps=2)
+        d_hits = torch.empty(n * 16, dtype=torch.uint8, device=d_rays.device)
+        for _ in range(2):
+            dbh.closest_device(d_rays, d_tmax, d_hits)
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(5):
+            dbh.closest_device(d_rays, d_tmax, d_hits)
+        b.record(stream)
+        torch.cuda.synchronize()
+        dbh.check_traversal()
+        ms = a.elapsed_time(b) / 5
+        h_hits = torch.empty_like(d_hits)
+        host_bh.closest_device(d_rays, d_tmax, h_hits)
+        torch.cuda.synchronize()
+        t_same = bool(torch.equal(d_hits.view(torch.int32).view(n, 4)[:, 1], h_hits.view(torch.int32).view(n, 4)[:, 1]))
+        info = dbh.info()
+        out[f"max_leaf_{max_leaf}"] = {
+            "build_s_including_upload": round(t_dev, 3), "sah_cost": round(sah_cost(dbh.nodes), 3),
+            "closest_mrays_per_s": round(n / (ms * 1e-3) / 1e6, 2), "depth": info.maxDepth,
+            "hit_distances_equal_host_build": t_same}
+        log(f"device build (leaves <= {max_leaf}): {t_dev:.3f}s, SAH {out[f'max_leaf_{max_leaf}']['sah_cost']}, "
+            f"{n / (ms * 1e-3) / 1e6:.1f} Mrays/s, distances equal: {t_same}")
+        dbh.free()
+    out["host_build"] = {"sah_cost": round(sah_cost(host_bh.nodes), 3), "closest_mrays_per_s": round(host_mrays, 2)}
+    return out
+
+
 def batch_leg(args, dev, stream) -> dict:
     """64 scenes of uneven size in one launch sequence: triangle counts log-uniform in [1e4, 2e6]
     (displaced spheres and triangle soups), resolutions 256^2 / 512^2 / 1024^2, 4 / 16 / 64 spp,
@@ -599,6 +649,7 @@ def main() -> int:
         extra["coherent_anyhit_mrays_per_s"] = round(nc / (ms_ca * 1e-3) / 1e6, 2)
         extra["coherent_launch_config"] = describe({cbest: 0.0})[0]
         bh.set_trace_config(best.persistent, best.lds_stack, best.refill_lanes, best.grid_scale, best.chunk)
+        extra["device_build"] = device_build_leg(args, bh, d_rays, d_tmax, n, stream, mrays)
         # End to end through host buffers (H2D + trace + Intersection_C expansion + D2H), 1M rays.
         m = 1 << 20
         t1 = time.perf_counter()

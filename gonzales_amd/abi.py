@@ -100,6 +100,10 @@ class GzMisLaunch(ctypes.Structure):
                 ("chunk", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
 
 
+class GzBuildOptions(ctypes.Structure):
+    _fields_ = [("maxLeaf", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+
+
 class GzMisParams(ctypes.Structure):
     _fields_ = [("maxDepth", ctypes.c_int32), ("flags", ctypes.c_int32), ("zsSeed", ctypes.c_int64),
                 ("zsLog2Spp", ctypes.c_int32), ("zsBase4Digits", ctypes.c_int32), ("rrSeed", ctypes.c_uint64)]
@@ -174,6 +178,8 @@ SIGNATURES = {
     "gz_gpu_camera_paths_device": (ctypes.c_int, [_P, _I64, _I64, _P, _P]),
     "gz_gpu_film_add_device": (ctypes.c_int, [_P, _P, _I64, ctypes.c_float, ctypes.c_float, _P, _P, _P, _P, _P]),
     "gz_gpu_film_resolve_device": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
+    "gz_gpu_build_scene": (ctypes.c_void_p, [_P, _I64, _P, _I64, _P, _P, _P, _P, _I64, _P]),
+    "gz_gpu_export_bvh2": (_I64, [_P, _P, _I64, _P]),
     "gz_batch_create": (ctypes.c_void_p, [_P, ctypes.c_int32, _I64]),
     "gz_batch_info": (ctypes.c_int, [_P, _P]),
     "gz_batch_set_launch": (ctypes.c_int, [_P, _P]),

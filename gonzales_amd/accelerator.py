@@ -193,7 +193,10 @@ class BoundingHierarchy:
         self.prim_ids["materialIndex"][sel] = base + self.prim_ids["id1"][sel]
 
     def upload_to_gpu(self) -> None:
-        """uploadToGPU (BoundingHierarchy.swift:152-202) through gz_gpu_upload_scene."""
+        """uploadToGPU (BoundingHierarchy.swift:152-202) through gz_gpu_upload_scene; a no-op when
+        the hierarchy already has a handle (built on the device, or uploaded before)."""
+        if self.handle is not None:
+            return
         lib = abi.load()
         self.prepare_materials()
         meshes = self.scene.meshes
@@ -305,3 +308,56 @@ def build_bounding_hierarchy(scene: Scene, prims: Primitives) -> BoundingHierarc
     bh = BoundingHierarchy(scene, ids, nodes)
     bh.prepare_materials()
     return bh
+
+
+def build_bounding_hierarchy_gpu(scene: Scene, prims: Primitives, max_leaf: int = 4) -> BoundingHierarchy:
+    """The same hierarchy object built on the device (gz_gpu_build_scene: a Morton-order linear
+    BVH). The handle is ready for every device entry point; the tree is also read back in the
+    BVH2 layout (gz_gpu_export_bvh2) with the PrimIds in leaf order, for host consumers."""
+    lib = abi.load()
+    bh = BoundingHierarchy(scene, pack_prim_ids(prims), np.zeros(0, abi.BVH2_NODE))
+    bh.prepare_materials()
+    meshes = scene.meshes
+    meshes_c = scene.meshes_c()
+    pts = np.array([m.points.size for m in meshes] or [0], np.int64)
+    faces = np.array([m.face_indices.size for m in meshes] or [0], np.int64)
+    vtx = np.array([m.vertex_indices.size for m in meshes] or [0], np.int64)
+    opt = abi.GzBuildOptions(max_leaf)
+    h = lib.gz_gpu_build_scene(abi.ptr(bh.prim_ids), len(bh.prim_ids), ctypes.addressof(meshes_c), len(meshes),
+                               abi.ptr(pts), abi.ptr(faces), abi.ptr(vtx), abi.ptr(bh.materials_c),
+                               len(bh.materials_c), ctypes.byref(opt))
+    if not h:
+        raise abi.GzError(abi.GZ_E_INVALID, "gz_gpu_build_scene: " + abi.last_error())
+    bh.handle = h
+    n = len(bh.prim_ids)
+    nodes = np.zeros(max(2 * n, 1), abi.BVH2_NODE)
+    order = np.zeros(max(n, 1), np.int32)
+    count = lib.gz_gpu_export_bvh2(h, abi.ptr(nodes), len(nodes), abi.ptr(order))
+    if count < 0:
+        abi.check(int(count), "gz_gpu_export_bvh2")
+    bh.nodes = nodes[:count].copy()
+    bh.prim_ids = np.ascontiguousarray(bh.prim_ids[order[:n]])
+    return bh
+
+
+def export_bvh2(bh: BoundingHierarchy) -> tuple[np.ndarray, np.ndarray]:
+    """(BVH2 nodes, leaf-order -> caller index) of a handle (gz_gpu_export_bvh2)."""
+    n = len(bh.prim_ids)
+    nodes = np.zeros(max(2 * n, 1), abi.BVH2_NODE)
+    order = np.zeros(max(n, 1), np.int32)
+    count = abi.load().gz_gpu_export_bvh2(bh.handle, abi.ptr(nodes), len(nodes), abi.ptr(order))
+    if count < 0:
+        abi.check(int(count), "gz_gpu_export_bvh2")
+    return nodes[:count].copy(), order[:n].copy()
+
+
+def sah_cost(nodes: np.ndarray, traversal: float = 0.5, intersect: float = 1.0) -> float:
+    """Surface-area cost of a BVH2 in the host builder's model (a node visit costs 1/2 of a
+    primitive test, BoundingHierarchyBuilder.swift:225-249): sum over interior nodes of
+    SA(node) / SA(root) x traversal + over leaves of SA(leaf) / SA(root) x count x intersect."""
+    lo = np.stack([nodes["boundsMinX"], nodes["boundsMinY"], nodes["boundsMinZ"]], 1).astype(np.float64)
+    hi = np.stack([nodes["boundsMaxX"], nodes["boundsMaxY"], nodes["boundsMaxZ"]], 1).astype(np.float64)
+    d = np.maximum(hi - lo, 0.0)
+    sa = 2 * (d[:, 0] * d[:, 1] + d[:, 1] * d[:, 2] + d[:, 2] * d[:, 0])
+    leaf = nodes["count"] > 0
+    return float((sa[~leaf].sum() * traversal + (sa[leaf] * nodes["count"][leaf]).sum() * intersect) / sa[0])

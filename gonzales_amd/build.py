@@ -24,7 +24,7 @@ ORACLE = ROOT / "oracle"
 ARCH = os.environ.get("GZ_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["gz_scene.hip", "gz_traverse.hip", "gz_shade.hip", "gz_zsobol.hip", "gz_mis.hip", "gz_film.hip",
-               "gz_batch.hip"]
+               "gz_batch.hip", "gz_build.hip"]
 CXX_SOURCES = ["bvh_build.cpp", "sobol_table.cpp"]
 
 # Device code: no fused multiply-add contraction (the traversal is bit-exact against the

@@ -1,0 +1,560 @@
+// BVH construction on the device: a linear BVH (Morton order, Karras 2012 hierarchy) written
+// straight into the traversal kernels' layout. The reference builds its SAH BVH2 on the host
+// (Accelerators/BoundingHierarchyBuilder.swift:199-339) and uploads it; here only the primIds
+// and the meshes go to the device and the tree is made there:
+//
+//   gather   one lane per primitive: PrimId -> mesh -> vertexIndices -> points (the decode of
+//            kernel.mojo:266-300), the triangle's box (Triangle.worldBound) and the box centroid;
+//            the centroids' bounds by wave reduction + one atomic per wave;
+//   morton   63-bit codes (21 bits per axis) of the centroids in their bounds;
+//   sort     rocPRIM radix sort of (code, primitive) pairs: the sorted order is the leaf order;
+//   hierarchy one lane per internal node i of the N - 1: its key range and split by the longest
+//            common prefix of the codes (ties broken by position), children and parents;
+//   refit    one lane per leaf climbs to the root; at each node the second child to arrive
+//            (an atomic counter) writes the union of the two child boxes (exact min/max);
+//   emit     every internal node whose range holds more than maxLeaf primitives becomes a
+//            64-byte child-pair record; a child range of at most maxLeaf primitives becomes a
+//            leaf: its primitives are consecutive in the sorted order, so the leaf is a run of
+//            48-byte triangle records. primIds are stored in the same order, so a hit's
+//            primitive index is a leaf-order position, as after a host build.
+//
+// Every step is a bounded loop (no lane waits on another); a node record is written only by its
+// own lane and a leaf count only by the leaf's parent.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "gz_internal.h"
+
+namespace gz {
+
+constexpr int kBuildBlock = 256;
+
+struct Box {
+        float lo[3], hi[3];
+};
+
+// Order-preserving map of a float to an unsigned key (for atomicMin/atomicMax on floats).
+__device__ __forceinline__ unsigned fkey(float f) {
+        const unsigned b = __float_as_uint(f);
+        return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float fval(unsigned k) {
+        return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+__device__ __forceinline__ uint64_t spread21(uint64_t v) {
+        v &= 0x1fffffull;
+        v = (v | (v << 32)) & 0x1f00000000ffffull;
+        v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+        v = (v | (v << 8)) & 0x100f00f00f00f00full;
+        v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+        v = (v | (v << 2)) & 0x1249249249249249ull;
+        return v;
+}
+
+// Primitive k: its vertices (w = 0), its box and its centroid; unknown or non-triangle
+// primitives (the reference skips them in a leaf) get an empty box and are flagged. Indices
+// outside the mesh arrays set err bit 4 (the handle is then rejected).
+__global__ __launch_bounds__(kBuildBlock) void k_gather(const PrimId_C *__restrict__ prims, int n,
+                                                        const DevMesh *__restrict__ meshes, int64_t nMeshes,
+                                                        float4 *__restrict__ verts, Box *__restrict__ boxes,
+                                                        float4 *__restrict__ cent, unsigned *__restrict__ cb,
+                                                        int *__restrict__ flags) {
+        const int k = (int)(blockIdx.x * kBuildBlock + threadIdx.x);
+        bool valid = false;
+        float c[3] = {0.0f, 0.0f, 0.0f};
+        if (k < n) {
+                const PrimId_C pr = prims[k];
+                int64_t mesh = -1, base = -1;
+                if (pr.type == 0) {
+                        mesh = pr.id1;
+                        base = pr.id2;
+                } else if ((pr.type == 1 || pr.type == 2 || pr.type == 3) && pr.id2 != -1) {
+                        mesh = pr.id2 >> 32;
+                        base = (pr.id2 & 0xFFFFFFFFLL) * 3;
+                }
+                Box b;
+                float v[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                int skip = 0;
+                if (mesh < 0 && base < 0) {
+                        skip = kTriSkip;
+                } else if (mesh < 0 || mesh >= nMeshes || base < 0 || base + 2 >= meshes[mesh].vertexIndexCount) {
+                        atomicOr(&flags[0], 4);
+                        skip = kTriSkip;
+                } else {
+                        const DevMesh m = meshes[mesh];
+                        for (int q = 0; q < 3; ++q) {
+                                const int64_t vi = m.vertexIndices[base + q];
+                                if (vi < 0 || vi * 4 + 2 >= m.pointFloats) {
+                                        atomicOr(&flags[0], 4);
+                                        skip = kTriSkip;
+                                        break;
+                                }
+                                v[q][0] = m.points[vi * 4];
+                                v[q][1] = m.points[vi * 4 + 1];
+                                v[q][2] = m.points[vi * 4 + 2];
+                        }
+                }
+                if (skip) {
+                        for (int a = 0; a < 3; ++a) {
+                                b.lo[a] = INFINITY;
+                                b.hi[a] = -INFINITY;
+                        }
+                } else {
+                        for (int a = 0; a < 3; ++a) {
+                                b.lo[a] = fminf(v[0][a], fminf(v[1][a], v[2][a]));
+                                b.hi[a] = fmaxf(v[0][a], fmaxf(v[1][a], v[2][a]));
+                                c[a] = 0.5f * (b.lo[a] + b.hi[a]);
+                                // the traversal's fast slab form needs finite boxes within 2^40
+                                if (!(b.lo[a] >= -0x1p40f && b.hi[a] <= 0x1p40f)) atomicOr(&flags[0], 1);
+                        }
+                        valid = true;
+                }
+                if (skip) atomicOr(&flags[0], 2);
+                verts[3 * (size_t)k] = make_float4(v[0][0], v[0][1], v[0][2], 0.0f);
+                verts[3 * (size_t)k + 1] = make_float4(v[1][0], v[1][1], v[1][2], 0.0f);
+                verts[3 * (size_t)k + 2] = make_float4(v[2][0], v[2][1], v[2][2], __int_as_float(skip));
+                boxes[k] = b;
+                cent[k] = make_float4(c[0], c[1], c[2], valid ? 1.0f : 0.0f);
+        }
+        // centroid bounds: wave min/max, one atomic per wave and axis
+        for (int a = 0; a < 3; ++a) {
+                unsigned lo = valid ? fkey(c[a]) : 0xffffffffu, hi = valid ? fkey(c[a]) : 0u;
+                for (int off = 32; off >= 1; off >>= 1) {
+                        lo = min(lo, (unsigned)__shfl_xor((int)lo, off));
+                        hi = max(hi, (unsigned)__shfl_xor((int)hi, off));
+                }
+                if ((threadIdx.x & 63) == 0) {
+                        atomicMin(&cb[a], lo);
+                        atomicMax(&cb[3 + a], hi);
+                }
+        }
+}
+
+__global__ void k_build_init(unsigned *cb, int *flags) {
+        if (threadIdx.x < 3) {
+                cb[threadIdx.x] = 0xffffffffu;
+                cb[3 + threadIdx.x] = 0u;
+        }
+        if (threadIdx.x == 0) {
+                flags[0] = 0;
+                flags[1] = 0;  // emitted interior nodes
+                flags[2] = 0;  // deepest interior node (edges from the root)
+                flags[3] = 0;  // leaves
+        }
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_morton(const float4 *__restrict__ cent, int n,
+                                                        const unsigned *__restrict__ cb, uint64_t *__restrict__ keys,
+                                                        uint32_t *__restrict__ vals) {
+        const int k = (int)(blockIdx.x * kBuildBlock + threadIdx.x);
+        if (k >= n) return;
+        const float4 c = cent[k];
+        uint64_t code = 0x7fffffffffffffffull;  // skipped primitives go last
+        if (c.w != 0.0f) {
+                const float p[3] = {c.x, c.y, c.z};
+                uint64_t q[3];
+                for (int a = 0; a < 3; ++a) {
+                        const float lo = fval(cb[a]), hi = fval(cb[3 + a]);
+                        const float ext = hi - lo;
+                        float t = ext > 0.0f ? (p[a] - lo) / ext : 0.0f;
+                        t = fminf(fmaxf(t, 0.0f), 1.0f);
+                        q[a] = (uint64_t)fminf(t * 2097152.0f, 2097151.0f);
+                }
+                code = (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
+        }
+        keys[k] = code;
+        vals[k] = (uint32_t)k;
+}
+
+// Length of the common prefix of sorted keys i and j (positions break ties), -1 outside [0, n).
+__device__ __forceinline__ int delta(const uint64_t *__restrict__ keys, int n, int i, int j) {
+        if (j < 0 || j >= n) return -1;
+        const uint64_t a = keys[i], b = keys[j];
+        if (a == b) return 64 + __clz((unsigned)(i ^ j));
+        return __clzll((long long)(a ^ b));
+}
+
+// Internal node i of the N - 1 (Karras, "Maximizing parallelism in the construction of BVHs,
+// octrees and k-d trees", HPG 2012, section 4): range [first, last] and split; child c >= 0 is
+// internal node c, c < 0 is leaf ~c (a sorted position).
+__global__ __launch_bounds__(kBuildBlock) void k_hierarchy(const uint64_t *__restrict__ keys, int n,
+                                                           int4 *__restrict__ node, int *__restrict__ parent,
+                                                           int *__restrict__ leafParent) {
+        const int i = (int)(blockIdx.x * kBuildBlock + threadIdx.x);
+        if (i >= n - 1) return;
+        const int d = delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1) >= 0 ? 1 : -1;
+        const int dmin = delta(keys, n, i, i - d);
+        int lmax = 2;
+        while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+        int l = 0;
+        for (int t = lmax >> 1; t >= 1; t >>= 1)
+                if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+        const int j = i + l * d;
+        const int dnode = delta(keys, n, i, j);
+        int s = 0;
+        for (int div = 2;; div <<= 1) {
+                const int t = (l + div - 1) / div;
+                if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+                if (t <= 1) break;
+        }
+        const int g = i + s * d + (d < 0 ? -1 : 0);
+        const int first = min(i, j), last = max(i, j);
+        const int left = first == g ? ~g : g;
+        const int right = last == g + 1 ? ~(g + 1) : g + 1;
+        node[i] = make_int4(first, last, left, right);
+        if (left >= 0)
+                parent[left] = i;
+        else
+                leafParent[g] = i;
+        if (right >= 0)
+                parent[right] = i;
+        else
+                leafParent[g + 1] = i;
+}
+
+__device__ __forceinline__ Box box_union(const Box &a, const Box &b) {
+        Box r;
+        for (int q = 0; q < 3; ++q) {
+                r.lo[q] = fminf(a.lo[q], b.lo[q]);
+                r.hi[q] = fmaxf(a.hi[q], b.hi[q]);
+        }
+        return r;
+}
+
+// Bottom-up boxes: the lane of leaf position p climbs while it is the second child to arrive.
+__global__ __launch_bounds__(kBuildBlock) void k_refit(const int4 *__restrict__ node, const int *__restrict__ parent,
+                                                       const int *__restrict__ leafParent,
+                                                       const uint32_t *__restrict__ order,
+                                                       const Box *__restrict__ primBox, Box *nodeBox,
+                                                       unsigned *arrivals, int n) {
+        const int p = (int)(blockIdx.x * kBuildBlock + threadIdx.x);
+        if (p >= n) return;
+        int cur = leafParent[p];
+        while (cur >= 0) {
+                __threadfence();
+                if (atomicAdd(&arrivals[cur], 1u) == 0u) return;
+                __threadfence();
+                const int4 nd = node[cur];
+                const Box a = nd.z >= 0 ? nodeBox[nd.z] : primBox[order[~nd.z]];
+                const Box b = nd.w >= 0 ? nodeBox[nd.w] : primBox[order[~nd.w]];
+                nodeBox[cur] = box_union(a, b);
+                cur = cur == 0 ? -1 : parent[cur];
+        }
+}
+
+// Triangle records and primIds in leaf (sorted) order.
+__global__ __launch_bounds__(kBuildBlock) void k_sorted(const uint32_t *__restrict__ order,
+                                                        const float4 *__restrict__ verts,
+                                                        const PrimId_C *__restrict__ primsIn, int n,
+                                                        TriRec *__restrict__ tris, PrimId_C *__restrict__ primsOut) {
+        const int j = (int)(blockIdx.x * kBuildBlock + threadIdx.x);
+        if (j >= n) return;
+        const uint32_t k = order[j];
+        TriRec t;
+        t.p0 = verts[3 * (size_t)k];
+        t.p1 = verts[3 * (size_t)k + 1];
+        t.p2 = verts[3 * (size_t)k + 2];
+        t.p0.w = __int_as_float(0);
+        t.p1.w = __int_as_float(j);
+        tris[j] = t;
+        primsOut[j] = primsIn[k];
+}
+
+__device__ __forceinline__ Box child_box(int c, const Box *__restrict__ nodeBox, const Box *__restrict__ primBox,
+                                         const uint32_t *__restrict__ order) {
+        return c >= 0 ? nodeBox[c] : primBox[order[~c]];
+}
+
+// Child-pair records of the internal nodes holding more than maxLeaf primitives, leaf counts in
+// the first triangle record of each leaf, and the depth / node / leaf counts.
+__global__ __launch_bounds__(kBuildBlock) void k_emit(const int4 *__restrict__ node, const int *__restrict__ parent,
+                                                      const Box *__restrict__ nodeBox, const Box *__restrict__ primBox,
+                                                      const uint32_t *__restrict__ order, int n, int maxLeaf,
+                                                      NodePair *__restrict__ pairs, TriRec *__restrict__ tris,
+                                                      int *__restrict__ flags) {
+        const int i = (int)(blockIdx.x * kBuildBlock + threadIdx.x);
+        if (i >= n - 1) return;
+        const int4 nd = node[i];
+        if (nd.y - nd.x + 1 <= maxLeaf) return;
+        int desc[2];
+        Box cb[2];
+        const int g = nd.z >= 0 ? nd.z : ~nd.z;  // split: left [first, g], right [g + 1, last]
+        for (int side = 0; side < 2; ++side) {
+                const int c = side == 0 ? nd.z : nd.w;
+                const int cFirst = side == 0 ? nd.x : g + 1;
+                const int cLast = side == 0 ? g : nd.y;
+                cb[side] = child_box(c, nodeBox, primBox, order);
+                const int size = cLast - cFirst + 1;
+                if (size <= maxLeaf) {
+                        desc[side] = ~cFirst;
+                        tris[cFirst].p0.w = __int_as_float(size);
+                        atomicAdd(&flags[3], 1);
+                } else {
+                        desc[side] = c;
+                }
+        }
+        NodePair p;
+        p.a = make_float4(cb[0].lo[0], cb[0].lo[1], cb[0].lo[2], cb[0].hi[0]);
+        p.b = make_float4(cb[0].hi[1], cb[0].hi[2], cb[1].lo[0], cb[1].lo[1]);
+        p.c = make_float4(cb[1].lo[2], cb[1].hi[0], cb[1].hi[1], cb[1].hi[2]);
+        p.d = make_int4(desc[0], desc[1], 0, 0);
+        pairs[i] = p;
+        atomicAdd(&flags[1], 1);
+        int depth = 0;
+        for (int c = i; c != 0; c = parent[c]) ++depth;
+        atomicMax(&flags[2], depth + 1);
+}
+
+__global__ void k_root_leaf(TriRec *tris, int n) {
+        if (threadIdx.x == 0 && blockIdx.x == 0) tris[0].p0.w = __int_as_float(n);
+}
+
+template <class T>
+static bool dalloc(T **p, size_t count, const char *what) {
+        return hip_ok(hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T)), what);
+}
+
+}  // namespace gz
+
+using namespace gz;
+
+extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const TriangleMesh_C *meshes,
+                                    int64_t nMeshes, const int64_t *ptsCounts, const int64_t *faceCounts,
+                                    const int64_t *vtxCounts, const Material_C *materials, int64_t nMaterials,
+                                    const GzBuildOptions *options) {
+        clear_error();
+        const int maxLeaf = options ? options->maxLeaf : 4;
+        if (nPrims < 0 || nMeshes < 0 || nMaterials < 0 || maxLeaf < 1 || maxLeaf > 64) {
+                set_error("gz_gpu_build_scene: negative count or maxLeaf %d outside 1..64", maxLeaf);
+                return nullptr;
+        }
+        if ((nPrims > 0 && !prims) || (nMeshes > 0 && (!meshes || !ptsCounts || !vtxCounts)) ||
+            (nMaterials > 0 && !materials)) {
+                set_error("gz_gpu_build_scene: null array with a non-zero count");
+                return nullptr;
+        }
+        if (nPrims >= (1LL << 30)) {
+                set_error("gz_gpu_build_scene: %lld primitives (at most 2^30 - 1)", (long long)nPrims);
+                return nullptr;
+        }
+        if (!gz_gpu_available()) {
+                set_error("gz_gpu_build_scene: no gfx950 device visible");
+                return nullptr;
+        }
+        Scene *s = new (std::nothrow) Scene();
+        if (!s) {
+                set_error("gz_gpu_build_scene: out of host memory");
+                return nullptr;
+        }
+        if (!hip_ok(hipGetDevice(&s->device), "hipGetDevice") ||
+            !scene_upload_common(s, prims, nPrims, meshes, nMeshes, ptsCounts, faceCounts, vtxCounts, materials,
+                                 nMaterials)) {
+                scene_release(s);
+                return nullptr;
+        }
+        const int n = (int)nPrims;
+        s->primCount = nPrims;
+        s->meshCount = nMeshes;
+        s->materialCount = nMaterials;
+        s->empty = n == 0;
+        if (n == 0) return s;
+        const hipStream_t st = s->stream;
+        float4 *verts = nullptr, *cent = nullptr;
+        Box *primBox = nullptr, *nodeBox = nullptr;
+        unsigned *cb = nullptr, *arrivals = nullptr;
+        int *flags = nullptr, *parent = nullptr, *leafParent = nullptr;
+        uint64_t *keys = nullptr, *keys2 = nullptr;
+        uint32_t *vals = nullptr, *order = nullptr;
+        int4 *node = nullptr;
+        PrimId_C *sortedPrims = nullptr;
+        void *sortTmp = nullptr;
+        size_t sortBytes = 0;
+        const int m = std::max(n - 1, 1);
+        bool ok = dalloc(&verts, 3 * (size_t)n, "build vertices") && dalloc(&cent, n, "build centroids") &&
+                  dalloc(&primBox, n, "build boxes") && dalloc(&nodeBox, m, "build node boxes") &&
+                  dalloc(&cb, 6, "build bounds") && dalloc(&arrivals, m, "build counters") &&
+                  dalloc(&flags, 4, "build flags") && dalloc(&parent, m, "build parents") &&
+                  dalloc(&leafParent, n, "build parents") && dalloc(&keys, n, "build keys") &&
+                  dalloc(&keys2, n, "build keys") && dalloc(&vals, n, "build order") && dalloc(&order, n, "build order") &&
+                  dalloc(&node, m, "build nodes") && dalloc(&sortedPrims, n, "build primIds") &&
+                  dalloc(&s->dTris, n, "build triangle records") && dalloc(&s->dNodes, m, "build node records");
+        if (ok) ok = hip_ok(rocprim::radix_sort_pairs(nullptr, sortBytes, keys, keys2, vals, order, (unsigned)n, 0, 63, st),
+                            "radix sort size") &&
+                     hip_ok(hipMalloc(&sortTmp, std::max<size_t>(sortBytes, 1)), "radix sort storage");
+        int h[4] = {0, 0, 0, 0};
+        if (ok) {
+                const unsigned gb = (unsigned)((n + kBuildBlock - 1) / kBuildBlock);
+                hipLaunchKernelGGL(k_build_init, dim3(1), dim3(64), 0, st, cb, flags);
+                hipLaunchKernelGGL(k_gather, dim3(gb), dim3(kBuildBlock), 0, st, s->dPrims, n, s->dMeshes, nMeshes, verts,
+                                   primBox, cent, cb, flags);
+                hipLaunchKernelGGL(k_morton, dim3(gb), dim3(kBuildBlock), 0, st, cent, n, cb, keys, vals);
+                ok = hip_ok(hipGetLastError(), "build launch") &&
+                     hip_ok(rocprim::radix_sort_pairs(sortTmp, sortBytes, keys, keys2, vals, order, (unsigned)n, 0, 63, st),
+                            "radix sort");
+        }
+        if (ok) {
+                const unsigned gb = (unsigned)((n + kBuildBlock - 1) / kBuildBlock);
+                hipLaunchKernelGGL(k_sorted, dim3(gb), dim3(kBuildBlock), 0, st, order, verts, s->dPrims, n, s->dTris,
+                                   sortedPrims);
+                if (n > 1) {
+                        const unsigned gi = (unsigned)((n - 1 + kBuildBlock - 1) / kBuildBlock);
+                        ok = hip_ok(hipMemsetAsync(arrivals, 0, sizeof(unsigned) * (size_t)m, st), "build counters");
+                        hipLaunchKernelGGL(k_hierarchy, dim3(gi), dim3(kBuildBlock), 0, st, keys2, n, node, parent,
+                                           leafParent);
+                        hipLaunchKernelGGL(k_refit, dim3(gb), dim3(kBuildBlock), 0, st, node, parent, leafParent, order,
+                                           primBox, nodeBox, arrivals, n);
+                        if (n > maxLeaf)
+                                hipLaunchKernelGGL(k_emit, dim3(gi), dim3(kBuildBlock), 0, st, node, parent, nodeBox,
+                                                   primBox, order, n, maxLeaf, s->dNodes, s->dTris, flags);
+                }
+                if (n <= maxLeaf) hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(64), 0, st, s->dTris, n);
+                ok = ok && hip_ok(hipGetLastError(), "build launch") &&
+                     hip_ok(hipMemcpyAsync(h, flags, sizeof h, hipMemcpyDeviceToHost, st), "build flags") &&
+                     hip_ok(hipStreamSynchronize(st), "build");
+        }
+        if (ok && (h[0] & 4)) {
+                set_error("gz_gpu_build_scene: a primId references a mesh, index or vertex outside the mesh arrays");
+                ok = false;
+        }
+        if (ok) {
+                // primIds in leaf order replace the caller's order on the device
+                (void)hipFree(s->dPrims);
+                s->dPrims = sortedPrims;
+                sortedPrims = nullptr;
+                s->dBuildOrder = order;
+                order = nullptr;
+                s->nodeRecords = n > maxLeaf ? m : 0;
+                s->triCount = n;
+                s->boundsSafe = !(h[0] & 3);  // finite boxes within 2^40, and no empty (skipped) ones
+                if (n <= maxLeaf) {
+                        s->rootDesc = ~0;
+                        s->interiorCount = 0;
+                        s->maxDepth = 0;
+                        s->nodeCount = 1;
+                } else {
+                        s->rootDesc = 0;
+                        s->interiorCount = h[1];
+                        s->maxDepth = h[2];
+                        s->nodeCount = (int64_t)h[1] + h[3];
+                }
+                s->deviceBytes += (int64_t)(sizeof(TriRec) * (size_t)n + sizeof(NodePair) * (size_t)m);
+        }
+        void *tmp[] = {verts, cent, primBox, nodeBox, cb, arrivals, flags, parent, leafParent, keys, keys2, vals,
+                       order, node, sortedPrims, sortTmp};
+        for (void *p : tmp)
+                if (p) (void)hipFree(p);
+        if (!ok) {
+                scene_release(s);
+                return nullptr;
+        }
+        return s;
+}
+
+// The handle's tree in the reference's BVH2 layout (BVH2Node.swift: depth-first, left child at
+// index + 1, right child at `offset`, leaves over [offset, offset + count) of the primIds in leaf
+// order). order[j] is the caller's primitive index of leaf-order position j (for a handle from
+// gz_gpu_upload_scene, the identity). Returns the node count, or a negative GZ_E_* code
+// (GZ_E_INVALID when nodeCapacity is too small: the count needed is 2 x leaves - 1 <= 2 x primCount).
+extern "C" int64_t gz_gpu_export_bvh2(void *handle, BVH2Node *nodes, int64_t nodeCapacity, int32_t *order) {
+        clear_error();
+        if (!handle || nodeCapacity < 0 || (nodeCapacity > 0 && !nodes)) {
+                set_error("gz_gpu_export_bvh2: bad arguments");
+                return GZ_E_INVALID;
+        }
+        Scene *s = static_cast<Scene *>(handle);
+        std::lock_guard<std::mutex> lock(s->mu);
+        if (!enter_device(s)) return GZ_E_HIP;
+        if (s->empty) return 0;
+        std::vector<NodePair> pairs;
+        std::vector<TriRec> tris;
+        std::vector<uint32_t> built;
+        try {
+                // a host upload numbers its records densely; a device build leaves unused slots
+                pairs.resize((size_t)s->nodeRecords);
+                tris.resize((size_t)s->triCount);
+                if (s->dBuildOrder) built.resize((size_t)s->triCount);
+        } catch (const std::bad_alloc &) {
+                set_error("gz_gpu_export_bvh2: out of host memory");
+                return GZ_E_ALLOC;
+        }
+        if (!hip_ok(hipStreamSynchronize(s->stream), "export sync") ||
+            (!pairs.empty() &&
+             !hip_ok(hipMemcpy(pairs.data(), s->dNodes, pairs.size() * sizeof(NodePair), hipMemcpyDeviceToHost),
+                     "export nodes")) ||
+            !hip_ok(hipMemcpy(tris.data(), s->dTris, tris.size() * sizeof(TriRec), hipMemcpyDeviceToHost), "export tris") ||
+            (!built.empty() &&
+             !hip_ok(hipMemcpy(built.data(), s->dBuildOrder, built.size() * sizeof(uint32_t), hipMemcpyDeviceToHost),
+                     "export order")))
+                return GZ_E_HIP;
+        if (order)
+                for (size_t j = 0; j < tris.size(); ++j) {
+                        int32_t k;
+                        std::memcpy(&k, &tris[j].p1.w, 4);
+                        order[j] = built.empty() ? k : (int32_t)built[(size_t)k];
+                }
+        auto leaf_count = [&](int32_t first) {
+                int32_t c;
+                std::memcpy(&c, &tris[(size_t)first].p0.w, 4);
+                return c;
+        };
+        struct Item {
+                int32_t desc;
+                float box[6];
+                int64_t parentSlot;  // node whose offset becomes this node's index (-1: none)
+        };
+        std::vector<Item> stack;
+        int64_t count = 0;
+        // the root's box is the union of its children's (the traversal never tests it)
+        float rootBox[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        if (s->rootDesc >= 0 && (int64_t)s->rootDesc < (int64_t)pairs.size()) {
+                const NodePair &r = pairs[(size_t)s->rootDesc];
+                const float L[6] = {r.a.x, r.a.y, r.a.z, r.a.w, r.b.x, r.b.y};
+                const float R[6] = {r.b.z, r.b.w, r.c.x, r.c.y, r.c.z, r.c.w};
+                for (int q = 0; q < 3; ++q) {
+                        rootBox[q] = std::min(L[q], R[q]);
+                        rootBox[3 + q] = std::max(L[3 + q], R[3 + q]);
+                }
+        }
+        stack.push_back(Item{s->rootDesc, {rootBox[0], rootBox[1], rootBox[2], rootBox[3], rootBox[4], rootBox[5]}, -1});
+        while (!stack.empty()) {
+                const Item it = stack.back();
+                stack.pop_back();
+                if (count >= nodeCapacity) {
+                        set_error("gz_gpu_export_bvh2: more than %lld nodes", (long long)nodeCapacity);
+                        return GZ_E_INVALID;
+                }
+                const int64_t me = count++;
+                if (it.parentSlot >= 0) nodes[it.parentSlot].offset = (int32_t)me;
+                BVH2Node &b = nodes[me];
+                b.boundsMinX = it.box[0];
+                b.boundsMinY = it.box[1];
+                b.boundsMinZ = it.box[2];
+                b.boundsMaxX = it.box[3];
+                b.boundsMaxY = it.box[4];
+                b.boundsMaxZ = it.box[5];
+                if (it.desc < 0) {
+                        if ((int64_t)~it.desc >= (int64_t)tris.size()) {
+                                set_error("gz_gpu_export_bvh2: leaf record %d outside %zu", ~it.desc, tris.size());
+                                return GZ_E_INVALID;
+                        }
+                        b.offset = ~it.desc;
+                        b.count = leaf_count(~it.desc);
+                        continue;
+                }
+                if ((int64_t)it.desc >= (int64_t)pairs.size()) {
+                        set_error("gz_gpu_export_bvh2: node record %d outside %zu", it.desc, pairs.size());
+                        return GZ_E_INVALID;
+                }
+                b.count = 0;
+                b.offset = -1;
+                const NodePair &p = pairs[(size_t)it.desc];
+                // right first on the stack so the left child is visited next (index me + 1)
+                stack.push_back(Item{p.d.y, {p.b.z, p.b.w, p.c.x, p.c.y, p.c.z, p.c.w}, me});
+                stack.push_back(Item{p.d.x, {p.a.x, p.a.y, p.a.z, p.a.w, p.b.x, p.b.y}, -1});
+        }
+        return count;
+}

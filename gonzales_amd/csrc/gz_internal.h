@@ -71,7 +71,9 @@ struct Scene {
         bool empty = true;  // no BVH nodes: every query misses
 
         NodePair *dNodes = nullptr;
+        int64_t nodeRecords = 0;  // child-pair records allocated at dNodes
         TriRec *dTris = nullptr;
+        uint32_t *dBuildOrder = nullptr;  // device build: caller's primitive index per leaf position
         PrimId_C *dPrims = nullptr;
         DevMesh *dMeshes = nullptr;
         Material_C *dMaterials = nullptr;
@@ -103,6 +105,14 @@ bool hip_ok(hipError_t e, const char *what);
 bool enter_device(Scene *s);
 // Ensures the staging buffer holds at least `bytes`.
 bool ensure_stage(Scene *s, size_t bytes);
+
+// Uploads primIds, materials and the per-mesh arrays, creates the handle's stream, error flag
+// and counters (gz_scene.hip); shared by the host-BVH upload and the device build.
+bool scene_upload_common(Scene *s, const PrimId_C *prims, int64_t nPrims, const TriangleMesh_C *meshes,
+                         int64_t nMeshes, const int64_t *ptsCounts, const int64_t *faceCounts,
+                         const int64_t *vtxCounts, const Material_C *materials, int64_t nMaterials);
+// Frees a handle and everything it owns.
+void scene_release(Scene *s);
 
 // Kernel launchers (gz_traverse.hip).
 int launch_trace(Scene *s, const Ray_C *d_rays, const float *d_tMax, int64_t n, GzHit *d_hits,

@@ -61,20 +61,18 @@ __global__ void k_queue_clear(QueueLevel *ctl, int levels) {
         }
 }
 
-// Runs body(i) for every entry i of level `level` (the identity 0 .. count-1 when qin is null)
-// and appends each i for which body returns true to qout. Every lane of a wave reaches each
+// Runs body(i) for every entry i of a queue of `nin` entries (the identity 0 .. nin-1 when qin is
+// null), taken `chunk` at a time through the atomic `head`, and appends each i for which body
+// returns true to qout through the atomic counter `outN`. Every lane of a wave reaches each
 // ballot: the loop bounds are wave-uniform.
 template <typename Body>
-__device__ __forceinline__ void queue_loop(unsigned count, const int *__restrict__ qin, int *__restrict__ qout,
-                                           QueueLevel *ctl, int level, unsigned chunk, Body &&body) {
+__device__ __forceinline__ void queue_run(unsigned nin, const int *__restrict__ qin, unsigned *head,
+                                          int *__restrict__ qout, unsigned *outN, unsigned chunk, Body &&body) {
         const int lane = (int)(threadIdx.x & 63);
         const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        QueueLevel *in = ctl + level;
-        QueueLevel *out = ctl + level + 1;
-        const unsigned nin = qin ? in->n : count;
         for (;;) {
                 unsigned c = 0;
-                if (lane == 0) c = atomicAdd(&in->head, chunk);
+                if (lane == 0) c = atomicAdd(head, chunk);
                 c = __shfl(c, 0);
                 if (c >= nin) break;
                 const unsigned end = nin - c < chunk ? nin : c + chunk;
@@ -86,12 +84,171 @@ __device__ __forceinline__ void queue_loop(unsigned count, const int *__restrict
                         const uint64_t m = __ballot(go);
                         if (m != 0ull) {
                                 unsigned base = 0;
-                                if (lane == 0) base = atomicAdd(&out->n, (unsigned)__popcll(m));
+                                if (lane == 0) base = atomicAdd(outN, (unsigned)__popcll(m));
                                 base = __shfl(base, 0);
                                 if (go) qout[base + (unsigned)__popcll(m & below)] = i;
                         }
                 }
         }
+}
+
+// The level form: entries of level `level` (identity at level 0), survivors to level + 1.
+template <typename Body>
+__device__ __forceinline__ void queue_loop(unsigned count, const int *__restrict__ qin, int *__restrict__ qout,
+                                           QueueLevel *ctl, int level, unsigned chunk, Body &&body) {
+        const unsigned nin = qin ? ctl[level].n : count;
+        queue_run(nin, qin, &ctl[level].head, qout, &ctl[level + 1].n, chunk, body);
+}
+
+// ---- Split wavefront: one trace kernel and two shade kernels per bounce level ----------------
+//
+// k_wf_trace: persistent lanes with refill trace the closest hit of every path of the level's
+// queue (the batch traversal's refill scheme: idle lanes take new paths when `refill` of them
+// are idle), store it, and sort the path by the BSDF its hit shades with into the diffuse queue
+// (misses and stopped paths too) or the queue of the other BSDFs: one ballot per queue and
+// step, one atomic per wave and queue. k_wf_shade<kAll> then runs the rest of the bounce
+// (bounce_rest) over one class queue: the diffuse kernel is the small one (4 waves/SIMD), only
+// the other class pays for the 256-register kernel. Survivors go to the next level's queue.
+struct WfLevel {
+        unsigned qn, qhead;  // the level's queue (appended by the previous level's shade kernels)
+        unsigned dn, dhead;  // diffuse class
+        unsigned an, ahead;  // other BSDFs
+        unsigned pad[2];
+};
+
+__global__ void k_wf_clear(WfLevel *ctl, int levels) {
+        for (int k = threadIdx.x; k < levels; k += blockDim.x) ctl[k] = WfLevel{0u, 0u, 0u, 0u, 0u, 0u, {0u, 0u}};
+}
+
+template <bool kStats>
+__global__ __launch_bounds__(kBlock) void k_wf_trace(MisArgs a, const GzPath *__restrict__ paths, unsigned count,
+                                                     const int *__restrict__ qin, WfLevel *ctl, int level,
+                                                     GzHit *__restrict__ hits, int *__restrict__ qd,
+                                                     int *__restrict__ qa, bool split, int refill, unsigned chunk,
+                                                     uint32_t *__restrict__ stats) {
+        Stack<kLdsMax> S;
+        Lane L;
+        int cur = -1;       // path index in flight on this lane
+        bool done = false;  // this lane finished a path this step
+        unsigned resBase = 0, resCount = 0;
+        bool counterDone = false, exhausted = false;
+        const int laneId = (int)(threadIdx.x & 63);
+        const uint64_t below = laneId == 0 ? 0ull : (~0ull >> (64 - laneId));
+        WfLevel &lv = ctl[level];
+        const unsigned nin = qin ? lv.qn : count;
+        for (;;) {
+                if (!exhausted) {
+                        const uint64_t idle = __ballot(cur < 0);
+                        const int nIdle = __popcll(idle);
+                        if (nIdle >= refill || nIdle == 64) {
+                                const unsigned rank = (unsigned)__popcll(idle & below);
+                                const unsigned need = (unsigned)nIdle;
+                                unsigned fresh = 0, take = 0;
+                                if (resCount < need && !counterDone) {
+                                        take = need - resCount > chunk ? need - resCount : chunk;
+                                        const int leader = __ffsll((unsigned long long)idle) - 1;
+                                        if (laneId == leader) fresh = atomicAdd(&lv.qhead, take);
+                                        fresh = __shfl(fresh, leader);
+                                        if (fresh >= nin || nin - fresh <= take) counterDone = true;
+                                }
+                                if (cur < 0) {
+                                        unsigned mine = 0xffffffffu;
+                                        if (rank < resCount)
+                                                mine = resBase + rank;
+                                        else if (rank - resCount < take)
+                                                mine = fresh + (rank - resCount);
+                                        if (mine < nin) {
+                                                cur = qin ? qin[mine] : (int)mine;
+                                                const GzPath &p = paths[cur];
+                                                if (path_live(a, p)) {
+                                                        lane_init(L, p.ray, INFINITY, a.rootDesc, a.sceneSafe);
+                                                        done = a.empty;  // no nodes: a miss, nothing to step
+                                                } else {
+                                                        L.hitPrim = -1;
+                                                        L.nInterior = L.nTris = 0;
+                                                        done = true;  // stopped: the shade kernel clears it
+                                                }
+                                        }
+                                }
+                                if (take > 0) {
+                                        const unsigned used = need - resCount;
+                                        resBase = fresh + used;
+                                        resCount = take - used;
+                                } else {
+                                        const unsigned used = need < resCount ? need : resCount;
+                                        resBase += used;
+                                        resCount -= used;
+                                }
+                                if (counterDone && (resCount == 0 || resBase >= nin)) exhausted = true;
+                        }
+                }
+                if (cur >= 0 && !done) {
+                        const int st = step<false, kStats, kLdsMax>(L, S, a.nodes, a.tris);
+                        if (st != kStepGo) {
+                                if (st == kStepOverflow) {
+                                        atomicOr(a.err, 1);
+                                        L.hitPrim = -1;
+                                }
+                                done = true;
+                        }
+                }
+                // finished paths: store the hit, sort into the two class queues
+                bool toA = false, toD = false;
+                if (cur >= 0 && done) {
+                        GzHit h;
+                        h.prim = L.hitPrim;
+                        h.t = L.hitPrim >= 0 ? L.tBest : INFINITY;
+                        h.u = L.hitPrim >= 0 ? L.bu : 0.0f;
+                        h.v = L.hitPrim >= 0 ? L.bv : 0.0f;
+                        hits[cur] = h;
+                        if (kStats) {
+                                stats[3 * (size_t)cur] += L.nInterior;
+                                stats[3 * (size_t)cur + 1] += L.nTris;
+                        }
+                        toA = split && hit_needs_all_bsdfs(a, L.hitPrim);
+                        toD = !toA;
+                }
+                const uint64_t md = __ballot(toD), ma = __ballot(toA);
+                if (md != 0ull) {
+                        unsigned b = 0;
+                        if (laneId == 0) b = atomicAdd(&lv.dn, (unsigned)__popcll(md));
+                        b = __shfl(b, 0);
+                        if (toD) qd[b + (unsigned)__popcll(md & below)] = cur;
+                }
+                if (ma != 0ull) {
+                        unsigned b = 0;
+                        if (laneId == 0) b = atomicAdd(&lv.an, (unsigned)__popcll(ma));
+                        b = __shfl(b, 0);
+                        if (toA) qa[b + (unsigned)__popcll(ma & below)] = cur;
+                }
+                if (cur >= 0 && done) {
+                        cur = -1;
+                        done = false;
+                }
+                if (exhausted && __ballot(cur >= 0) == 0ull) break;
+        }
+}
+
+template <bool kStats, bool kAll>
+__global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_wf_shade(MisArgs a, GzPath *__restrict__ paths,
+                                                                      const GzHit *__restrict__ hits,
+                                                                      const int *__restrict__ qin, WfLevel *ctl,
+                                                                      int level, int *__restrict__ qout,
+                                                                      unsigned chunk, uint32_t *__restrict__ stats) {
+        WfLevel &lv = ctl[level];
+        queue_run(kAll ? lv.an : lv.dn, qin, kAll ? &lv.ahead : &lv.dhead, qout, &ctl[level + 1].qn, chunk,
+                  [&](int i) {
+                          const GzPath p = paths[i];
+                          if (!(p.flags & GZ_PATH_ACTIVE)) return false;
+                          if (p.bounce > a.maxDepth) {
+                                  paths[i].flags = p.flags & ~GZ_PATH_ACTIVE;
+                                  return false;
+                          }
+                          const GzHit h = hits[i];
+                          Work w;
+                          return bounce_rest<kStats, kAll, false>(a, paths, i, p, h.prim >= 0, h.prim, h.u, h.v, w,
+                                                                  stats, AdjArgs{nullptr, nullptr}, nullptr);
+                  });
 }
 
 template <bool kStats, bool kAll>
@@ -136,6 +293,12 @@ struct MisQueue {
         size_t cap = 0;
         QueueLevel *ctl = nullptr;
         int levels = 0;
+        // split wavefront: class queues, closest hits, level counters
+        int *qd = nullptr, *qa = nullptr;
+        GzHit *hits = nullptr;
+        size_t wfCap = 0;
+        WfLevel *wctl = nullptr;
+        int wfLevels = 0;
 };
 
 struct MisQueues {
@@ -146,7 +309,7 @@ void release_mis_queues(MisQueues *m) {
         if (!m) return;
         for (auto &kv : m->perStream) {
                 MisQueue &q = kv.second;
-                void *ptrs[] = {q.q[0], q.q[1], q.ctl};
+                void *ptrs[] = {q.q[0], q.q[1], q.ctl, q.qd, q.qa, q.hits, q.wctl};
                 for (void *p : ptrs)
                         if (p) (void)hipFree(p);
         }
@@ -176,6 +339,33 @@ static MisQueue *mis_queue(Scene *s, hipStream_t stream, size_t count, int level
                 q.levels = levels;
         }
         return &q;
+}
+
+// mis_queue plus the split wavefront's class queues, hits and level counters.
+static MisQueue *wf_queue(Scene *s, hipStream_t stream, size_t count, int levels) {
+        MisQueue *q = mis_queue(s, stream, count, levels);
+        if (!q) return nullptr;
+        if (q->wfCap < count) {
+                void *old[] = {q->qd, q->qa, q->hits};
+                for (void *p : old)
+                        if (p) (void)hipFree(p);
+                q->qd = q->qa = nullptr;
+                q->hits = nullptr;
+                q->wfCap = 0;
+                if (!hip_ok(hipMalloc(&q->qd, count * sizeof(int)), "class queue") ||
+                    !hip_ok(hipMalloc(&q->qa, count * sizeof(int)), "class queue") ||
+                    !hip_ok(hipMalloc(&q->hits, count * sizeof(GzHit)), "closest hits"))
+                        return nullptr;
+                q->wfCap = count;
+        }
+        if (q->wfLevels < levels) {
+                if (q->wctl) (void)hipFree(q->wctl);
+                q->wctl = nullptr;
+                q->wfLevels = 0;
+                if (!hip_ok(hipMalloc(&q->wctl, (size_t)levels * sizeof(WfLevel)), "wavefront counters")) return nullptr;
+                q->wfLevels = levels;
+        }
+        return q;
 }
 
 // Grid of a persistent queue kernel: gridScale x the blocks resident per CU x CUs, and no more
@@ -430,6 +620,44 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
         const hipStream_t st = static_cast<hipStream_t>(stream);
         uint32_t *stats = reinterpret_cast<uint32_t *>(d_stats);
         const bool all = !s->mis->allDiffuse;
+        if (toEnd && s->misLaunch.queued == 2) {
+                if (count > 0x7fffffffLL) {
+                        set_error("%s: %lld paths exceed one queue (2^31 - 1)", who, (long long)count);
+                        return GZ_E_INVALID;
+                }
+                const int levels = params->maxDepth + 2;
+                MisQueue *q = wf_queue(s, st, (size_t)count, levels);
+                if (!q) return GZ_E_ALLOC;
+                const GzMisLaunch &c = s->misLaunch;
+                const GzTraceConfig &tc = s->traceConfig;
+                const int refill = tc.refillLanes > 0 && tc.refillLanes <= 64 ? tc.refillLanes : 56;
+                const unsigned tchunk = (unsigned)(tc.chunk >= 1 && tc.chunk <= 4096 ? tc.chunk : 64);
+                auto ktrace = d_stats ? k_wf_trace<true> : k_wf_trace<false>;
+                auto kd = d_stats ? k_wf_shade<true, false> : k_wf_shade<false, false>;
+                auto ka = d_stats ? k_wf_shade<true, true> : k_wf_shade<false, true>;
+                int perCU = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, ktrace, kBlock, 0) != hipSuccess || perCU < 1)
+                        perCU = 1;
+                const unsigned tgrid = (unsigned)std::max<int64_t>(
+                        1, std::min<int64_t>((int64_t)s->cuCount * perCU * c.gridScale, (count + kBlock - 1) / kBlock));
+                const unsigned chunk = (unsigned)c.chunk;
+                const unsigned gd = queue_grid(s, kd, c.block, count, chunk);
+                const unsigned ga = queue_grid(s, ka, c.block, count, chunk);
+                hipLaunchKernelGGL(k_wf_clear, dim3(1), dim3(256), 0, st, q->wctl, levels);
+                for (int level = 0; level <= params->maxDepth; ++level) {
+                        const int *qin = level == 0 ? nullptr : q->q[level & 1];
+                        int *qout = q->q[(level + 1) & 1];
+                        hipLaunchKernelGGL(ktrace, dim3(tgrid), dim3(kBlock), 0, st, a, d_paths, (unsigned)count, qin,
+                                           q->wctl, level, q->hits, q->qd, q->qa, all, refill, tchunk, stats);
+                        hipLaunchKernelGGL(kd, dim3(gd), dim3(c.block), 0, st, a, d_paths, q->hits, q->qd, q->wctl,
+                                           level, qout, chunk, stats);
+                        if (all)
+                                hipLaunchKernelGGL(ka, dim3(ga), dim3(c.block), 0, st, a, d_paths, q->hits, q->qa, q->wctl,
+                                                   level, qout, chunk, stats);
+                        if (!hip_ok(hipGetLastError(), who)) return GZ_E_HIP;
+                }
+                return GZ_OK;
+        }
         if (toEnd && s->misLaunch.queued) {
                 if (count > 0x7fffffffLL) {
                         set_error("%s: %lld paths exceed one queue (2^31 - 1)", who, (long long)count);
@@ -464,7 +692,7 @@ extern "C" int gz_gpu_set_mis_launch(void *handle, const GzMisLaunch *launch) {
                 return GZ_E_INVALID;
         }
         const GzMisLaunch &c = *launch;
-        if ((c.queued != 0 && c.queued != 1) || (c.block != 64 && c.block != 128 && c.block != 256) ||
+        if (c.queued < 0 || c.queued > 2 || (c.block != 64 && c.block != 128 && c.block != 256) ||
             c.gridScale < 1 || c.gridScale > 16 || c.chunk < 64 || c.chunk > 4096 || c.chunk % 64 != 0) {
                 set_error("gz_gpu_set_mis_launch: queued %d block %d gridScale %d chunk %d out of range", c.queued,
                           c.block, c.gridScale, c.chunk);

@@ -67,7 +67,8 @@ static void free_scene(Scene *s) {
         if (s->stream) (void)hipStreamSynchronize(s->stream);
         release_mis(s->mis);
         release_mis_queues(s->misQueues);
-        void *ptrs[] = {s->dNodes, s->dTris, s->dPrims, s->dMeshes, s->dMaterials, s->dErr, s->dStage, s->dCounters};
+        void *ptrs[] = {s->dNodes, s->dTris, s->dPrims, s->dMeshes, s->dMaterials, s->dErr, s->dStage, s->dCounters,
+                        s->dBuildOrder};
         for (void *p : ptrs)
                 if (p) (void)hipFree(p);
         for (int64_t i = 0; i < s->meshBufferCount; ++i)
@@ -87,6 +88,48 @@ static bool upload(T **dst, const T *src, int64_t count, Scene *s, const char *w
                 return false;
         return true;
 }
+
+bool scene_upload_common(Scene *s, const PrimId_C *prims, int64_t nPrims, const TriangleMesh_C *meshes,
+                         int64_t nMeshes, const int64_t *ptsCounts, const int64_t *faceCounts,
+                         const int64_t *vtxCounts, const Material_C *materials, int64_t nMaterials) {
+        if (!s->stream && !hip_ok(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate"))
+                return false;
+        bool ok = upload(&s->dPrims, prims, nPrims, s, "upload primIds") &&
+                  upload(&s->dMaterials, materials, nMaterials, s, "upload materials") &&
+                  hip_ok(hipMalloc((void **)&s->dErr, sizeof(int)), "hipMalloc(err)") &&
+                  hip_ok(hipMemset(s->dErr, 0, sizeof(int)), "hipMemset(err)") &&
+                  hip_ok(hipMalloc((void **)&s->dCounters, sizeof(unsigned long long) * kCounterRing),
+                         "hipMalloc(counters)") &&
+                  hip_ok(hipDeviceGetAttribute(&s->cuCount, hipDeviceAttributeMultiprocessorCount, s->device),
+                         "hipDeviceGetAttribute");
+        s->deviceBytes += (int64_t)(sizeof(unsigned long long) * kCounterRing + sizeof(int));
+        if (ok && nMeshes > 0) {
+                // Per-mesh arrays for the shading kernels (kernel.mojo:488-531 uploads the same).
+                std::vector<DevMesh> dm((size_t)nMeshes);
+                s->meshBuffers = new (std::nothrow) void *[(size_t)(3 * nMeshes)]();
+                if (!s->meshBuffers) {
+                        set_error("scene upload: out of host memory");
+                        ok = false;
+                }
+                for (int64_t i = 0; ok && i < nMeshes; ++i) {
+                        float *pts = nullptr;
+                        int64_t *fi = nullptr, *vi = nullptr;
+                        const int64_t nf = faceCounts ? faceCounts[i] : 0;
+                        ok = upload(&pts, meshes[i].points, ptsCounts[i], s, "upload points") &&
+                             upload(&vi, meshes[i].vertexIndices, vtxCounts[i], s, "upload vertexIndices") &&
+                             upload(&fi, nf > 0 ? meshes[i].faceIndices : nullptr, nf, s, "upload faceIndices");
+                        s->meshBuffers[3 * i] = pts;
+                        s->meshBuffers[3 * i + 1] = vi;
+                        s->meshBuffers[3 * i + 2] = fi;
+                        s->meshBufferCount = 3 * (i + 1);
+                        dm[(size_t)i] = DevMesh{pts, fi, vi, ptsCounts[i], vtxCounts[i]};
+                }
+                if (ok) ok = upload(&s->dMeshes, dm.data(), nMeshes, s, "upload meshes");
+        }
+        return ok;
+}
+
+void scene_release(Scene *s) { free_scene(s); }
 
 static inline int32_t as_int(float f) {
         int32_t i;
@@ -292,41 +335,12 @@ extern "C" void *gz_gpu_upload_scene(const BVH2Node *nodes, int64_t nNodes, cons
                 return nullptr;
         }
 
+        s->nodeRecords = (int64_t)pairs.size();
         bool ok = hip_ok(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate") &&
                   upload(&s->dNodes, pairs.data(), (int64_t)pairs.size(), s, "upload nodes") &&
                   upload(&s->dTris, tris.data(), (int64_t)tris.size(), s, "upload triangles") &&
-                  upload(&s->dPrims, prims, nPrims, s, "upload primIds") &&
-                  upload(&s->dMaterials, materials, nMaterials, s, "upload materials") &&
-                  hip_ok(hipMalloc((void **)&s->dErr, sizeof(int)), "hipMalloc(err)") &&
-                  hip_ok(hipMemset(s->dErr, 0, sizeof(int)), "hipMemset(err)") &&
-                  hip_ok(hipMalloc((void **)&s->dCounters, sizeof(unsigned long long) * kCounterRing),
-                         "hipMalloc(counters)") &&
-                  hip_ok(hipDeviceGetAttribute(&s->cuCount, hipDeviceAttributeMultiprocessorCount, s->device),
-                         "hipDeviceGetAttribute");
-        s->deviceBytes += (int64_t)(sizeof(unsigned long long) * kCounterRing + sizeof(int));
-        if (ok && nMeshes > 0) {
-                // Per-mesh arrays for the shading kernels (kernel.mojo:488-531 uploads the same).
-                std::vector<DevMesh> dm((size_t)nMeshes);
-                s->meshBuffers = new (std::nothrow) void *[(size_t)(3 * nMeshes)]();
-                if (!s->meshBuffers) {
-                        set_error("gz_gpu_upload_scene: out of host memory");
-                        ok = false;
-                }
-                for (int64_t i = 0; ok && i < nMeshes; ++i) {
-                        float *pts = nullptr;
-                        int64_t *fi = nullptr, *vi = nullptr;
-                        const int64_t nf = faceCounts ? faceCounts[i] : 0;
-                        ok = upload(&pts, meshes[i].points, ptsCounts[i], s, "upload points") &&
-                             upload(&vi, meshes[i].vertexIndices, vtxCounts[i], s, "upload vertexIndices") &&
-                             upload(&fi, nf > 0 ? meshes[i].faceIndices : nullptr, nf, s, "upload faceIndices");
-                        s->meshBuffers[3 * i] = pts;
-                        s->meshBuffers[3 * i + 1] = vi;
-                        s->meshBuffers[3 * i + 2] = fi;
-                        s->meshBufferCount = 3 * (i + 1);
-                        dm[(size_t)i] = DevMesh{pts, fi, vi, ptsCounts[i], vtxCounts[i]};
-                }
-                if (ok) ok = upload(&s->dMeshes, dm.data(), nMeshes, s, "upload meshes");
-        }
+                  scene_upload_common(s, prims, nPrims, meshes, nMeshes, ptsCounts, faceCounts, vtxCounts, materials,
+                                      nMaterials);
         if (!ok) {
                 free_scene(s);
                 return nullptr;

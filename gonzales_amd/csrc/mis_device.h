@@ -438,26 +438,24 @@ __device__ __forceinline__ void adjoint_vertex(const MisArgs &a, const Bsdf &b, 
 // One bounce of path i. kStats: also adds the bounce's traversal work (all its rays) to
 // stats[i] and counts the bounce (interior, tris, bounces). kAdj: replays the bounce of the
 // forward pass and adds the gradient of the path's radiance (lds: per-block accumulator).
-// One bounce of path i; returns whether the path goes on to another bounce.
+// Whether a path record takes part in a bounce (VolumePathIntegrator.swift:400-402 stops it past
+// maxDepth).
+__device__ __forceinline__ bool path_live(const MisArgs &a, const GzPath &p) {
+        return (p.flags & GZ_PATH_ACTIVE) && p.bounce <= a.maxDepth;
+}
+
+// The bounce of path i (record p, already loaded and live) after its closest hit: `traced` and
+// (prim, hu, hv) are the closest hit of p.ray. Returns whether the path goes on.
 template <bool kStats, bool kAll, bool kAdj>
-__device__ __forceinline__ bool bounce_one(const MisArgs &a, GzPath *__restrict__ paths, int64_t i,
-                                           uint32_t *__restrict__ stats, const AdjArgs &adj, double *lds) {
-        GzPath p = paths[i];
-        if (!(p.flags & GZ_PATH_ACTIVE)) return false;
-        if (p.bounce > a.maxDepth) {
-                paths[i].flags = p.flags & ~GZ_PATH_ACTIVE;
-                return false;
-        }
-        Work w;
+__device__ __forceinline__ bool bounce_rest(const MisArgs &a, GzPath *__restrict__ paths, int64_t i, GzPath p,
+                                            bool traced, int prim, float hu, float hv, Work &w,
+                                            uint32_t *__restrict__ stats, const AdjArgs &adj, double *lds) {
         f3 T = mk(p.throughput[0], p.throughput[1], p.throughput[2]);
         f3 E = mk(p.estimate[0], p.estimate[1], p.estimate[2]);
         bool specular = (p.flags & GZ_PATH_SPECULAR) != 0;
         bool go = false;
-        int prim = -1;
-        float th, hu, hv;
         Interaction si;
-        const bool hit = trace_ray<false, kStats>(a, p.ray, INFINITY, prim, th, hu, hv, w) &&
-                         surface_interaction(a, p.ray, prim, hu, hv, si);
+        const bool hit = traced && surface_interaction(a, p.ray, prim, hu, hv, si);
         ZSobolState zs;
         zs.morton = (a.zs.log2spp >= 64 ? 0ull : zs_morton2((uint32_t)p.pixelX, (uint32_t)p.pixelY) << a.zs.log2spp) |
                     (uint64_t)(uint32_t)p.sampleIndex;
@@ -593,6 +591,33 @@ __device__ __forceinline__ bool bounce_one(const MisArgs &a, GzPath *__restrict_
                 stats[3 * i + 2] += 1;
         }
         return go;
+}
+
+// One bounce of path i; returns whether the path goes on to another bounce.
+template <bool kStats, bool kAll, bool kAdj>
+__device__ __forceinline__ bool bounce_one(const MisArgs &a, GzPath *__restrict__ paths, int64_t i,
+                                           uint32_t *__restrict__ stats, const AdjArgs &adj, double *lds) {
+        const GzPath p = paths[i];
+        if (!(p.flags & GZ_PATH_ACTIVE)) return false;
+        if (p.bounce > a.maxDepth) {
+                paths[i].flags = p.flags & ~GZ_PATH_ACTIVE;
+                return false;
+        }
+        Work w;
+        int prim = -1;
+        float th, hu, hv;
+        const bool traced = trace_ray<false, kStats>(a, p.ray, INFINITY, prim, th, hu, hv, w);
+        return bounce_rest<kStats, kAll, kAdj>(a, paths, i, p, traced, prim, hu, hv, w, stats, adj, lds);
+}
+
+// The material class a hit shades with: true when its BSDF is not the diffuse one (area-light
+// surfaces shade with materials[0], as in surface_interaction and bsdf_make).
+__device__ __forceinline__ bool hit_needs_all_bsdfs(const MisArgs &a, int prim) {
+        if (prim < 0 || prim >= a.primCount) return false;
+        const PrimId_C pr = a.prims[prim];
+        int m = pr.type == 3 ? 0 : (int)pr.materialIndex;
+        if (m < 0 || m >= a.nMats) m = 0;
+        return a.mats[m].type != GZ_MAT_DIFFUSE;
 }
 
 // Host side (gz_mis.hip): the device arguments of a scene's MIS tables and parameters (false

@@ -358,8 +358,11 @@ int gz_gpu_mis_trace_device(void *handle, struct GzPath *d_paths, int64_t count,
 /* Launch configuration of gz_gpu_mis_trace_device, _trace_stats_device and _adjoint_device
  * (results never depend on it; no counterpart in the reference):
  *   queued    : 1 = persistent waves over device path queues, compacted after every bounce by a
- *               wave ballot and prefix popcount (one atomic per wave); 0 = every bounce launches
- *               one lane per path of all `count` (stopped paths idle in their waves);
+ *               wave ballot and prefix popcount (one atomic per wave); 2 = the same queues with
+ *               each bounce split in a closest-hit kernel (lanes refilled from the queue, hits
+ *               sorted by material class into two queues) and one shade kernel per class, the
+ *               diffuse one small; 0 = every bounce launches one lane per path of all `count`
+ *               (stopped paths idle in their waves). The adjoint replays queue 1 for 1 and 2;
  *   block     : threads per workgroup, 64, 128 or 256;
  *   gridScale : persistent grid = gridScale x the workgroups resident per CU x CUs (1..16);
  *   chunk     : queue entries a wave takes per atomic, a multiple of 64 in 64..4096. */
@@ -551,6 +554,39 @@ int gz_sobol_matrices(uint32_t *out);
 int64_t gz_bvh2_build(const float *primBounds, int64_t n, struct BVH2Node *nodesOut,
                       int64_t *primOrderOut);
 
+/* ---------------------------------------------------------------------------------------- */
+/* BVH build on the device (the next step of the reference's roadmap,                         */
+/* docs/A0_development_roadmap.md:143-146; its builder is the host SAH of                     */
+/* Accelerators/BoundingHierarchyBuilder.swift).                                              */
+/* ---------------------------------------------------------------------------------------- */
+
+/* maxLeaf: subtrees of at most this many primitives become one leaf (1..64, default 4). */
+struct GzBuildOptions {
+        int32_t maxLeaf;
+        int32_t reserved[7];
+};
+
+/* gz_gpu_upload_scene without the BVH: uploads the primIds (any order), meshes and materials and
+ * builds a linear BVH on the device (63-bit Morton codes of the triangle-box centroids, radix
+ * sort, Karras hierarchy, bottom-up boxes, subtrees of <= maxLeaf primitives collapsed into
+ * leaves). The handle keeps the primIds in leaf order, so hit primitive indices are leaf-order
+ * positions (gz_gpu_export_bvh2 gives each position's index in the caller's array). Every other
+ * entry point takes the handle as one from gz_gpu_upload_scene. NULL on failure with
+ * gz_last_error() set; options may be NULL. */
+void *gz_gpu_build_scene(const struct PrimId_C *primIds, int64_t primIdsCount,
+                         const struct TriangleMesh_C *meshes, int64_t meshCount,
+                         const int64_t *meshPointsCounts, const int64_t *meshFaceIndicesCounts,
+                         const int64_t *meshVertexIndicesCounts, const struct Material_C *materials,
+                         int64_t materialCount, const struct GzBuildOptions *options);
+
+/* The handle's tree in the reference's BVH2 layout (BVH2Node.swift: depth-first from node 0,
+ * left child at index + 1, right child at `offset`, a leaf over [offset, offset + count) of the
+ * primIds in leaf order), for host consumers such as BoundingHierarchy's CPU traversal.
+ * order (primIdsCount entries, may be NULL): the caller's index of the primitive at leaf-order
+ * position j. Returns the node count, or a negative GZ_E_* code (GZ_E_INVALID when more than
+ * nodeCapacity nodes; 2 x primIdsCount always suffices). */
+int64_t gz_gpu_export_bvh2(void *handle, struct BVH2Node *nodes, int64_t nodeCapacity, int32_t *order);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
@@ -579,6 +615,7 @@ static_assert(sizeof(struct GzPathAdjoint) == 24, "GzPathAdjoint layout");
 static_assert(sizeof(struct GzMisLaunch) == 32, "GzMisLaunch layout");
 static_assert(sizeof(struct GzBatchScene) == 280, "GzBatchScene layout");
 static_assert(sizeof(struct GzBatchInfo) == 32, "GzBatchInfo layout");
+static_assert(sizeof(struct GzBuildOptions) == 32, "GzBuildOptions layout");
 #endif
 
 #endif /* GONZALES_AMD_H */

@@ -206,14 +206,16 @@ def test_mixed_materials_match_oracle(gpu, sky):
     _check(got, ref, min_frac=0.98)
 
 
-LAUNCHES = [(0, 256, 1, 256), (1, 256, 1, 256), (1, 64, 1, 64), (1, 128, 2, 1024), (1, 256, 4, 4096)]
+LAUNCHES = [(0, 256, 1, 256), (1, 256, 1, 256), (1, 64, 1, 64), (1, 128, 2, 1024), (1, 256, 4, 4096),
+            (2, 256, 1, 64), (2, 128, 2, 256)]
 
 
 @pytest.mark.parametrize("mixed", [False, True])
 def test_queued_launches_equal_the_full_relaunch(gpu, mixed):
-    """Every launch configuration (device path queues compacted by ballot, or one lane per path
-    every bounce) leaves bit-identical path records and work counts: the configuration only
-    changes which lane runs a path and when."""
+    """Every launch configuration (device path queues compacted by ballot, one lane per path
+    every bounce, or the split wavefront: a trace kernel sorting hits by material class, then one
+    shade kernel per class) leaves bit-identical path records and work counts: the configuration
+    only changes which lane runs a path and when."""
     import torch
     from mis_helpers import mixed_material_scene
     if mixed:
@@ -286,7 +288,8 @@ def test_bad_launch_configurations_are_rejected(gpu):
     bh, _ = scenes.cornell_hierarchy()
     bh.upload_to_gpu()
     lib = abi.load()
-    for cfg in [(2, 256, 1, 256), (1, 96, 1, 256), (1, 512, 1, 256), (1, 256, 0, 256), (1, 256, 17, 256),
+    for cfg in [(3, 256, 1, 256), (-1, 256, 1, 256), (1, 96, 1, 256), (1, 512, 1, 256), (1, 256, 0, 256),
+                (1, 256, 17, 256),
                 (1, 256, 1, 32), (1, 256, 1, 100), (1, 256, 1, 8192)]:
         c = abi.GzMisLaunch(*cfg)
         assert lib.gz_gpu_set_mis_launch(bh.handle, ctypes.byref(c)) == abi.GZ_E_INVALID, cfg
