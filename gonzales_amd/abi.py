@@ -100,6 +100,12 @@ class GzMisLaunch(ctypes.Structure):
                 ("chunk", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
 
 
+class GzPlyInfo(ctypes.Structure):
+    _fields_ = [("vertexCount", ctypes.c_int64), ("faceCount", ctypes.c_int64), ("indexCount", ctypes.c_int64),
+                ("hasNormals", ctypes.c_int32), ("hasUvs", ctypes.c_int32), ("hasFaceIndices", ctypes.c_int32),
+                ("bigEndian", ctypes.c_int32)]
+
+
 class GzBuildOptions(ctypes.Structure):
     _fields_ = [("maxLeaf", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
 
@@ -178,6 +184,7 @@ SIGNATURES = {
     "gz_gpu_camera_paths_device": (ctypes.c_int, [_P, _I64, _I64, _P, _P]),
     "gz_gpu_film_add_device": (ctypes.c_int, [_P, _P, _I64, ctypes.c_float, ctypes.c_float, _P, _P, _P, _P, _P]),
     "gz_gpu_film_resolve_device": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
+    "gz_ply_parse": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P]),
     "gz_gpu_build_scene": (ctypes.c_void_p, [_P, _I64, _P, _I64, _P, _P, _P, _P, _I64, _P]),
     "gz_gpu_export_bvh2": (_I64, [_P, _P, _I64, _P]),
     "gz_batch_create": (ctypes.c_void_p, [_P, ctypes.c_int32, _I64]),

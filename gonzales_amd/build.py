@@ -25,7 +25,7 @@ ARCH = os.environ.get("GZ_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["gz_scene.hip", "gz_traverse.hip", "gz_shade.hip", "gz_zsobol.hip", "gz_mis.hip", "gz_film.hip",
                "gz_batch.hip", "gz_build.hip"]
-CXX_SOURCES = ["bvh_build.cpp", "sobol_table.cpp"]
+CXX_SOURCES = ["bvh_build.cpp", "sobol_table.cpp", "gz_ply.cpp"]
 
 # Device code: no fused multiply-add contraction (the traversal is bit-exact against the
 # reference's separately rounded arithmetic), IEEE division/sqrt (hipcc's default), and the

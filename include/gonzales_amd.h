@@ -555,6 +555,29 @@ int64_t gz_bvh2_build(const float *primBounds, int64_t n, struct BVH2Node *nodes
                       int64_t *primOrderOut);
 
 /* ---------------------------------------------------------------------------------------- */
+/* Mesh ingest (Shape/PlyMesh.swift, Shape/Triangle.swift:560-647).                           */
+/* ---------------------------------------------------------------------------------------- */
+
+struct GzPlyInfo {
+        int64_t vertexCount;
+        int64_t faceCount;
+        int64_t indexCount;     /* vertex indices of all faces (3 per triangle) */
+        int32_t hasNormals;
+        int32_t hasUvs;
+        int32_t hasFaceIndices;
+        int32_t bigEndian;
+};
+
+/* Reads a binary PLY file held in memory as PlyMesh does (PlyMesh.swift:24-344): the formats,
+ * vertex property sets and face lists it accepts, and nothing else (ASCII PLY is rejected there
+ * and here). Call with points == NULL to fill `info`, then with arrays of those sizes:
+ * points/normals 3 floats per vertex, uvs 2 per vertex, indices indexCount int64 (faces that
+ * are not triangles are kept as they are, as the reference does), faceIndices one per face.
+ * Returns GZ_OK or GZ_E_INVALID (malformed, unsupported or truncated) with gz_last_error() set. */
+int gz_ply_parse(const uint8_t *data, int64_t size, struct GzPlyInfo *info, float *points, float *normals,
+                 float *uvs, int64_t *indices, int64_t *faceIndices);
+
+/* ---------------------------------------------------------------------------------------- */
 /* BVH build on the device (the next step of the reference's roadmap,                         */
 /* docs/A0_development_roadmap.md:143-146; its builder is the host SAH of                     */
 /* Accelerators/BoundingHierarchyBuilder.swift).                                              */
@@ -616,6 +639,7 @@ static_assert(sizeof(struct GzMisLaunch) == 32, "GzMisLaunch layout");
 static_assert(sizeof(struct GzBatchScene) == 280, "GzBatchScene layout");
 static_assert(sizeof(struct GzBatchInfo) == 32, "GzBatchInfo layout");
 static_assert(sizeof(struct GzBuildOptions) == 32, "GzBuildOptions layout");
+static_assert(sizeof(struct GzPlyInfo) == 40, "GzPlyInfo layout");
 #endif
 
 #endif /* GONZALES_AMD_H */
