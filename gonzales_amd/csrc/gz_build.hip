@@ -7,9 +7,14 @@
 //            kernel.mojo:266-300), the triangle's box (Triangle.worldBound) and the box centroid;
 //            the centroids' bounds by wave reduction + one atomic per wave;
 //   morton   63-bit codes (21 bits per axis) of the centroids in their bounds;
-//   sort     rocPRIM radix sort of (code, primitive) pairs: the sorted order is the leaf order;
+//   sort     LSD radix sort of (code, primitive) pairs, 8 passes of 8 bits: per-block digit counts,
+//            one exclusive scan, a stable scatter ranked by wave ballots (no block waits on
+//            another); the sorted order is the leaf order;
 //   hierarchy one lane per internal node i of the N - 1: its key range and split by the longest
 //            common prefix of the codes (ties broken by position), children and parents;
+//   validate every internal node has one parent (the root none), every leaf one, and each child's
+//            range lies strictly inside its parent's: a tree, so every climb and descent ends;
+//            a scene that fails is rejected;
 //   refit    one lane per leaf climbs to the root; at each node the second child to arrive
 //            (an atomic counter) writes the union of the two child boxes (exact min/max);
 //   emit     every internal node whose range holds more than maxLeaf primitives becomes a
@@ -18,19 +23,21 @@
 //            48-byte triangle records. primIds are stored in the same order, so a hit's
 //            primitive index is a leaf-order position, as after a host build.
 //
-// Every step is a bounded loop (no lane waits on another); a node record is written only by its
-// own lane and a leaf count only by the leaf's parent.
+// Every loop is bounded (climbs and walks by the node count, with every index checked before it
+// is used), no lane waits on another, a node record is written only by its own lane and a leaf
+// count only by the leaf's parent.
 #include <algorithm>
 #include <cstring>
 #include <vector>
-
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include "gz_internal.h"
 
 namespace gz {
 
 constexpr int kBuildBlock = 256;
+
+// flags[0] bits
+enum : int { kFlagUnsafe = 1, kFlagSkip = 2, kFlagBadIndex = 4, kFlagBadTree = 8 };
 
 struct Box {
         float lo[3], hi[3];
@@ -82,14 +89,14 @@ __global__ __launch_bounds__(kBuildBlock) void k_gather(const PrimId_C *__restri
                 if (mesh < 0 && base < 0) {
                         skip = kTriSkip;
                 } else if (mesh < 0 || mesh >= nMeshes || base < 0 || base + 2 >= meshes[mesh].vertexIndexCount) {
-                        atomicOr(&flags[0], 4);
+                        atomicOr(&flags[0], kFlagBadIndex);
                         skip = kTriSkip;
                 } else {
                         const DevMesh m = meshes[mesh];
                         for (int q = 0; q < 3; ++q) {
                                 const int64_t vi = m.vertexIndices[base + q];
                                 if (vi < 0 || vi * 4 + 2 >= m.pointFloats) {
-                                        atomicOr(&flags[0], 4);
+                                        atomicOr(&flags[0], kFlagBadIndex);
                                         skip = kTriSkip;
                                         break;
                                 }
@@ -109,11 +116,11 @@ __global__ __launch_bounds__(kBuildBlock) void k_gather(const PrimId_C *__restri
                                 b.hi[a] = fmaxf(v[0][a], fmaxf(v[1][a], v[2][a]));
                                 c[a] = 0.5f * (b.lo[a] + b.hi[a]);
                                 // the traversal's fast slab form needs finite boxes within 2^40
-                                if (!(b.lo[a] >= -0x1p40f && b.hi[a] <= 0x1p40f)) atomicOr(&flags[0], 1);
+                                if (!(b.lo[a] >= -0x1p40f && b.hi[a] <= 0x1p40f)) atomicOr(&flags[0], kFlagUnsafe);
                         }
                         valid = true;
                 }
-                if (skip) atomicOr(&flags[0], 2);
+                if (skip) atomicOr(&flags[0], kFlagSkip);
                 verts[3 * (size_t)k] = make_float4(v[0][0], v[0][1], v[0][2], 0.0f);
                 verts[3 * (size_t)k + 1] = make_float4(v[1][0], v[1][1], v[1][2], 0.0f);
                 verts[3 * (size_t)k + 2] = make_float4(v[2][0], v[2][1], v[2][2], __int_as_float(skip));
@@ -170,6 +177,86 @@ __global__ __launch_bounds__(kBuildBlock) void k_morton(const float4 *__restrict
         vals[k] = (uint32_t)k;
 }
 
+// ---- LSD radix sort of (key, value) pairs, 8-bit digits --------------------------------------
+constexpr int kSortBlock = 256;
+constexpr int kSortWaves = kSortBlock / 64;
+
+__device__ __forceinline__ unsigned digit_of(uint64_t k, int shift) { return (unsigned)(k >> shift) & 255u; }
+
+// Per block: how many of its keys have each digit; counts is digit-major, [256][nBlocks].
+__global__ __launch_bounds__(kSortBlock) void k_digit_counts(const uint64_t *__restrict__ keys, int n, int shift,
+                                                             unsigned *__restrict__ counts, int nBlocks) {
+        __shared__ unsigned c[256];
+        for (int k = threadIdx.x; k < 256; k += kSortBlock) c[k] = 0u;
+        __syncthreads();
+        const int i = (int)(blockIdx.x * kSortBlock + threadIdx.x);
+        if (i < n) atomicAdd(&c[digit_of(keys[i], shift)], 1u);
+        __syncthreads();
+        for (int k = threadIdx.x; k < 256; k += kSortBlock) counts[(size_t)k * nBlocks + blockIdx.x] = c[k];
+}
+
+// Exclusive scan of `total` counts in place, by one block: each thread scans a contiguous
+// segment, the segment sums are scanned in LDS.
+__global__ __launch_bounds__(1024) void k_digit_scan(unsigned *__restrict__ counts, int total) {
+        __shared__ unsigned sums[1024];
+        const int t = (int)threadIdx.x;
+        const int per = (total + 1023) / 1024;
+        const int lo = min(t * per, total), hi = min(lo + per, total);
+        unsigned s = 0;
+        for (int k = lo; k < hi; ++k) s += counts[k];
+        sums[t] = s;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+                const unsigned v = t >= off ? sums[t - off] : 0u;
+                __syncthreads();
+                sums[t] += v;
+                __syncthreads();
+        }
+        unsigned run = sums[t] - s;
+        for (int k = lo; k < hi; ++k) {
+                const unsigned c = counts[k];
+                counts[k] = run;
+                run += c;
+        }
+}
+
+// Stable scatter: a key's slot is its digit's offset for the block + the keys of that digit in
+// earlier waves of the block + the lanes before it in its wave with the same digit (lanes with
+// the same digit found by one ballot per digit bit).
+__global__ __launch_bounds__(kSortBlock) void k_digit_scatter(const uint64_t *__restrict__ keysIn,
+                                                              const uint32_t *__restrict__ valsIn, int n, int shift,
+                                                              const unsigned *__restrict__ offsets, int nBlocks,
+                                                              uint64_t *__restrict__ keysOut,
+                                                              uint32_t *__restrict__ valsOut, int *__restrict__ flags) {
+        __shared__ unsigned waveCount[kSortWaves][256];
+        const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+        for (int k = threadIdx.x; k < kSortWaves * 256; k += kSortBlock) (&waveCount[0][0])[k] = 0u;
+        __syncthreads();
+        const int i = (int)(blockIdx.x * kSortBlock + threadIdx.x);
+        const bool valid = i < n;
+        const uint64_t key = valid ? keysIn[i] : 0ull;
+        const unsigned d = digit_of(key, shift);
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+                const uint64_t m = __ballot(valid && ((d >> b) & 1u));
+                peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        const unsigned rank = (unsigned)__popcll(peers & below);
+        if (valid && rank == 0u) waveCount[wave][d] = (unsigned)__popcll(peers);
+        __syncthreads();
+        if (!valid) return;
+        unsigned before = 0;
+        for (int w = 0; w < wave; ++w) before += waveCount[w][d];
+        const unsigned dst = offsets[(size_t)d * nBlocks + blockIdx.x] + before + rank;
+        if (dst >= (unsigned)n) {
+                atomicOr(&flags[0], kFlagBadTree);
+                return;
+        }
+        keysOut[dst] = key;
+        valsOut[dst] = valsIn[i];
+}
+
 // Length of the common prefix of sorted keys i and j (positions break ties), -1 outside [0, n).
 __device__ __forceinline__ int delta(const uint64_t *__restrict__ keys, int n, int i, int j) {
         if (j < 0 || j >= n) return -1;
@@ -183,7 +270,8 @@ __device__ __forceinline__ int delta(const uint64_t *__restrict__ keys, int n, i
 // internal node c, c < 0 is leaf ~c (a sorted position).
 __global__ __launch_bounds__(kBuildBlock) void k_hierarchy(const uint64_t *__restrict__ keys, int n,
                                                            int4 *__restrict__ node, int *__restrict__ parent,
-                                                           int *__restrict__ leafParent) {
+                                                           int *__restrict__ leafParent, int *__restrict__ parents,
+                                                           int *__restrict__ leafParents) {
         const int i = (int)(blockIdx.x * kBuildBlock + threadIdx.x);
         if (i >= n - 1) return;
         const int d = delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1) >= 0 ? 1 : -1;
@@ -206,14 +294,55 @@ __global__ __launch_bounds__(kBuildBlock) void k_hierarchy(const uint64_t *__res
         const int left = first == g ? ~g : g;
         const int right = last == g + 1 ? ~(g + 1) : g + 1;
         node[i] = make_int4(first, last, left, right);
-        if (left >= 0)
+        // every index below is in range: first <= g < last lie in [0, n) and g, g + 1 in [0, n - 1]
+        if (g < first || g >= last) return;  // no children recorded: the validation rejects the tree
+        if (left >= 0) {
                 parent[left] = i;
-        else
+                atomicAdd(&parents[left], 1);
+        } else {
                 leafParent[g] = i;
-        if (right >= 0)
+                atomicAdd(&leafParents[g], 1);
+        }
+        if (right >= 0) {
                 parent[right] = i;
-        else
+                atomicAdd(&parents[right], 1);
+        } else {
                 leafParent[g + 1] = i;
+                atomicAdd(&leafParents[g + 1], 1);
+        }
+}
+
+// A tree check: the root spans [0, n - 1] and has no parent, every other internal node and every
+// leaf has exactly one, and each child's range is the left or right part of its parent's split
+// (strictly smaller, so no path can come back to a node). Violations set kFlagBadTree.
+__global__ __launch_bounds__(kBuildBlock) void k_validate(const int4 *__restrict__ node, int n,
+                                                          const int *__restrict__ parents,
+                                                          const int *__restrict__ leafParents, int *__restrict__ flags) {
+        const int i = (int)(blockIdx.x * kBuildBlock + threadIdx.x);
+        const int m = n - 1;
+        bool bad = false;
+        if (i < n && leafParents[i] != 1) bad = true;
+        if (i < m) {
+                const int4 nd = node[i];
+                if (parents[i] != (i == 0 ? 0 : 1)) bad = true;
+                if (i == 0 && (nd.x != 0 || nd.y != m)) bad = true;
+                if (nd.x < 0 || nd.y >= n || nd.x >= nd.y) {
+                        bad = true;
+                } else {
+                        const int g = nd.z >= 0 ? nd.z : ~nd.z;
+                        if (g < nd.x || g >= nd.y) {
+                                bad = true;
+                        } else {
+                                if (nd.z >= 0 ? (nd.z >= m || node[nd.z].x != nd.x || node[nd.z].y != g) : nd.x != g)
+                                        bad = true;
+                                const int r = nd.w >= 0 ? nd.w : ~nd.w;
+                                if (r != g + 1 ||
+                                    (nd.w >= 0 ? (nd.w >= m || node[nd.w].x != g + 1 || node[nd.w].y != nd.y) : nd.y != g + 1))
+                                        bad = true;
+                        }
+                }
+        }
+        if (bad) atomicOr(&flags[0], kFlagBadTree);
 }
 
 __device__ __forceinline__ Box box_union(const Box &a, const Box &b) {
@@ -225,22 +354,45 @@ __device__ __forceinline__ Box box_union(const Box &a, const Box &b) {
         return r;
 }
 
+// The box of child descriptor c (internal node c, or leaf position ~c); false if c is out of range.
+__device__ __forceinline__ bool child_box(int c, const Box *nodeBox, const Box *__restrict__ primBox,
+                                          const uint32_t *__restrict__ order, int n, Box &out) {
+        if (c >= 0) {
+                if (c >= n - 1) return false;
+                out = nodeBox[c];
+                return true;
+        }
+        if (~c >= n) return false;
+        const uint32_t k = order[~c];
+        if (k >= (uint32_t)n) return false;
+        out = primBox[k];
+        return true;
+}
+
 // Bottom-up boxes: the lane of leaf position p climbs while it is the second child to arrive.
 __global__ __launch_bounds__(kBuildBlock) void k_refit(const int4 *__restrict__ node, const int *__restrict__ parent,
                                                        const int *__restrict__ leafParent,
                                                        const uint32_t *__restrict__ order,
                                                        const Box *__restrict__ primBox, Box *nodeBox,
-                                                       unsigned *arrivals, int n) {
+                                                       unsigned *arrivals, int n, int *__restrict__ flags) {
         const int p = (int)(blockIdx.x * kBuildBlock + threadIdx.x);
         if (p >= n) return;
+        const int m = n - 1;
         int cur = leafParent[p];
-        while (cur >= 0) {
+        for (int steps = 0; cur >= 0; ++steps) {
+                if (cur >= m || steps >= m) {
+                        atomicOr(&flags[0], kFlagBadTree);
+                        return;
+                }
                 __threadfence();
                 if (atomicAdd(&arrivals[cur], 1u) == 0u) return;
                 __threadfence();
                 const int4 nd = node[cur];
-                const Box a = nd.z >= 0 ? nodeBox[nd.z] : primBox[order[~nd.z]];
-                const Box b = nd.w >= 0 ? nodeBox[nd.w] : primBox[order[~nd.w]];
+                Box a, b;
+                if (!child_box(nd.z, nodeBox, primBox, order, n, a) || !child_box(nd.w, nodeBox, primBox, order, n, b)) {
+                        atomicOr(&flags[0], kFlagBadTree);
+                        return;
+                }
                 nodeBox[cur] = box_union(a, b);
                 cur = cur == 0 ? -1 : parent[cur];
         }
@@ -250,10 +402,15 @@ __global__ __launch_bounds__(kBuildBlock) void k_refit(const int4 *__restrict__ 
 __global__ __launch_bounds__(kBuildBlock) void k_sorted(const uint32_t *__restrict__ order,
                                                         const float4 *__restrict__ verts,
                                                         const PrimId_C *__restrict__ primsIn, int n,
-                                                        TriRec *__restrict__ tris, PrimId_C *__restrict__ primsOut) {
+                                                        TriRec *__restrict__ tris, PrimId_C *__restrict__ primsOut,
+                                                        int *__restrict__ flags) {
         const int j = (int)(blockIdx.x * kBuildBlock + threadIdx.x);
         if (j >= n) return;
-        const uint32_t k = order[j];
+        uint32_t k = order[j];
+        if (k >= (uint32_t)n) {
+                atomicOr(&flags[0], kFlagBadTree);
+                k = 0;
+        }
         TriRec t;
         t.p0 = verts[3 * (size_t)k];
         t.p1 = verts[3 * (size_t)k + 1];
@@ -264,10 +421,6 @@ __global__ __launch_bounds__(kBuildBlock) void k_sorted(const uint32_t *__restri
         primsOut[j] = primsIn[k];
 }
 
-__device__ __forceinline__ Box child_box(int c, const Box *__restrict__ nodeBox, const Box *__restrict__ primBox,
-                                         const uint32_t *__restrict__ order) {
-        return c >= 0 ? nodeBox[c] : primBox[order[~c]];
-}
 
 // Child-pair records of the internal nodes holding more than maxLeaf primitives, leaf counts in
 // the first triangle record of each leaf, and the depth / node / leaf counts.
@@ -277,17 +430,25 @@ __global__ __launch_bounds__(kBuildBlock) void k_emit(const int4 *__restrict__ n
                                                       NodePair *__restrict__ pairs, TriRec *__restrict__ tris,
                                                       int *__restrict__ flags) {
         const int i = (int)(blockIdx.x * kBuildBlock + threadIdx.x);
-        if (i >= n - 1) return;
+        const int m = n - 1;
+        if (i >= m) return;
         const int4 nd = node[i];
         if (nd.y - nd.x + 1 <= maxLeaf) return;
         int desc[2];
         Box cb[2];
         const int g = nd.z >= 0 ? nd.z : ~nd.z;  // split: left [first, g], right [g + 1, last]
+        if (nd.x < 0 || nd.y >= n || g < nd.x || g >= nd.y) {
+                atomicOr(&flags[0], kFlagBadTree);
+                return;
+        }
         for (int side = 0; side < 2; ++side) {
                 const int c = side == 0 ? nd.z : nd.w;
                 const int cFirst = side == 0 ? nd.x : g + 1;
                 const int cLast = side == 0 ? g : nd.y;
-                cb[side] = child_box(c, nodeBox, primBox, order);
+                if (!child_box(c, nodeBox, primBox, order, n, cb[side])) {
+                        atomicOr(&flags[0], kFlagBadTree);
+                        return;
+                }
                 const int size = cLast - cFirst + 1;
                 if (size <= maxLeaf) {
                         desc[side] = ~cFirst;
@@ -305,7 +466,13 @@ __global__ __launch_bounds__(kBuildBlock) void k_emit(const int4 *__restrict__ n
         pairs[i] = p;
         atomicAdd(&flags[1], 1);
         int depth = 0;
-        for (int c = i; c != 0; c = parent[c]) ++depth;
+        for (int c = i; c != 0; c = parent[c]) {
+                if (c < 0 || c >= m || depth >= m) {
+                        atomicOr(&flags[0], kFlagBadTree);
+                        return;
+                }
+                ++depth;
+        }
         atomicMax(&flags[2], depth + 1);
 }
 
@@ -365,26 +532,24 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
         const hipStream_t st = s->stream;
         float4 *verts = nullptr, *cent = nullptr;
         Box *primBox = nullptr, *nodeBox = nullptr;
-        unsigned *cb = nullptr, *arrivals = nullptr;
-        int *flags = nullptr, *parent = nullptr, *leafParent = nullptr;
+        unsigned *cb = nullptr, *arrivals = nullptr, *counts = nullptr;
+        int *flags = nullptr, *parent = nullptr, *leafParent = nullptr, *parents = nullptr, *leafParents = nullptr;
         uint64_t *keys = nullptr, *keys2 = nullptr;
-        uint32_t *vals = nullptr, *order = nullptr;
+        uint32_t *vals = nullptr, *vals2 = nullptr;
         int4 *node = nullptr;
         PrimId_C *sortedPrims = nullptr;
-        void *sortTmp = nullptr;
-        size_t sortBytes = 0;
         const int m = std::max(n - 1, 1);
+        const int sortBlocks = (n + kSortBlock - 1) / kSortBlock;
         bool ok = dalloc(&verts, 3 * (size_t)n, "build vertices") && dalloc(&cent, n, "build centroids") &&
                   dalloc(&primBox, n, "build boxes") && dalloc(&nodeBox, m, "build node boxes") &&
                   dalloc(&cb, 6, "build bounds") && dalloc(&arrivals, m, "build counters") &&
+                  dalloc(&counts, 256 * (size_t)sortBlocks, "build digit counts") &&
                   dalloc(&flags, 4, "build flags") && dalloc(&parent, m, "build parents") &&
-                  dalloc(&leafParent, n, "build parents") && dalloc(&keys, n, "build keys") &&
-                  dalloc(&keys2, n, "build keys") && dalloc(&vals, n, "build order") && dalloc(&order, n, "build order") &&
+                  dalloc(&leafParent, n, "build parents") && dalloc(&parents, m, "build parent counts") &&
+                  dalloc(&leafParents, n, "build parent counts") && dalloc(&keys, n, "build keys") &&
+                  dalloc(&keys2, n, "build keys") && dalloc(&vals, n, "build order") && dalloc(&vals2, n, "build order") &&
                   dalloc(&node, m, "build nodes") && dalloc(&sortedPrims, n, "build primIds") &&
                   dalloc(&s->dTris, n, "build triangle records") && dalloc(&s->dNodes, m, "build node records");
-        if (ok) ok = hip_ok(rocprim::radix_sort_pairs(nullptr, sortBytes, keys, keys2, vals, order, (unsigned)n, 0, 63, st),
-                            "radix sort size") &&
-                     hip_ok(hipMalloc(&sortTmp, std::max<size_t>(sortBytes, 1)), "radix sort storage");
         int h[4] = {0, 0, 0, 0};
         if (ok) {
                 const unsigned gb = (unsigned)((n + kBuildBlock - 1) / kBuildBlock);
@@ -392,21 +557,36 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
                 hipLaunchKernelGGL(k_gather, dim3(gb), dim3(kBuildBlock), 0, st, s->dPrims, n, s->dMeshes, nMeshes, verts,
                                    primBox, cent, cb, flags);
                 hipLaunchKernelGGL(k_morton, dim3(gb), dim3(kBuildBlock), 0, st, cent, n, cb, keys, vals);
-                ok = hip_ok(hipGetLastError(), "build launch") &&
-                     hip_ok(rocprim::radix_sort_pairs(sortTmp, sortBytes, keys, keys2, vals, order, (unsigned)n, 0, 63, st),
-                            "radix sort");
+                // 8 stable passes of 8 bits; an even number of passes leaves the result in keys, vals
+                uint64_t *kin = keys, *kout = keys2;
+                uint32_t *vin = vals, *vout = vals2;
+                for (int shift = 0; shift < 64; shift += 8) {
+                        hipLaunchKernelGGL(k_digit_counts, dim3((unsigned)sortBlocks), dim3(kSortBlock), 0, st, kin, n,
+                                           shift, counts, sortBlocks);
+                        hipLaunchKernelGGL(k_digit_scan, dim3(1), dim3(1024), 0, st, counts, 256 * sortBlocks);
+                        hipLaunchKernelGGL(k_digit_scatter, dim3((unsigned)sortBlocks), dim3(kSortBlock), 0, st, kin, vin,
+                                           n, shift, counts, sortBlocks, kout, vout, flags);
+                        std::swap(kin, kout);
+                        std::swap(vin, vout);
+                }
+                ok = hip_ok(hipGetLastError(), "build launch");
         }
+        uint32_t *order = vals;  // sorted primitive indices (leaf order)
         if (ok) {
                 const unsigned gb = (unsigned)((n + kBuildBlock - 1) / kBuildBlock);
                 hipLaunchKernelGGL(k_sorted, dim3(gb), dim3(kBuildBlock), 0, st, order, verts, s->dPrims, n, s->dTris,
-                                   sortedPrims);
+                                   sortedPrims, flags);
                 if (n > 1) {
                         const unsigned gi = (unsigned)((n - 1 + kBuildBlock - 1) / kBuildBlock);
-                        ok = hip_ok(hipMemsetAsync(arrivals, 0, sizeof(unsigned) * (size_t)m, st), "build counters");
-                        hipLaunchKernelGGL(k_hierarchy, dim3(gi), dim3(kBuildBlock), 0, st, keys2, n, node, parent,
-                                           leafParent);
+                        ok = hip_ok(hipMemsetAsync(arrivals, 0, sizeof(unsigned) * (size_t)m, st), "build counters") &&
+                             hip_ok(hipMemsetAsync(parents, 0, sizeof(int) * (size_t)m, st), "build counters") &&
+                             hip_ok(hipMemsetAsync(leafParents, 0, sizeof(int) * (size_t)n, st), "build counters");
+                        hipLaunchKernelGGL(k_hierarchy, dim3(gi), dim3(kBuildBlock), 0, st, keys, n, node, parent,
+                                           leafParent, parents, leafParents);
+                        hipLaunchKernelGGL(k_validate, dim3(gb), dim3(kBuildBlock), 0, st, node, n, parents, leafParents,
+                                           flags);
                         hipLaunchKernelGGL(k_refit, dim3(gb), dim3(kBuildBlock), 0, st, node, parent, leafParent, order,
-                                           primBox, nodeBox, arrivals, n);
+                                           primBox, nodeBox, arrivals, n, flags);
                         if (n > maxLeaf)
                                 hipLaunchKernelGGL(k_emit, dim3(gi), dim3(kBuildBlock), 0, st, node, parent, nodeBox,
                                                    primBox, order, n, maxLeaf, s->dNodes, s->dTris, flags);
@@ -416,7 +596,11 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
                      hip_ok(hipMemcpyAsync(h, flags, sizeof h, hipMemcpyDeviceToHost, st), "build flags") &&
                      hip_ok(hipStreamSynchronize(st), "build");
         }
-        if (ok && (h[0] & 4)) {
+        if (ok && (h[0] & kFlagBadTree)) {
+                set_error("gz_gpu_build_scene: the device build produced an inconsistent tree (rejected)");
+                ok = false;
+        }
+        if (ok && (h[0] & kFlagBadIndex)) {
                 set_error("gz_gpu_build_scene: a primId references a mesh, index or vertex outside the mesh arrays");
                 ok = false;
         }
@@ -426,10 +610,10 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
                 s->dPrims = sortedPrims;
                 sortedPrims = nullptr;
                 s->dBuildOrder = order;
-                order = nullptr;
+                vals = nullptr;
                 s->nodeRecords = n > maxLeaf ? m : 0;
                 s->triCount = n;
-                s->boundsSafe = !(h[0] & 3);  // finite boxes within 2^40, and no empty (skipped) ones
+                s->boundsSafe = !(h[0] & (kFlagUnsafe | kFlagSkip));  // finite boxes within 2^40, none empty
                 if (n <= maxLeaf) {
                         s->rootDesc = ~0;
                         s->interiorCount = 0;
@@ -443,8 +627,8 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
                 }
                 s->deviceBytes += (int64_t)(sizeof(TriRec) * (size_t)n + sizeof(NodePair) * (size_t)m);
         }
-        void *tmp[] = {verts, cent, primBox, nodeBox, cb, arrivals, flags, parent, leafParent, keys, keys2, vals,
-                       order, node, sortedPrims, sortTmp};
+        void *tmp[] = {verts, cent, primBox, nodeBox, cb, arrivals, counts, flags, parent, leafParent, parents,
+                       leafParents, keys, keys2, vals, vals2, node, sortedPrims};
         for (void *p : tmp)
                 if (p) (void)hipFree(p);
         if (!ok) {

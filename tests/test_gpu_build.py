@@ -83,13 +83,21 @@ def _check_tree(bh, max_leaf):
     return depth.max()
 
 
+def _step(msg):
+    print(f"[build test] {msg}", flush=True)
+
+
 @pytest.mark.parametrize("max_leaf", [1, 2, 4, 8])
 def test_device_build_traversal_matches_oracle_on_the_exported_tree(gpu, max_leaf):
     sc, prims = _soup_scene(20000)
+    _step(f"device build, 20000 triangles, max_leaf {max_leaf}")
     bh = build_bounding_hierarchy_gpu(sc, prims, max_leaf=max_leaf)
+    _step(f"built: {len(bh.nodes)} nodes; device traversal of 8192 rays")
     rays, tmax = _rays(8192, 0.0, 1.0, 5)
     got = bh.intersect_batch_gpu(rays, tmax)
+    _step("oracle traversal on the exported tree")
     ref = _oracle(bh, rays, tmax)
+    _step("compare")
     assert got.tobytes() == ref.tobytes()
     assert np.array_equal(bh.occluded(rays, tmax), ref["hit"].astype(bool))
     info = bh.info()
