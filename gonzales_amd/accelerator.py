@@ -112,6 +112,16 @@ class Primitives:
         return Primitives(self.kind[order], self.mesh[order], self.tri[order], self.idx[order], self.material[order])
 
 
+def take_records(a: np.ndarray, index=None) -> np.ndarray:
+    """a[index] (or a copy of a) for a structured array, with the padding bytes zero: NumPy copies
+    the fields of a dtype with holes and leaves the holes of the new array uninitialised."""
+    src = a if index is None else a[index]
+    out = np.zeros(len(src), a.dtype)
+    for f in a.dtype.names:
+        out[f] = src[f]
+    return out
+
+
 def pack_prim_ids(prims: Primitives) -> np.ndarray:
     """PrimId packing of BoundingHierarchy.swift:21-58 (before prepareMaterials)."""
     out = np.zeros(len(prims), abi.PRIM_ID)
@@ -336,7 +346,7 @@ def build_bounding_hierarchy_gpu(scene: Scene, prims: Primitives, max_leaf: int 
     if count < 0:
         abi.check(int(count), "gz_gpu_export_bvh2")
     bh.nodes = nodes[:count].copy()
-    bh.prim_ids = np.ascontiguousarray(bh.prim_ids[order[:n]])
+    bh.prim_ids = take_records(bh.prim_ids, order[:n])
     return bh
 
 

@@ -702,6 +702,22 @@ extern "C" int64_t gz_gpu_export_bvh2(void *handle, BVH2Node *nodes, int64_t nod
                         rootBox[q] = std::min(L[q], R[q]);
                         rootBox[3 + q] = std::max(L[3 + q], R[3 + q]);
                 }
+        } else if (s->rootDesc < 0 && (int64_t)~s->rootDesc < (int64_t)tris.size()) {
+                // a root leaf: the box of its triangles (skipped primitives have none)
+                const int32_t first = ~s->rootDesc, count = leaf_count(first);
+                for (int32_t j = first; j < first + count && j < (int32_t)tris.size(); ++j) {
+                        int32_t skip;
+                        std::memcpy(&skip, &tris[(size_t)j].p2.w, 4);
+                        if (skip & kTriSkip) continue;
+                        const float4 v[3] = {tris[(size_t)j].p0, tris[(size_t)j].p1, tris[(size_t)j].p2};
+                        for (const float4 &p : v) {
+                                const float c[3] = {p.x, p.y, p.z};
+                                for (int q = 0; q < 3; ++q) {
+                                        rootBox[q] = std::min(rootBox[q], c[q]);
+                                        rootBox[3 + q] = std::max(rootBox[3 + q], c[q]);
+                                }
+                        }
+                }
         }
         stack.push_back(Item{s->rootDesc, {rootBox[0], rootBox[1], rootBox[2], rootBox[3], rootBox[4], rootBox[5]}, -1});
         while (!stack.empty()) {

@@ -17,7 +17,8 @@ import pytest
 import oracle
 from gonzales_amd import abi, scenes
 from gonzales_amd.accelerator import (Primitives, Scene, TriangleMesh, build_bounding_hierarchy,
-                                      build_bounding_hierarchy_gpu, export_bvh2, pack_prim_ids, sah_cost)
+                                      build_bounding_hierarchy_gpu, export_bvh2, pack_prim_ids, sah_cost,
+                                      take_records)
 
 pytestmark = pytest.mark.gpu
 
@@ -87,23 +88,53 @@ def _step(msg):
     print(f"[build test] {msg}", flush=True)
 
 
+def _same_records(got, ref, what):
+    """Intersection records equal byte for byte; on a difference, a short report (not a byte diff)."""
+    g = got.view(np.uint8).reshape(len(got), -1)
+    r = ref.view(np.uint8).reshape(len(ref), -1)
+    bad = np.nonzero((g != r).any(1))[0]
+    if len(bad):
+        k = bad[:5]
+        print(f"{what}: {len(bad)} of {len(got)} records differ; first rays {k.tolist()}", flush=True)
+        print("  got hit/t/id2", got["hit"][k].tolist(), got["tHit"][k].tolist(), got["primId"]["id2"][k].tolist())
+        print("  ref hit/t/id2", ref["hit"][k].tolist(), ref["tHit"][k].tolist(), ref["primId"]["id2"][k].tolist())
+        cols = np.nonzero((g[bad] != r[bad]).any(0))[0]
+        print("  differing byte offsets", cols.tolist())
+        for f in ("u", "v"):
+            print(f"  {f} got", got[f][k].view(np.uint32).tolist(), "ref", ref[f][k].view(np.uint32).tolist())
+        print("  id1/mat/type got", got["primId"]["id1"][k].tolist(), got["primId"]["materialIndex"][k].tolist(),
+              got["primId"]["type"][k].tolist())
+        print("  id1/mat/type ref", ref["primId"]["id1"][k].tolist(), ref["primId"]["materialIndex"][k].tolist(),
+              ref["primId"]["type"][k].tolist())
+    return len(bad) == 0
+
+
 @pytest.mark.parametrize("max_leaf", [1, 2, 4, 8])
 def test_device_build_traversal_matches_oracle_on_the_exported_tree(gpu, max_leaf):
-    sc, prims = _soup_scene(20000)
-    _step(f"device build, 20000 triangles, max_leaf {max_leaf}")
+    sc, prims = _soup_scene(60000)
+    _step(f"device build, 60000 triangles, max_leaf {max_leaf}")
     bh = build_bounding_hierarchy_gpu(sc, prims, max_leaf=max_leaf)
     _step(f"built: {len(bh.nodes)} nodes; device traversal of 8192 rays")
     rays, tmax = _rays(8192, 0.0, 1.0, 5)
     got = bh.intersect_batch_gpu(rays, tmax)
     _step("oracle traversal on the exported tree")
     ref = _oracle(bh, rays, tmax)
+    _step("host upload of the exported tree")
+    from gonzales_amd.accelerator import BoundingHierarchy
+    up = BoundingHierarchy(sc, take_records(bh.prim_ids), bh.nodes.copy())
+    up.prepare_materials()
+    up.upload_to_gpu()
+    via_upload = up.intersect_batch_gpu(rays, tmax)
+    up.free()
     _step("compare")
-    assert got.tobytes() == ref.tobytes()
+    ok_upload = _same_records(via_upload, ref, "host upload of the export vs oracle")
+    ok_dev = _same_records(got, ref, "device-built handle vs oracle")
+    assert ok_upload and ok_dev
     assert np.array_equal(bh.occluded(rays, tmax), ref["hit"].astype(bool))
     info = bh.info()
     depth = _check_tree(bh, max_leaf)
     assert info.maxDepth == depth
-    assert 0.3 < ref["hit"].mean() < 1.0
+    assert ref["hit"].mean() > 0.02  # tiny soup triangles: a few percent of the rays hit
     bh.free()
 
 
@@ -137,9 +168,9 @@ def test_device_build_scenes_and_mis_trace(gpu):
         scene, prims = make()
         dev = build_bounding_hierarchy_gpu(scene, prims)
         rays, tmax = _rays(4096, -1.0, 1.5, 7)
-        assert dev.intersect_batch_gpu(rays, tmax).tobytes() == _oracle(dev, rays, tmax).tobytes()
+        assert _same_records(dev.intersect_batch_gpu(rays, tmax), _oracle(dev, rays, tmax), "scene")
         from gonzales_amd.accelerator import BoundingHierarchy
-        up = BoundingHierarchy(scene, dev.prim_ids.copy(), dev.nodes.copy())
+        up = BoundingHierarchy(scene, take_records(dev.prim_ids), dev.nodes.copy())
         up.prepare_materials()
         up.upload_to_gpu()
         _, cam = scenes.cornell_hierarchy()
@@ -195,11 +226,16 @@ def test_device_build_edge_cases(gpu):
         sc = Scene([mesh], np.zeros(1, abi.MATERIAL))
         bh = build_bounding_hierarchy_gpu(sc, Primitives.geometric(0, count, 0), max_leaf=max_leaf)
         rays, tmax = _rays(512, 0.3, 0.7, 8)
+        # slightly tilted: a zero direction component gives 0 x inf = NaN in that axis' slab, which
+        # the reference's select-form slab test turns into a miss of every box
+        d = np.array([0.01, 0.02, 1.0]) / np.linalg.norm([0.01, 0.02, 1.0])
         rays["orgZ"] = 0.0
-        rays["dirX"], rays["dirY"], rays["dirZ"] = 0.0, 0.0, 1.0
+        rays["dirX"], rays["dirY"], rays["dirZ"] = d
         got = bh.intersect_batch_gpu(rays, tmax)
-        assert got.tobytes() == _oracle(bh, rays, tmax).tobytes(), count
-        assert got["hit"].any()
+        assert _same_records(got, _oracle(bh, rays, tmax), f"{count} identical triangles"), count
+        if not got["hit"].any():
+            print(f"{count} identical triangles, max_leaf {max_leaf}: no hit; nodes {bh.nodes[:8].tolist()}", flush=True)
+        assert got["hit"].any(), count
         _check_tree(bh, max_leaf)
         bh.free()
     # a skipped primitive (non-triangle shape: id2 == -1) among triangles is never hit
@@ -224,11 +260,11 @@ def test_device_build_edge_cases(gpu):
 def test_device_build_rejects_bad_indices(gpu):
     sc, prims = _soup_scene(100, seed=2)
     ids = pack_prim_ids(prims)
-    bad = ids.copy()
+    bad = take_records(ids)
     bad["id2"][5] = (0 << 32) | 10_000  # triangle past the mesh
     assert not _raw_build(bad, sc.meshes)
     assert "outside" in abi.last_error()
-    bad = ids.copy()
+    bad = take_records(ids)
     bad["id2"][7] = (3 << 32) | 1  # mesh 3 of 1
     assert not _raw_build(bad, sc.meshes)
     assert not _raw_build(ids, sc.meshes, max_leaf=0)
