@@ -123,8 +123,7 @@ __global__ void k_wf_clear(WfLevel *ctl, int levels) {
 template <bool kStats>
 __global__ __launch_bounds__(kBlock) void k_wf_trace(MisArgs a, const GzPath *__restrict__ paths, unsigned count,
                                                      const int *__restrict__ qin, WfLevel *ctl, int level,
-                                                     GzHit *__restrict__ hits, int *__restrict__ qd,
-                                                     int *__restrict__ qa, bool split, int refill, unsigned chunk,
+                                                     GzHit *__restrict__ hits, int refill, unsigned chunk,
                                                      uint32_t *__restrict__ stats) {
         Stack<kLdsMax> S;
         Lane L;
@@ -192,8 +191,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(MisArgs a, const GzPath *__
                                 done = true;
                         }
                 }
-                // finished paths: store the hit, sort into the two class queues
-                bool toA = false, toD = false;
+                // a finished path: store its hit (sorted into the class queues by k_wf_split)
                 if (cur >= 0 && done) {
                         GzHit h;
                         h.prim = L.hitPrim;
@@ -205,27 +203,39 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(MisArgs a, const GzPath *__
                                 stats[3 * (size_t)cur] += L.nInterior;
                                 stats[3 * (size_t)cur + 1] += L.nTris;
                         }
-                        toA = split && hit_needs_all_bsdfs(a, L.hitPrim);
-                        toD = !toA;
-                }
-                const uint64_t md = __ballot(toD), ma = __ballot(toA);
-                if (md != 0ull) {
-                        unsigned b = 0;
-                        if (laneId == 0) b = atomicAdd(&lv.dn, (unsigned)__popcll(md));
-                        b = __shfl(b, 0);
-                        if (toD) qd[b + (unsigned)__popcll(md & below)] = cur;
-                }
-                if (ma != 0ull) {
-                        unsigned b = 0;
-                        if (laneId == 0) b = atomicAdd(&lv.an, (unsigned)__popcll(ma));
-                        b = __shfl(b, 0);
-                        if (toA) qa[b + (unsigned)__popcll(ma & below)] = cur;
-                }
-                if (cur >= 0 && done) {
                         cur = -1;
                         done = false;
                 }
                 if (exhausted && __ballot(cur >= 0) == 0ull) break;
+        }
+}
+
+// The level's queue, 64 entries per wave in order, sorted by the BSDF class of their hits into
+// the two class queues: one ballot per class and 64 entries, one atomic per wave and class.
+__global__ __launch_bounds__(kMisBlock) void k_wf_split(MisArgs a, unsigned count, const int *__restrict__ qin,
+                                                        WfLevel *ctl, int level, const GzHit *__restrict__ hits,
+                                                        int *__restrict__ qd, int *__restrict__ qa, bool split) {
+        WfLevel &lv = ctl[level];
+        const unsigned nin = qin ? lv.qn : count;
+        const int lane = (int)(threadIdx.x & 63);
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        const unsigned waves = gridDim.x * (blockDim.x / 64);
+        for (unsigned w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); (size_t)w * 64 < nin; w += waves) {
+                const unsigned j = w * 64 + (unsigned)lane;
+                int i = -1;
+                if (j < nin) i = qin ? qin[j] : (int)j;
+                const bool toA = i >= 0 && split && hit_needs_all_bsdfs(a, hits[i].prim);
+                const bool toD = i >= 0 && !toA;
+                const uint64_t md = __ballot(toD), ma = __ballot(toA);
+                unsigned bd = 0, ba = 0;
+                if (lane == 0) {
+                        if (md) bd = atomicAdd(&lv.dn, (unsigned)__popcll(md));
+                        if (ma) ba = atomicAdd(&lv.an, (unsigned)__popcll(ma));
+                }
+                bd = __shfl(bd, 0);
+                ba = __shfl(ba, 0);
+                if (toD) qd[bd + (unsigned)__popcll(md & below)] = i;
+                if (toA) qa[ba + (unsigned)__popcll(ma & below)] = i;
         }
 }
 
@@ -642,13 +652,17 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                         1, std::min<int64_t>((int64_t)s->cuCount * perCU * c.gridScale, (count + kBlock - 1) / kBlock));
                 const unsigned chunk = (unsigned)c.chunk;
                 const unsigned gd = queue_grid(s, kd, c.block, count, chunk);
+                const unsigned sgrid = (unsigned)std::max<int64_t>(
+                        1, std::min<int64_t>((int64_t)s->cuCount * 8, (count + kMisBlock - 1) / kMisBlock));
                 const unsigned ga = queue_grid(s, ka, c.block, count, chunk);
                 hipLaunchKernelGGL(k_wf_clear, dim3(1), dim3(256), 0, st, q->wctl, levels);
                 for (int level = 0; level <= params->maxDepth; ++level) {
                         const int *qin = level == 0 ? nullptr : q->q[level & 1];
                         int *qout = q->q[(level + 1) & 1];
                         hipLaunchKernelGGL(ktrace, dim3(tgrid), dim3(kBlock), 0, st, a, d_paths, (unsigned)count, qin,
-                                           q->wctl, level, q->hits, q->qd, q->qa, all, refill, tchunk, stats);
+                                           q->wctl, level, q->hits, refill, tchunk, stats);
+                        hipLaunchKernelGGL(k_wf_split, dim3(sgrid), dim3(kMisBlock), 0, st, a, (unsigned)count, qin,
+                                           q->wctl, level, q->hits, q->qd, q->qa, all);
                         hipLaunchKernelGGL(kd, dim3(gd), dim3(c.block), 0, st, a, d_paths, q->hits, q->qd, q->wctl,
                                            level, qout, chunk, stats);
                         if (all)
