@@ -214,6 +214,20 @@ def make_material(kind: str, params: dict) -> np.ndarray:
     raise ValueError(f"material type {kind!r} is not supported by the MIS kernel")
 
 
+def material_c_table(gz: np.ndarray) -> np.ndarray:
+    """The Material_C table the wavefront diffuse shade reads, as prepareMaterials builds it
+    (BoundingHierarchy.swift:91-150): diffuse -> its reflectance, conductor and coated conductor
+    -> 0.7 grey, dielectric -> 0.9 grey (all shaded as diffuse there)."""
+    out = np.zeros(len(gz), abi.MATERIAL)
+    out["type"] = 1
+    out["albedo"] = 0.5
+    t = gz["type"]
+    out["albedo"][t == abi.GZ_MAT_DIFFUSE] = gz["reflectance"][t == abi.GZ_MAT_DIFFUSE]
+    out["albedo"][(t == abi.GZ_MAT_CONDUCTOR) | (t == abi.GZ_MAT_COATED_CONDUCTOR)] = 0.7
+    out["albedo"][t == abi.GZ_MAT_DIELECTRIC] = 0.9
+    return out
+
+
 @dataclass
 class IngestedScene:
     scene: Scene
@@ -368,13 +382,10 @@ def ingest(path: str) -> IngestedScene:
                     while tk.next() != "]":
                         pass
     gz = abi.concat(mats, abi.GZ_MATERIAL) if mats else make_material("diffuse", {})
-    placeholder = np.zeros(len(gz), abi.MATERIAL)
-    placeholder["type"] = 1
-    placeholder["albedo"] = 0.5
-    out.scene = Scene(meshes, placeholder, light_emission=np.asarray(emission, np.float32).reshape(-1, 3))
+    out.scene = Scene(meshes, material_c_table(gz), light_emission=np.asarray(emission, np.float32).reshape(-1, 3))
     out.prims = Primitives.concat(parts)
     out.materials = gz
     return out
 
 
-__all__ = ["IngestedScene", "ingest", "read_ply", "tokenize", "transform_points", "make_material"]
+__all__ = ["IngestedScene", "ingest", "read_ply", "tokenize", "transform_points", "make_material", "material_c_table"]

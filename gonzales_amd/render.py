@@ -38,3 +38,72 @@ class FrameRenderer:
             self.integrator.trace(self.d_paths, count, stream)
             self.film.add(self.d_paths, count, stream)
             first += count
+
+
+class TileRenderer:
+    """The frame loop the reference renders with (Core/Tile.swift:19-219, "mode A"): camera paths
+    with their filter weight, then up to maxDepth + 1 rounds of closest hit + the wavefront diffuse
+    shade (gz_gpu_shade_device: emission ends a path, a diffuse hit bounces it, anything else
+    stops it; no next-event estimation, no Russian roulette), each path's estimate into the film.
+    The reference draws each path's PCG stream seeds (pcgState, pcgInc) from its xoshiro generator
+    before every shade; here they come from a seeded torch generator on the device.
+
+    Everything stays on the device: GzPath records from the camera kernel, PathState_C records
+    for the shade, rays gathered for the traversal, Intersection_C records for the shade, and
+    the estimates copied back into the GzPath records for the film."""
+
+    def __init__(self, camera: PerspectiveCamera, bh, film: DeviceFilm, max_depth: int, batch_paths: int = 1 << 22,
+                 seed: int = 0):
+        import torch
+        self.camera, self.bh, self.film, self.max_depth = camera, bh, film, int(max_depth)
+        spp = camera.spp
+        self.batch = max(spp, (min(batch_paths, camera.path_count) // spp) * spp)
+        dev = film.light.device
+        n = self.batch
+        self.d_gz = torch.empty(n * abi.GZ_PATH.itemsize, dtype=torch.uint8, device=dev)
+        self.d_ps = torch.zeros(n * abi.PATH_STATE.itemsize, dtype=torch.uint8, device=dev)
+        self.d_rays = torch.empty(n * abi.RAY.itemsize, dtype=torch.uint8, device=dev)
+        self.d_tmax = torch.full((n,), float("inf"), dtype=torch.float32, device=dev)
+        self.d_hits = torch.empty(n * abi.GZ_HIT.itemsize, dtype=torch.uint8, device=dev)
+        self.d_isect = torch.empty(n * abi.INTERSECTION.itemsize, dtype=torch.uint8, device=dev)
+        self.gen = torch.Generator(device=dev)
+        self.gen.manual_seed(seed)
+
+    def render(self, stream=None) -> None:
+        """Adds one frame (every path sample of the crop) to the film."""
+        import torch
+        lib = abi.load()
+        st = abi.stream_ptr(stream)
+        gz_size, ps_size = abi.GZ_PATH.itemsize, abi.PATH_STATE.itemsize
+        off_ray = abi.GZ_PATH.fields["ray"][1]
+        off_est = abi.GZ_PATH.fields["estimate"][1]
+        p_thr = abi.PATH_STATE.fields["throughput"][1]
+        p_est = abi.PATH_STATE.fields["estimate"][1]
+        p_pcg = abi.PATH_STATE.fields["pcgState"][1]
+        p_act = abi.PATH_STATE.fields["active"][1]
+        one = torch.tensor([1.0, 1.0, 1.0], dtype=torch.float32, device=self.d_ps.device).view(torch.uint8)
+        n_all = self.camera.path_count
+        first = 0
+        while first < n_all:
+            n = min(self.batch, n_all - first)
+            self.camera.paths_device(self.d_gz, first, n, stream)
+            gz = self.d_gz[:n * gz_size].view(n, gz_size)
+            ps = self.d_ps[:n * ps_size].view(n, ps_size)
+            ps.zero_()
+            ps[:, 0:24] = gz[:, off_ray:off_ray + 24]
+            ps[:, p_thr:p_thr + 12] = one
+            ps[:, p_act] = 1
+            rays = self.d_rays[:n * 24].view(n, 24)
+            for _ in range(self.max_depth + 1):
+                rays.copy_(ps[:, 0:24])
+                self.bh.closest_device(self.d_rays[:n * 24], self.d_tmax[:n], self.d_hits[:n * 16], stream=stream)
+                abi.check(lib.gz_gpu_expand_hits_device(self.bh.handle, abi.ptr(self.d_hits), abi.ptr(self.d_tmax), n,
+                                                         abi.ptr(self.d_isect), st), "gz_gpu_expand_hits_device")
+                seeds = torch.randint(-(1 << 63), (1 << 63) - 1, (n, 2), dtype=torch.int64,
+                                      device=self.d_ps.device, generator=self.gen)
+                ps[:, p_pcg:p_pcg + 16] = seeds.view(torch.uint8).view(n, 16)
+                abi.check(lib.gz_gpu_shade_device(self.bh.handle, abi.ptr(self.d_ps), n, abi.ptr(self.d_isect), st),
+                          "gz_gpu_shade_device")
+            gz[:, off_est:off_est + 12] = ps[:, p_est:p_est + 12]
+            self.film.add(self.d_gz, n, stream)
+            first += n
