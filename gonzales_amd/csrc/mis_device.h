@@ -74,8 +74,10 @@ struct MisArgs {
 // Per-lane traversal state shared by the rays of one bounce: one stack (so the three inline
 // traversals do not each get their own private array) and the work counters (interior visits
 // and triangle tests, all rays).
+constexpr int kLdsMis = kLdsWide;
+
 struct Work {
-        Stack<kLdsMax> S;
+        Stack<kLdsMis> S;
         uint32_t interior = 0, tris = 0;
 };
 
@@ -88,7 +90,7 @@ __device__ __forceinline__ bool trace_ray(const MisArgs &a, const Ray_C &ray, fl
         lane_init(L, ray, tMax, a.rootDesc, a.sceneSafe);
         int st;
         do {
-                st = step<kAny, kStats, kLdsMax>(L, w.S, a.nodes, a.tris);
+                st = step<kAny, kStats, kLdsMis>(L, w.S, a.nodes, a.tris);
         } while (st == kStepGo);
         if (kStats) {
                 w.interior += L.nInterior;

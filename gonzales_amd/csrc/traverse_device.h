@@ -150,17 +150,29 @@ __device__ __forceinline__ void lane_init(Lane &L, const Ray_C ray, float tMax, 
 
 // The first entries of every lane's stack, [entry][lane of the block]: a lane's pushes and pops
 // hit its own column, so a wave's 64 lanes touch 64 consecutive words (no bank conflicts).
+// Two sizes: 16 rows (16 KB per 256-lane block) for the batch traversal kernels, which run many
+// waves per CU, and 32 rows for the path kernels, whose deep traversals of large scenes would
+// otherwise spill to private (scratch) memory and which run 2 to 4 waves per SIMD.
 constexpr int kLdsMax = 16;
+constexpr int kLdsWide = 32;
 __shared__ int g_stackLds[kLdsMax * kBlock];
+__shared__ int g_stackLdsWide[kLdsWide * kBlock];
 
 // Traversal stack of GZ_STACK_CAPACITY entries: the first kLds live in LDS, the rest in
 // per-lane private memory.
 template <int kLds>
 struct Stack {
+        static_assert(kLds <= kLdsWide, "LDS stack rows");
         int priv[GZ_STACK_CAPACITY - kLds > 0 ? GZ_STACK_CAPACITY - kLds : 1];
+        __device__ __forceinline__ static int *rows() {
+                if constexpr (kLds > kLdsMax)
+                        return g_stackLdsWide;
+                else
+                        return g_stackLds;
+        }
         __device__ __forceinline__ void put(int sp, int v) {
                 if (kLds > 0 && sp < kLds)
-                        g_stackLds[sp * kBlock + threadIdx.x] = v;
+                        rows()[sp * kBlock + threadIdx.x] = v;
                 else
                         priv[sp - kLds] = v;
         }
@@ -168,7 +180,7 @@ struct Stack {
                 // The volatile LDS read keeps the two loads apart: merged, they become one generic
                 // (flat) load that pays the latency of both memories on every pop.
                 typedef __attribute__((address_space(3))) volatile int LdsInt;
-                if (kLds > 0 && sp < kLds) return *(LdsInt *)&g_stackLds[sp * kBlock + threadIdx.x];
+                if (kLds > 0 && sp < kLds) return *(LdsInt *)&rows()[sp * kBlock + threadIdx.x];
                 return priv[sp - kLds];
         }
 };
