@@ -106,6 +106,8 @@ class MisLaunch:
 
 
 MIS_CANDIDATES = (
+    MisLaunch(3, 256, 1, 64),
+    MisLaunch(3, 256, 1, 256),
     MisLaunch(2, 256, 1, 64),
     MisLaunch(2, 256, 1, 256),
     MisLaunch(0, 256, 1, 256),

@@ -110,14 +110,23 @@ __device__ __forceinline__ void queue_loop(unsigned count, const int *__restrict
 // (bounce_rest) over one class queue: the diffuse kernel is the small one (4 waves/SIMD), only
 // the other class pays for the 256-register kernel. Survivors go to the next level's queue.
 struct WfLevel {
-        unsigned qn, qhead;  // the level's queue (appended by the previous level's shade kernels)
-        unsigned dn, dhead;  // diffuse class
-        unsigned an, ahead;  // other BSDFs
-        unsigned pad[2];
+        unsigned qn, qhead;    // the level's queue (appended by the previous level's shade kernels)
+        unsigned dn, dhead;    // diffuse class
+        unsigned an, ahead;    // other BSDFs
+        unsigned sn, shead;    // shadow rays (full wavefront)
+        unsigned mn, mhead;    // BSDF rays (full wavefront)
+        unsigned fdhead, fahead;  // finish passes over the two class queues
+        unsigned pad[4];
 };
+static_assert(sizeof(WfLevel) == 64, "WfLevel");
 
 __global__ void k_wf_clear(WfLevel *ctl, int levels) {
-        for (int k = threadIdx.x; k < levels; k += blockDim.x) ctl[k] = WfLevel{0u, 0u, 0u, 0u, 0u, 0u, {0u, 0u}};
+        for (int k = threadIdx.x; k < levels; k += blockDim.x) {
+                WfLevel z;
+                unsigned *u = reinterpret_cast<unsigned *>(&z);
+                for (int j = 0; j < 16; ++j) u[j] = 0u;
+                ctl[k] = z;
+        }
 }
 
 template <bool kStats>
@@ -210,32 +219,72 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(MisArgs a, const GzPath *__
         }
 }
 
-// The level's queue, 64 entries per wave in order, sorted by the BSDF class of their hits into
-// the two class queues: one ballot per class and 64 entries, one atomic per wave and class.
+// The level's queue sorted by the BSDF class of its hits into the two class queues. A block takes
+// tiles of kSplitPer x 256 entries (each wave kSplitPer runs of 64) and appends each class with
+// one atomic per tile: counters that every wave of the chip hits are the kernel's cost, not the
+// bytes it moves.
+constexpr int kSplitPer = 8;
+
 __global__ __launch_bounds__(kMisBlock) void k_wf_split(MisArgs a, unsigned count, const int *__restrict__ qin,
                                                         WfLevel *ctl, int level, const GzHit *__restrict__ hits,
                                                         int *__restrict__ qd, int *__restrict__ qa, bool split) {
+        constexpr int kWaves = kMisBlock / 64;
+        constexpr size_t kTile = (size_t)kMisBlock * kSplitPer;
+        __shared__ unsigned sCnt[kWaves][2];
+        __shared__ unsigned sBase[2];
         WfLevel &lv = ctl[level];
-        const unsigned nin = qin ? lv.qn : count;
+        const size_t nin = qin ? lv.qn : count;
         const int lane = (int)(threadIdx.x & 63);
+        const int wave = (int)(threadIdx.x >> 6);
         const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        const unsigned waves = gridDim.x * (blockDim.x / 64);
-        for (unsigned w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); (size_t)w * 64 < nin; w += waves) {
-                const unsigned j = w * 64 + (unsigned)lane;
-                int i = -1;
-                if (j < nin) i = qin ? qin[j] : (int)j;
-                const bool toA = i >= 0 && split && hit_needs_all_bsdfs(a, hits[i].prim);
-                const bool toD = i >= 0 && !toA;
-                const uint64_t md = __ballot(toD), ma = __ballot(toA);
-                unsigned bd = 0, ba = 0;
-                if (lane == 0) {
-                        if (md) bd = atomicAdd(&lv.dn, (unsigned)__popcll(md));
-                        if (ma) ba = atomicAdd(&lv.an, (unsigned)__popcll(ma));
+        for (size_t t0 = (size_t)blockIdx.x * kTile; t0 < nin; t0 += (size_t)gridDim.x * kTile) {
+                const size_t w0 = t0 + (size_t)wave * 64 * kSplitPer;
+                int idx[kSplitPer];
+                unsigned bitsD = 0, bitsA = 0;
+                unsigned cd = 0, ca = 0;
+#pragma unroll
+                for (int k = 0; k < kSplitPer; ++k) {
+                        const size_t j = w0 + (size_t)k * 64 + (size_t)lane;
+                        int i = -1;
+                        if (j < nin) i = qin ? qin[j] : (int)j;
+                        idx[k] = i;
+                        const bool toA = i >= 0 && split && hit_needs_all_bsdfs(a, hits[i].prim);
+                        const bool toD = i >= 0 && !toA;
+                        bitsA |= (unsigned)toA << k;
+                        bitsD |= (unsigned)toD << k;
+                        cd += (unsigned)__popcll(__ballot(toD));
+                        ca += (unsigned)__popcll(__ballot(toA));
                 }
-                bd = __shfl(bd, 0);
-                ba = __shfl(ba, 0);
-                if (toD) qd[bd + (unsigned)__popcll(md & below)] = i;
-                if (toA) qa[ba + (unsigned)__popcll(ma & below)] = i;
+                if (lane == 0) {
+                        sCnt[wave][0] = cd;
+                        sCnt[wave][1] = ca;
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                        unsigned td = 0, ta = 0;
+                        for (int w = 0; w < kWaves; ++w) {
+                                td += sCnt[w][0];
+                                ta += sCnt[w][1];
+                        }
+                        sBase[0] = td ? atomicAdd(&lv.dn, td) : 0u;
+                        sBase[1] = ta ? atomicAdd(&lv.an, ta) : 0u;
+                }
+                __syncthreads();
+                unsigned od = sBase[0], oa = sBase[1];
+                for (int w = 0; w < wave; ++w) {
+                        od += sCnt[w][0];
+                        oa += sCnt[w][1];
+                }
+#pragma unroll
+                for (int k = 0; k < kSplitPer; ++k) {
+                        const bool toD = (bitsD >> k) & 1u, toA = (bitsA >> k) & 1u;
+                        const uint64_t md = __ballot(toD), ma = __ballot(toA);
+                        if (toD) qd[od + (unsigned)__popcll(md & below)] = idx[k];
+                        if (toA) qa[oa + (unsigned)__popcll(ma & below)] = idx[k];
+                        od += (unsigned)__popcll(md);
+                        oa += (unsigned)__popcll(ma);
+                }
+                __syncthreads();  // sCnt and sBase are the next tile's
         }
 }
 
@@ -296,6 +345,188 @@ __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 3) void k_mis_adjoint_queue(M
         }
 }
 
+// ---- Full wavefront: every ray of a bounce traced by a traversal kernel -------------------------
+//
+// Per level: k_wf_trace (closest hits of the paths' rays), k_wf_split (class queues), then per
+// class k_wf_phase (shade_phase: the bounce up to its shadow and BSDF rays, which it appends to
+// two ray queues, and the continuation), k_wf_rays twice (any hit of the shadow rays, closest hit
+// of the BSDF rays: lanes refilled from the queue, 7 to 8 waves per SIMD) and k_wf_finish over the
+// class queues (finish_phase). The shade kernels trace nothing, so every traversal runs at the
+// traversal kernels' occupancy.
+
+// Lanes refilled from a queue of path indices; ray `rays[i]` with tMax `tmax`; the any hit
+// into occ[i] or the closest hit into hits[i].
+template <bool kAny, bool kStats>
+__global__ __launch_bounds__(kBlock) void k_wf_rays(MisArgs a, const Ray_C *__restrict__ rays, float tmax,
+                                                    const int *__restrict__ queue, const unsigned *nptr,
+                                                    unsigned *head, uint8_t *__restrict__ occ, GzHit *__restrict__ hits,
+                                                    int refill, unsigned chunk, uint32_t *__restrict__ stats) {
+        Stack<kLdsMax> S;
+        Lane L;
+        int cur = -1;
+        bool done = false;
+        unsigned resBase = 0, resCount = 0;
+        bool counterDone = false, exhausted = false;
+        const int laneId = (int)(threadIdx.x & 63);
+        const uint64_t below = laneId == 0 ? 0ull : (~0ull >> (64 - laneId));
+        const unsigned nin = *nptr;
+        for (;;) {
+                if (!exhausted) {
+                        const uint64_t idle = __ballot(cur < 0);
+                        const int nIdle = __popcll(idle);
+                        if (nIdle >= refill || nIdle == 64) {
+                                const unsigned rank = (unsigned)__popcll(idle & below);
+                                const unsigned need = (unsigned)nIdle;
+                                unsigned fresh = 0, take = 0;
+                                if (resCount < need && !counterDone) {
+                                        take = need - resCount > chunk ? need - resCount : chunk;
+                                        const int leader = __ffsll((unsigned long long)idle) - 1;
+                                        if (laneId == leader) fresh = atomicAdd(head, take);
+                                        fresh = __shfl(fresh, leader);
+                                        if (fresh >= nin || nin - fresh <= take) counterDone = true;
+                                }
+                                if (cur < 0) {
+                                        unsigned mine = 0xffffffffu;
+                                        if (rank < resCount)
+                                                mine = resBase + rank;
+                                        else if (rank - resCount < take)
+                                                mine = fresh + (rank - resCount);
+                                        if (mine < nin) {
+                                                cur = queue[mine];
+                                                lane_init(L, rays[cur], tmax, a.rootDesc, a.sceneSafe);
+                                                done = a.empty;
+                                        }
+                                }
+                                if (take > 0) {
+                                        const unsigned used = need - resCount;
+                                        resBase = fresh + used;
+                                        resCount = take - used;
+                                } else {
+                                        const unsigned used = need < resCount ? need : resCount;
+                                        resBase += used;
+                                        resCount -= used;
+                                }
+                                if (counterDone && (resCount == 0 || resBase >= nin)) exhausted = true;
+                        }
+                }
+                if (cur >= 0 && !done) {
+                        const int st = step<kAny, kStats, kLdsMax>(L, S, a.nodes, a.tris);
+                        if (st != kStepGo) {
+                                if (st == kStepOverflow) {
+                                        atomicOr(a.err, 1);
+                                        L.hitPrim = -1;
+                                }
+                                done = true;
+                        }
+                }
+                if (cur >= 0 && done) {
+                        if (kAny) {
+                                occ[cur] = L.hitPrim >= 0 ? 1 : 0;
+                        } else {
+                                GzHit h;
+                                h.prim = L.hitPrim;
+                                h.t = L.hitPrim >= 0 ? L.tBest : tmax;
+                                h.u = L.hitPrim >= 0 ? L.bu : 0.0f;
+                                h.v = L.hitPrim >= 0 ? L.bv : 0.0f;
+                                hits[cur] = h;
+                        }
+                        if (kStats) {
+                                stats[3 * (size_t)cur] += L.nInterior;
+                                stats[3 * (size_t)cur + 1] += L.nTris;
+                        }
+                        cur = -1;
+                        done = false;
+                }
+                if (exhausted && __ballot(cur >= 0) == 0ull) break;
+        }
+}
+
+// Appends i to q at the counter n for the lanes with `want`: one ballot, one atomic per wave.
+__device__ __forceinline__ void wave_append(bool want, int i, int *__restrict__ q, unsigned *n, uint64_t below,
+                                            int lane) {
+        const uint64_t m = __ballot(want);
+        if (m == 0ull) return;
+        unsigned b = 0;
+        if (lane == 0) b = atomicAdd(n, (unsigned)__popcll(m));
+        b = __shfl(b, 0);
+        if (want) q[b + (unsigned)__popcll(m & below)] = i;
+}
+
+template <bool kStats, bool kAll>
+__global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_wf_phase(
+        MisArgs a, GzPath *__restrict__ paths, const GzHit *__restrict__ hits, const int *__restrict__ qin,
+        WfLevel *ctl, int level, int *__restrict__ qnext, int *__restrict__ qs, int *__restrict__ qm,
+        Pending *__restrict__ pend, Ray_C *__restrict__ srays, Ray_C *__restrict__ mrays, unsigned chunk,
+        uint32_t *__restrict__ stats) {
+        WfLevel &lv = ctl[level];
+        const unsigned nin = kAll ? lv.an : lv.dn;
+        unsigned *head = kAll ? &lv.ahead : &lv.dhead;
+        const int lane = (int)(threadIdx.x & 63);
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        for (;;) {
+                unsigned c = 0;
+                if (lane == 0) c = atomicAdd(head, chunk);
+                c = __shfl(c, 0);
+                if (c >= nin) break;
+                const unsigned end = nin - c < chunk ? nin : c + chunk;
+                for (unsigned j0 = c; j0 < end; j0 += 64) {
+                        const unsigned j = j0 + (unsigned)lane;
+                        int i = -1;
+                        if (j < end) i = qin[j];
+                        bool go = false, shadow = false, mis = false;
+                        if (i >= 0) {
+                                const GzPath p = paths[i];
+                                Pending pd;
+                                pd.flags = 0;
+                                if (!(p.flags & GZ_PATH_ACTIVE)) {
+                                } else if (p.bounce > a.maxDepth) {
+                                        paths[i].flags = p.flags & ~GZ_PATH_ACTIVE;
+                                } else {
+                                        const GzHit h = hits[i];
+                                        Ray_C sr, mr;
+                                        go = shade_phase<kStats, kAll>(a, paths, i, p, h.prim >= 0, h.prim, h.u, h.v, pd,
+                                                                       sr, mr, stats);
+                                        shadow = (pd.flags & kPendShadow) != 0;
+                                        mis = (pd.flags & kPendMis) != 0;
+                                        if (shadow) srays[i] = sr;
+                                        if (mis) mrays[i] = mr;
+                                }
+                                pend[i] = pd;
+                        }
+                        wave_append(go, i, qnext, &ctl[level + 1].qn, below, lane);
+                        wave_append(shadow, i, qs, &lv.sn, below, lane);
+                        wave_append(mis, i, qm, &lv.mn, below, lane);
+                }
+        }
+}
+
+__global__ __launch_bounds__(kMisBlock) void k_wf_finish(MisArgs a, GzPath *__restrict__ paths,
+                                                         const int *__restrict__ qin, const unsigned *nptr,
+                                                         unsigned *head, const Pending *__restrict__ pend,
+                                                         const uint8_t *__restrict__ occ,
+                                                         const Ray_C *__restrict__ mrays,
+                                                         const GzHit *__restrict__ mhits) {
+        const unsigned nin = *nptr;
+        const unsigned chunk = 256;
+        const int lane = (int)(threadIdx.x & 63);
+        for (;;) {
+                unsigned c = 0;
+                if (lane == 0) c = atomicAdd(head, chunk);
+                c = __shfl(c, 0);
+                if (c >= nin) break;
+                const unsigned end = nin - c < chunk ? nin : c + chunk;
+                for (unsigned j = c + (unsigned)lane; j < end; j += 64) {
+                        const int i = qin[j];
+                        const Pending pd = pend[i];
+                        if (!(pd.flags & kPendWork)) continue;
+                        const bool o = (pd.flags & kPendShadow) ? occ[i] != 0 : false;
+                        const Ray_C mr = (pd.flags & kPendMis) ? mrays[i] : Ray_C{0, 0, 0, 0, 0, 0};
+                        const GzHit mh = (pd.flags & kPendMis) ? mhits[i] : GzHit{-1, 0.0f, 0.0f, 0.0f};
+                        finish_phase(a, paths, i, pd, o, mr, mh);
+                }
+        }
+}
+
 // Queue storage of a handle, per stream (grow-only): two ping-pong index arrays and the levels'
 // counters. Traces queued on one stream run in order and share it; each stream has its own.
 struct MisQueue {
@@ -307,6 +538,13 @@ struct MisQueue {
         int *qd = nullptr, *qa = nullptr;
         GzHit *hits = nullptr;
         size_t wfCap = 0;
+        // full wavefront: ray queues, pending terms, rays and their results
+        int *qs = nullptr, *qm = nullptr;
+        Pending *pend = nullptr;
+        Ray_C *srays = nullptr, *mrays = nullptr;
+        uint8_t *occ = nullptr;
+        GzHit *mhits = nullptr;
+        size_t fullCap = 0;
         WfLevel *wctl = nullptr;
         int wfLevels = 0;
 };
@@ -319,7 +557,8 @@ void release_mis_queues(MisQueues *m) {
         if (!m) return;
         for (auto &kv : m->perStream) {
                 MisQueue &q = kv.second;
-                void *ptrs[] = {q.q[0], q.q[1], q.ctl, q.qd, q.qa, q.hits, q.wctl};
+                void *ptrs[] = {q.q[0], q.q[1], q.ctl, q.qd, q.qa, q.hits, q.wctl, q.qs, q.qm, q.pend, q.srays, q.mrays,
+                                q.occ, q.mhits};
                 for (void *p : ptrs)
                         if (p) (void)hipFree(p);
         }
@@ -378,6 +617,33 @@ static MisQueue *wf_queue(Scene *s, hipStream_t stream, size_t count, int levels
         return q;
 }
 
+// wf_queue plus the full wavefront's ray queues, pending terms, rays and results.
+static MisQueue *full_queue(Scene *s, hipStream_t stream, size_t count, int levels) {
+        MisQueue *q = wf_queue(s, stream, count, levels);
+        if (!q) return nullptr;
+        if (q->fullCap < count) {
+                void *old[] = {q->qs, q->qm, q->pend, q->srays, q->mrays, q->occ, q->mhits};
+                for (void *p : old)
+                        if (p) (void)hipFree(p);
+                q->qs = q->qm = nullptr;
+                q->pend = nullptr;
+                q->srays = q->mrays = nullptr;
+                q->occ = nullptr;
+                q->mhits = nullptr;
+                q->fullCap = 0;
+                if (!hip_ok(hipMalloc(&q->qs, count * sizeof(int)), "shadow queue") ||
+                    !hip_ok(hipMalloc(&q->qm, count * sizeof(int)), "BSDF ray queue") ||
+                    !hip_ok(hipMalloc(&q->pend, count * sizeof(Pending)), "pending terms") ||
+                    !hip_ok(hipMalloc(&q->srays, count * sizeof(Ray_C)), "shadow rays") ||
+                    !hip_ok(hipMalloc(&q->mrays, count * sizeof(Ray_C)), "BSDF rays") ||
+                    !hip_ok(hipMalloc(&q->occ, count), "shadow results") ||
+                    !hip_ok(hipMalloc(&q->mhits, count * sizeof(GzHit)), "BSDF ray hits"))
+                        return nullptr;
+                q->fullCap = count;
+        }
+        return q;
+}
+
 // Grid of a persistent queue kernel: gridScale x the blocks resident per CU x CUs, and no more
 // blocks than the first level has chunks of work for.
 template <typename K>
@@ -389,6 +655,12 @@ static unsigned queue_grid(Scene *s, K kern, int block, int64_t count, unsigned 
         const int64_t wavesPerBlock = block / 64;
         const int64_t need = ((count + chunk - 1) / chunk + wavesPerBlock - 1) / wavesPerBlock;
         return (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, need));
+}
+
+// Grid of k_wf_split: up to 8 blocks per CU, no more than the first level has tiles.
+static unsigned split_grid(const Scene *s, int64_t count) {
+        const int64_t tiles = (count + (int64_t)kMisBlock * kSplitPer - 1) / ((int64_t)kMisBlock * kSplitPer);
+        return (unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)s->cuCount * 8, tiles));
 }
 
 // Light and material tables of one scene handle.
@@ -630,6 +902,65 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
         const hipStream_t st = static_cast<hipStream_t>(stream);
         uint32_t *stats = reinterpret_cast<uint32_t *>(d_stats);
         const bool all = !s->mis->allDiffuse;
+        if (toEnd && s->misLaunch.queued == 3) {
+                if (count > 0x7fffffffLL) {
+                        set_error("%s: %lld paths exceed one queue (2^31 - 1)", who, (long long)count);
+                        return GZ_E_INVALID;
+                }
+                const int levels = params->maxDepth + 2;
+                MisQueue *q = full_queue(s, st, (size_t)count, levels);
+                if (!q) return GZ_E_ALLOC;
+                const GzMisLaunch &c = s->misLaunch;
+                const GzTraceConfig &tc = s->traceConfig;
+                const int refill = tc.refillLanes > 0 && tc.refillLanes <= 64 ? tc.refillLanes : 56;
+                const unsigned tchunk = (unsigned)(tc.chunk >= 1 && tc.chunk <= 4096 ? tc.chunk : 64);
+                auto ktrace = d_stats ? k_wf_trace<true> : k_wf_trace<false>;
+                auto kany = d_stats ? k_wf_rays<true, true> : k_wf_rays<true, false>;
+                auto kclose = d_stats ? k_wf_rays<false, true> : k_wf_rays<false, false>;
+                auto kd = d_stats ? k_wf_phase<true, false> : k_wf_phase<false, false>;
+                auto ka = d_stats ? k_wf_phase<true, true> : k_wf_phase<false, true>;
+                auto tgrid_of = [&](auto kern) {
+                        int perCU = 0;
+                        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, kBlock, 0) != hipSuccess ||
+                            perCU < 1)
+                                perCU = 1;
+                        return (unsigned)std::max<int64_t>(
+                                1, std::min<int64_t>((int64_t)s->cuCount * perCU * c.gridScale, (count + kBlock - 1) / kBlock));
+                };
+                const unsigned tgrid = tgrid_of(ktrace), agrid = tgrid_of(kany), cgrid = tgrid_of(kclose);
+                const unsigned chunk = (unsigned)c.chunk;
+                const unsigned gd = queue_grid(s, kd, c.block, count, chunk);
+                const unsigned ga = queue_grid(s, ka, c.block, count, chunk);
+                const unsigned sgrid = (unsigned)std::max<int64_t>(
+                        1, std::min<int64_t>((int64_t)s->cuCount * 8, (count + kMisBlock - 1) / kMisBlock));
+                const unsigned spgrid = split_grid(s, count);
+                hipLaunchKernelGGL(k_wf_clear, dim3(1), dim3(256), 0, st, q->wctl, levels);
+                for (int level = 0; level <= params->maxDepth; ++level) {
+                        const int *qin = level == 0 ? nullptr : q->q[level & 1];
+                        int *qout = q->q[(level + 1) & 1];
+                        WfLevel *lv = q->wctl + level;
+                        hipLaunchKernelGGL(ktrace, dim3(tgrid), dim3(kBlock), 0, st, a, d_paths, (unsigned)count, qin,
+                                           q->wctl, level, q->hits, refill, tchunk, stats);
+                        hipLaunchKernelGGL(k_wf_split, dim3(spgrid), dim3(kMisBlock), 0, st, a, (unsigned)count, qin,
+                                           q->wctl, level, q->hits, q->qd, q->qa, all);
+                        hipLaunchKernelGGL(kd, dim3(gd), dim3(c.block), 0, st, a, d_paths, q->hits, q->qd, q->wctl, level,
+                                           qout, q->qs, q->qm, q->pend, q->srays, q->mrays, chunk, stats);
+                        if (all)
+                                hipLaunchKernelGGL(ka, dim3(ga), dim3(c.block), 0, st, a, d_paths, q->hits, q->qa, q->wctl,
+                                                   level, qout, q->qs, q->qm, q->pend, q->srays, q->mrays, chunk, stats);
+                        hipLaunchKernelGGL(kany, dim3(agrid), dim3(kBlock), 0, st, a, q->srays, 1.0f - 0.00001f, q->qs,
+                                           &lv->sn, &lv->shead, q->occ, nullptr, refill, chunk, stats);
+                        hipLaunchKernelGGL(kclose, dim3(cgrid), dim3(kBlock), 0, st, a, q->mrays, INFINITY, q->qm,
+                                           &lv->mn, &lv->mhead, nullptr, q->mhits, refill, chunk, stats);
+                        hipLaunchKernelGGL(k_wf_finish, dim3(sgrid), dim3(kMisBlock), 0, st, a, d_paths, q->qd, &lv->dn,
+                                           &lv->fdhead, q->pend, q->occ, q->mrays, q->mhits);
+                        if (all)
+                                hipLaunchKernelGGL(k_wf_finish, dim3(sgrid), dim3(kMisBlock), 0, st, a, d_paths, q->qa,
+                                                   &lv->an, &lv->fahead, q->pend, q->occ, q->mrays, q->mhits);
+                        if (!hip_ok(hipGetLastError(), who)) return GZ_E_HIP;
+                }
+                return GZ_OK;
+        }
         if (toEnd && s->misLaunch.queued == 2) {
                 if (count > 0x7fffffffLL) {
                         set_error("%s: %lld paths exceed one queue (2^31 - 1)", who, (long long)count);
@@ -652,8 +983,6 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                         1, std::min<int64_t>((int64_t)s->cuCount * perCU * c.gridScale, (count + kBlock - 1) / kBlock));
                 const unsigned chunk = (unsigned)c.chunk;
                 const unsigned gd = queue_grid(s, kd, c.block, count, chunk);
-                const unsigned sgrid = (unsigned)std::max<int64_t>(
-                        1, std::min<int64_t>((int64_t)s->cuCount * 8, (count + kMisBlock - 1) / kMisBlock));
                 const unsigned ga = queue_grid(s, ka, c.block, count, chunk);
                 hipLaunchKernelGGL(k_wf_clear, dim3(1), dim3(256), 0, st, q->wctl, levels);
                 for (int level = 0; level <= params->maxDepth; ++level) {
@@ -661,8 +990,8 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                         int *qout = q->q[(level + 1) & 1];
                         hipLaunchKernelGGL(ktrace, dim3(tgrid), dim3(kBlock), 0, st, a, d_paths, (unsigned)count, qin,
                                            q->wctl, level, q->hits, refill, tchunk, stats);
-                        hipLaunchKernelGGL(k_wf_split, dim3(sgrid), dim3(kMisBlock), 0, st, a, (unsigned)count, qin,
-                                           q->wctl, level, q->hits, q->qd, q->qa, all);
+                        hipLaunchKernelGGL(k_wf_split, dim3(split_grid(s, count)), dim3(kMisBlock), 0, st, a,
+                                           (unsigned)count, qin, q->wctl, level, q->hits, q->qd, q->qa, all);
                         hipLaunchKernelGGL(kd, dim3(gd), dim3(c.block), 0, st, a, d_paths, q->hits, q->qd, q->wctl,
                                            level, qout, chunk, stats);
                         if (all)
@@ -706,7 +1035,7 @@ extern "C" int gz_gpu_set_mis_launch(void *handle, const GzMisLaunch *launch) {
                 return GZ_E_INVALID;
         }
         const GzMisLaunch &c = *launch;
-        if (c.queued < 0 || c.queued > 2 || (c.block != 64 && c.block != 128 && c.block != 256) ||
+        if (c.queued < 0 || c.queued > 3 || (c.block != 64 && c.block != 128 && c.block != 256) ||
             c.gridScale < 1 || c.gridScale > 16 || c.chunk < 64 || c.chunk > 4096 || c.chunk % 64 != 0) {
                 set_error("gz_gpu_set_mis_launch: queued %d block %d gridScale %d chunk %d out of range", c.queued,
                           c.block, c.gridScale, c.chunk);

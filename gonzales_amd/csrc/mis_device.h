@@ -595,6 +595,191 @@ __device__ __forceinline__ bool bounce_rest(const MisArgs &a, GzPath *__restrict
         return go;
 }
 
+// ---- The bounce in phases (the full wavefront launch) ------------------------------------------
+//
+// bounce_rest with its shadow ray and BSDF ray traced by other kernels: shade_phase does everything
+// up to those two rays and the continuation, leaving the light-strategy term (valid if the shadow
+// ray is unoccluded), the BSDF-strategy factors and the throughput before the continuation in a
+// Pending record; finish_phase adds the two terms once the rays are traced. The arithmetic is
+// bounce_rest's, operation for operation, so the records equal the one-kernel bounce's.
+enum : int { kPendWork = 1, kPendShadow = 2, kPendMis = 4 };
+
+struct Pending {
+        float T[3];   // throughput before the continuation
+        float lc[3];  // light-strategy term if the shadow ray is unoccluded
+        float bf[3];  // BSDF-strategy f |cos| (times the radiance the BSDF ray finds)
+        float wb;     // BSDF-strategy MIS weight
+        float bpdf;
+        float choicePdf;
+        int flags;
+        int pad[3];
+};
+static_assert(sizeof(Pending) == 64, "Pending");
+
+template <bool kStats, bool kAll>
+__device__ __forceinline__ bool shade_phase(const MisArgs &a, GzPath *__restrict__ paths, int64_t i, GzPath p,
+                                            bool traced, int prim, float hu, float hv, Pending &pd, Ray_C &sray,
+                                            Ray_C &mray, uint32_t *__restrict__ stats) {
+        f3 T = mk(p.throughput[0], p.throughput[1], p.throughput[2]);
+        f3 E = mk(p.estimate[0], p.estimate[1], p.estimate[2]);
+        bool specular = (p.flags & GZ_PATH_SPECULAR) != 0;
+        bool go = false;
+        pd.flags = 0;
+        Interaction si;
+        const bool hit = traced && surface_interaction(a, p.ray, prim, hu, hv, si);
+        ZSobolState zs;
+        zs.morton = (a.zs.log2spp >= 64 ? 0ull : zs_morton2((uint32_t)p.pixelX, (uint32_t)p.pixelY) << a.zs.log2spp) |
+                    (uint64_t)(uint32_t)p.sampleIndex;
+        zs.dim = p.dim;
+        ZSobolState lzs;
+        lzs.morton = 0;
+        lzs.dim = p.lightDim;
+        if (!hit) {
+                if ((p.bounce == 0 || specular) && a.hasSky) E = cadd(E, cmul(T, mk(a.sky[0], a.sky[1], a.sky[2])));
+        } else if (!cblack(T) && p.bounce < a.maxDepth) {
+                if ((p.bounce == 0 || specular) && si.areaLight >= 0 && si.areaLight < a.nLights) {
+                        const float4 Lb = a.lights[si.areaLight].L;
+                        const f3 Le = dot(si.normal, si.outgoing) > 0.0f ? mk(Lb.x, Lb.y, Lb.z) : mk(0.0f, 0.0f, 0.0f);
+                        E = cadd(E, cmul(T, Le));
+                }
+                Bsdf b;
+                bsdf_make(a, si, b);
+                if (p.bounce == 0) {
+                        const f3 al = bx_albedo<kAll>(b.b);
+                        p.albedo[0] = al.x;
+                        p.albedo[1] = al.y;
+                        p.albedo[2] = al.z;
+                        p.firstNormal[0] = si.normal.x;
+                        p.firstNormal[1] = si.normal.y;
+                        p.firstNormal[2] = si.normal.z;
+                }
+                float choicePdf;
+                const float ul = zs_get1d(a.zs, lzs, a.sobol);
+                const int choice = choose_light(a, ul, choicePdf);
+                float u0, u1;
+                zs_get2d(a.zs, zs, a.sobol, u0, u1);
+                LightSample ls;
+                light_sample(a, choice, si.position, u0, u1, ls);
+                pd.flags = kPendWork;
+                f3 lcU = mk(0.0f, 0.0f, 0.0f);
+                if (!cblack(ls.L) && !isinf(ls.pdf)) {
+                        // the unoccluded case of bounce_rest; the shadow ray decides it later
+                        LayerRng r1 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightF);
+                        const f3 scatter =
+                                mul(bsdf_eval_world<kAll>(b, si.outgoing, ls.dir, r1), absdot(ls.dir, si.shadingNormal));
+                        const f3 Lest = cmul(scatter, ls.L);
+                        const float lpdf = ls.pdf;
+                        LayerRng r2 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightPdf);
+                        const float bpdfL = bsdf_pdf_world<kAll>(b, si.outgoing, ls.dir, r2);
+                        const float wl = power_heuristic(lpdf, bpdfL);
+                        lcU = lpdf == 0.0f ? mk(0.0f, 0.0f, 0.0f) : divs(mul(Lest, wl), lpdf);
+                        // occluded(): origin from + 1e-4 (target - from), direction target - origin
+                        const f3 o = add(si.position, mul(sub(ls.target, si.position), 0.0001f));
+                        const f3 d = sub(ls.target, o);
+                        sray = Ray_C{o.x, o.y, o.z, d.x, d.y, d.z};
+                        pd.flags |= kPendShadow;
+                }
+                float v0, v1, v2;
+                zs_get3d(a.zs, zs, a.sobol, v0, v1, v2);
+                f3 bf, bwi;
+                float bpdf;
+                LayerRng r3 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallMisSample);
+                bsdf_sample_world<kAll>(b, si.outgoing, v0, v1, v2, bf, bwi, bpdf, r3);
+                bf = mul(bf, absdot(bwi, si.shadingNormal));
+                const bool weighted = choice >= a.nLights || (a.flags & GZ_MIS_FIX_LIGHT_PDF);
+                float wb = 0.0f;
+                if (weighted && !cblack(bf) && bpdf > 0.0f) {
+                        mray = spawn(si, bwi);
+                        const float lden = light_pdf(a, choice, si, bwi);
+                        wb = power_heuristic(bpdf, lden);
+                        pd.flags |= kPendMis;
+                }
+                pd.T[0] = T.x;
+                pd.T[1] = T.y;
+                pd.T[2] = T.z;
+                pd.lc[0] = lcU.x;
+                pd.lc[1] = lcU.y;
+                pd.lc[2] = lcU.z;
+                pd.bf[0] = bf.x;
+                pd.bf[1] = bf.y;
+                pd.bf[2] = bf.z;
+                pd.wb = wb;
+                pd.bpdf = bpdf;
+                pd.choicePdf = choicePdf;
+                float w0, w1, w2;
+                zs_get3d(a.zs, zs, a.sobol, w0, w1, w2);
+                f3 f, wi;
+                float pdf;
+                LayerRng r4 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallContinue);
+                bsdf_sample_world<kAll>(b, si.outgoing, w0, w1, w2, f, wi, pdf, r4);
+                if (pdf != 0.0f && !isnan(pdf)) {
+                        specular = bx_specular<kAll>(b.b);
+                        T = cmul(T, divs(mul(f, absdot(wi, si.normal)), pdf));
+                        p.ray = spawn(si, wi);
+                        go = true;
+                        if (cmax(T) < 1.0f && p.bounce > 1) {
+                                const float m = cmax(T);
+                                const float q = 1.0f - m > 0.0f ? 1.0f - m : 0.0f;
+                                const float u = rr_draw(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce);
+                                if (u < q)
+                                        go = false;
+                                else
+                                        T = divs(T, 1.0f - q);
+                        }
+                }
+        }
+        p.throughput[0] = T.x;
+        p.throughput[1] = T.y;
+        p.throughput[2] = T.z;
+        p.estimate[0] = E.x;
+        p.estimate[1] = E.y;
+        p.estimate[2] = E.z;
+        p.dim = zs.dim;
+        p.lightDim = lzs.dim;
+        p.flags = (go ? GZ_PATH_ACTIVE : 0) | (specular ? GZ_PATH_SPECULAR : 0);
+        if (go) p.bounce += 1;
+        paths[i] = p;
+        if (kStats) stats[3 * i + 2] += 1;
+        return go;
+}
+
+// Adds the bounce's light- and BSDF-strategy terms once the shadow ray (occ) and the BSDF ray
+// (mray, its closest hit mh) are traced.
+__device__ __forceinline__ void finish_phase(const MisArgs &a, GzPath *__restrict__ paths, int64_t i,
+                                             const Pending &pd, bool occ, const Ray_C &mray, const GzHit &mh) {
+        if (!(pd.flags & kPendWork)) return;
+        f3 lc = mk(0.0f, 0.0f, 0.0f);
+        if ((pd.flags & kPendShadow) && !occ) lc = mk(pd.lc[0], pd.lc[1], pd.lc[2]);
+        f3 bc = mk(0.0f, 0.0f, 0.0f);
+        if (pd.flags & kPendMis) {
+                const f3 bwi = mk(mray.dirX, mray.dirY, mray.dirZ);
+                Interaction bsi;
+                f3 bL = mk(0.0f, 0.0f, 0.0f);
+                bool ok = false;
+                if (mh.prim >= 0 && surface_interaction(a, mray, mh.prim, mh.u, mh.v, bsi)) {
+                        if (bsi.areaLight >= 0 && bsi.areaLight < a.nLights) {
+                                const float4 Lb = a.lights[bsi.areaLight].L;
+                                bL = dot(bsi.normal, bwi) > 0.0f ? mk(Lb.x, Lb.y, Lb.z) : mk(0.0f, 0.0f, 0.0f);
+                                ok = true;
+                        }
+                } else if (a.hasSky) {
+                        bL = mk(a.sky[0], a.sky[1], a.sky[2]);
+                        ok = true;
+                }
+                if (ok) {
+                        const f3 best = cmul(mk(pd.bf[0], pd.bf[1], pd.bf[2]), bL);
+                        bc = divs(mul(best, pd.wb), pd.bpdf);
+                }
+        }
+        const f3 est = cadd(lc, bc);
+        const GzPath &p = paths[i];
+        f3 E = mk(p.estimate[0], p.estimate[1], p.estimate[2]);
+        E = cadd(E, cmul(mk(pd.T[0], pd.T[1], pd.T[2]), divs(est, pd.choicePdf)));
+        paths[i].estimate[0] = E.x;
+        paths[i].estimate[1] = E.y;
+        paths[i].estimate[2] = E.z;
+}
+
 // One bounce of path i; returns whether the path goes on to another bounce.
 template <bool kStats, bool kAll, bool kAdj>
 __device__ __forceinline__ bool bounce_one(const MisArgs &a, GzPath *__restrict__ paths, int64_t i,

@@ -361,8 +361,11 @@ int gz_gpu_mis_trace_device(void *handle, struct GzPath *d_paths, int64_t count,
  *               wave ballot and prefix popcount (one atomic per wave); 2 = the same queues with
  *               each bounce split in a closest-hit kernel (lanes refilled from the queue, hits
  *               sorted by material class into two queues) and one shade kernel per class, the
- *               diffuse one small; 0 = every bounce launches one lane per path of all `count`
- *               (stopped paths idle in their waves). The adjoint replays queue 1 for 1 and 2;
+ *               diffuse one small; 3 = the full wavefront: as 2, with the shade kernels tracing
+ *               nothing (they queue each path's shadow ray and BSDF ray), an any-hit and a
+ *               closest-hit kernel trace those two ray queues, and a finish kernel per class adds
+ *               the two strategies' terms; 0 = every bounce launches one lane per path of all
+ *               `count` (stopped paths idle in their waves). The adjoint replays queue 1 for 1-3;
  *   block     : threads per workgroup, 64, 128 or 256;
  *   gridScale : persistent grid = gridScale x the workgroups resident per CU x CUs (1..16);
  *   chunk     : queue entries a wave takes per atomic, a multiple of 64 in 64..4096. */

@@ -207,15 +207,16 @@ def test_mixed_materials_match_oracle(gpu, sky):
 
 
 LAUNCHES = [(0, 256, 1, 256), (1, 256, 1, 256), (1, 64, 1, 64), (1, 128, 2, 1024), (1, 256, 4, 4096),
-            (2, 256, 1, 64), (2, 128, 2, 256)]
+            (2, 256, 1, 64), (2, 128, 2, 256), (3, 256, 1, 64), (3, 128, 1, 256)]
 
 
 @pytest.mark.parametrize("mixed", [False, True])
 def test_queued_launches_equal_the_full_relaunch(gpu, mixed):
     """Every launch configuration (device path queues compacted by ballot, one lane per path
-    every bounce, or the split wavefront: a trace kernel sorting hits by material class, then one
-    shade kernel per class) leaves bit-identical path records and work counts: the configuration
-    only changes which lane runs a path and when."""
+    every bounce, the split wavefront: a trace kernel sorting hits by material class, then one
+    shade kernel per class, or the full wavefront, whose shadow and BSDF rays are traced by their
+    own kernels between a shade and a finish kernel) leaves bit-identical path records and work
+    counts: the configuration only changes which lane runs a path and when."""
     import torch
     from mis_helpers import mixed_material_scene
     if mixed:
@@ -288,7 +289,7 @@ def test_bad_launch_configurations_are_rejected(gpu):
     bh, _ = scenes.cornell_hierarchy()
     bh.upload_to_gpu()
     lib = abi.load()
-    for cfg in [(3, 256, 1, 256), (-1, 256, 1, 256), (1, 96, 1, 256), (1, 512, 1, 256), (1, 256, 0, 256),
+    for cfg in [(4, 256, 1, 256), (-1, 256, 1, 256), (1, 96, 1, 256), (1, 512, 1, 256), (1, 256, 0, 256),
                 (1, 256, 17, 256),
                 (1, 256, 1, 32), (1, 256, 1, 100), (1, 256, 1, 8192)]:
         c = abi.GzMisLaunch(*cfg)
