@@ -264,7 +264,7 @@ def full_path_leg(args, dev, stream, cores: int, run_cpu: bool) -> dict:
     per_path = (2 * abi.GZ_PATH.itemsize * n + (PATH_RECORD_BYTES + SHADING_FETCH_BYTES) * st[2] + 64 * st[0]
                 + 40 * st[1]) / n
     from gonzales_amd.autotune import autotune_mis, describe
-    tune_n = min(rnd.batch, 1 << 21)
+    tune_n = min(rnd.batch, n)  # one whole batch: the tail of each bounce level depends on its size
     cam.paths_device(rnd.d_paths, n // 2 - tune_n // 2, tune_n)  # centre of the frame
     best, timings = autotune_mis(integ, rnd.d_paths[:tune_n * abi.GZ_PATH.itemsize].clone(), tune_n, reps=1)
     log(f"full path leg launch {best}")

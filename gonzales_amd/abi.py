@@ -97,7 +97,7 @@ class GzZSobolParams(ctypes.Structure):
 
 class GzMisLaunch(ctypes.Structure):
     _fields_ = [("queued", ctypes.c_int32), ("block", ctypes.c_int32), ("gridScale", ctypes.c_int32),
-                ("chunk", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
+                ("chunk", ctypes.c_int32), ("drainLevel", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
 class GzPlyInfo(ctypes.Structure):

@@ -103,11 +103,16 @@ class MisLaunch:
     block: int = 256
     grid_scale: int = 1
     chunk: int = 256
+    drain_level: int = 0
 
 
 MIS_CANDIDATES = (
     MisLaunch(3, 256, 1, 64),
+    MisLaunch(3, 256, 1, 64, 2),
+    MisLaunch(3, 256, 1, 64, 3),
     MisLaunch(3, 256, 1, 256),
+    MisLaunch(1, 256, 1, 64, 3),
+    MisLaunch(1, 256, 1, 256, 3),
     MisLaunch(2, 256, 1, 64),
     MisLaunch(2, 256, 1, 256),
     MisLaunch(0, 256, 1, 256),
@@ -130,7 +135,7 @@ def autotune_mis(integrator, d_start, count: int, reps: int = 2, candidates=MIS_
     s = torch.cuda.current_stream()
     timings = {}
     for cfg in candidates:
-        integrator.set_launch(cfg.queued, cfg.block, cfg.grid_scale, cfg.chunk)
+        integrator.set_launch(cfg.queued, cfg.block, cfg.grid_scale, cfg.chunk, cfg.drain_level)
         total = 0.0
         for rep in range(reps + 1):
             work.copy_(d_start)
@@ -144,7 +149,7 @@ def autotune_mis(integrator, d_start, count: int, reps: int = 2, candidates=MIS_
                 total += a.elapsed_time(b)
         timings[cfg] = total / reps
     best = min(timings, key=timings.get)
-    integrator.set_launch(best.queued, best.block, best.grid_scale, best.chunk)
+    integrator.set_launch(best.queued, best.block, best.grid_scale, best.chunk, best.drain_level)
     return best, timings
 
 

@@ -189,7 +189,7 @@ struct Batch {
         int maxItems = 0;
         std::vector<BatchRound> hRounds;
         std::vector<BatchItem> hItems;
-        GzMisLaunch launch{1, 256, 1, 256, {0, 0, 0, 0}};
+        GzMisLaunch launch{1, 256, 1, 256, 0, {0, 0, 0}};
         int cuCount = 256;
 
         int *dCursor = nullptr;
@@ -423,9 +423,10 @@ extern "C" int gz_batch_set_launch(void *batch, const GzMisLaunch *launch) {
         }
         const GzMisLaunch &c = *launch;
         if (c.queued != 1 || (c.block != 64 && c.block != 128 && c.block != 256) || c.gridScale < 1 ||
-            c.gridScale > 16 || c.chunk < 64 || c.chunk > 4096 || c.chunk % 64 != 0) {
-                set_error("gz_batch_set_launch: queued %d block %d gridScale %d chunk %d out of range (queued must be 1)",
-                          c.queued, c.block, c.gridScale, c.chunk);
+            c.gridScale > 16 || c.chunk < 64 || c.chunk > 4096 || c.chunk % 64 != 0 || c.drainLevel != 0) {
+                set_error("gz_batch_set_launch: queued %d block %d gridScale %d chunk %d drainLevel %d out of range "
+                          "(queued must be 1, drainLevel 0)",
+                          c.queued, c.block, c.gridScale, c.chunk, c.drainLevel);
                 return GZ_E_INVALID;
         }
         Batch *b = static_cast<Batch *>(batch);

@@ -2,6 +2,7 @@
 // bounce and path-replay adjoint. The bounce itself is in mis_device.h.
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <vector>
 
@@ -61,35 +62,100 @@ __global__ void k_queue_clear(QueueLevel *ctl, int levels) {
         }
 }
 
+// Entries of up to four runs of 64 queue entries held per lane until their appends are written
+// (k is wave-uniform, so the selects stay in registers).
+struct Keep4 {
+        int e0 = -1, e1 = -1, e2 = -1, e3 = -1;
+        __device__ __forceinline__ void set(int k, int v) {
+                if (k == 0)
+                        e0 = v;
+                else if (k == 1)
+                        e1 = v;
+                else if (k == 2)
+                        e2 = v;
+                else
+                        e3 = v;
+        }
+        __device__ __forceinline__ int get(int k) const { return k == 0 ? e0 : k == 1 ? e1 : k == 2 ? e2 : e3; }
+};
+
+// Writes the appends of `runs` held runs: bits[q] bit k = the lane's entry of run k goes to queue q.
+// One atomic per queue and flush: a counter that every wave of the chip hits serialises them.
+template <int kQ>
+__device__ __forceinline__ void flush_appends(const Keep4 &keep, const unsigned (&bits)[kQ], int runs,
+                                              int *const (&qout)[kQ], unsigned *const (&outN)[kQ], int lane,
+                                              uint64_t below) {
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+                unsigned total = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                        if (k < runs) total += (unsigned)__popcll(__ballot((bits[q] >> k) & 1u));
+                if (total == 0) continue;
+                unsigned base = 0;
+                if (lane == 0) base = atomicAdd(outN[q], total);
+                base = __shfl(base, 0);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                        if (k >= runs) break;
+                        const bool want = (bits[q] >> k) & 1u;
+                        const uint64_t m = __ballot(want);
+                        if (want) qout[q][base + (unsigned)__popcll(m & below)] = keep.get(k);
+                        base += (unsigned)__popcll(m);
+                }
+        }
+}
+
 // Runs body(i) for every entry i of a queue of `nin` entries (the identity 0 .. nin-1 when qin is
-// null), taken `chunk` at a time through the atomic `head`, and appends each i for which body
-// returns true to qout through the atomic counter `outN`. Every lane of a wave reaches each
-// ballot: the loop bounds are wave-uniform.
+// null), taken `chunk` (a multiple of 64) at a time through the atomic `head`; body returns a mask
+// of the kQ output queues i goes to, appended through their atomic counters outN after every four
+// runs of 64. Every lane of a wave reaches each ballot: the loop bounds are wave-uniform.
+template <int kQ, typename Body>
+__device__ __forceinline__ void queue_run_n(unsigned nin, const int *__restrict__ qin, unsigned *head, unsigned chunk,
+                                            int *const (&qout)[kQ], unsigned *const (&outN)[kQ], Body &&body) {
+        const int lane = (int)(threadIdx.x & 63);
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        Keep4 keep;
+        unsigned bits[kQ];
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) bits[q] = 0u;
+        int runs = 0;
+        unsigned c = 0, end = 0;
+        for (;;) {
+                if (c >= end) {
+                        unsigned t = 0;
+                        if (lane == 0) t = atomicAdd(head, chunk);
+                        t = __shfl(t, 0);
+                        if (t >= nin) break;
+                        c = t;
+                        end = nin - t < chunk ? nin : t + chunk;
+                }
+                const unsigned j = c + (unsigned)lane;
+                int i = -1;
+                if (j < end) i = qin ? qin[j] : (int)j;
+                const unsigned mask = i >= 0 ? (unsigned)body(i) : 0u;
+                keep.set(runs, i);
+#pragma unroll
+                for (int q = 0; q < kQ; ++q) bits[q] |= ((mask >> q) & 1u) << runs;
+                ++runs;
+                c += 64;
+                if (runs == 4) {
+                        flush_appends<kQ>(keep, bits, runs, qout, outN, lane, below);
+                        runs = 0;
+#pragma unroll
+                        for (int q = 0; q < kQ; ++q) bits[q] = 0u;
+                }
+        }
+        if (runs > 0) flush_appends<kQ>(keep, bits, runs, qout, outN, lane, below);
+}
+
+// One output queue: body returns whether i goes on.
 template <typename Body>
 __device__ __forceinline__ void queue_run(unsigned nin, const int *__restrict__ qin, unsigned *head,
                                           int *__restrict__ qout, unsigned *outN, unsigned chunk, Body &&body) {
-        const int lane = (int)(threadIdx.x & 63);
-        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        for (;;) {
-                unsigned c = 0;
-                if (lane == 0) c = atomicAdd(head, chunk);
-                c = __shfl(c, 0);
-                if (c >= nin) break;
-                const unsigned end = nin - c < chunk ? nin : c + chunk;
-                for (unsigned j0 = c; j0 < end; j0 += 64) {
-                        const unsigned j = j0 + (unsigned)lane;
-                        int i = -1;
-                        if (j < end) i = qin ? qin[j] : (int)j;
-                        const bool go = i >= 0 && body(i);
-                        const uint64_t m = __ballot(go);
-                        if (m != 0ull) {
-                                unsigned base = 0;
-                                if (lane == 0) base = atomicAdd(outN, (unsigned)__popcll(m));
-                                base = __shfl(base, 0);
-                                if (go) qout[base + (unsigned)__popcll(m & below)] = i;
-                        }
-                }
-        }
+        int *const qo[1] = {qout};
+        unsigned *const on[1] = {outN};
+        queue_run_n<1>(nin, qin, head, chunk, qo, on, [&](int i) { return body(i) ? 1u : 0u; });
 }
 
 // The level form: entries of level `level` (identity at level 0), survivors to level + 1.
@@ -115,8 +181,7 @@ struct WfLevel {
         unsigned an, ahead;    // other BSDFs
         unsigned sn, shead;    // shadow rays (full wavefront)
         unsigned mn, mhead;    // BSDF rays (full wavefront)
-        unsigned fdhead, fahead;  // finish passes over the two class queues
-        unsigned pad[4];
+        unsigned pad[6];
 };
 static_assert(sizeof(WfLevel) == 64, "WfLevel");
 
@@ -320,6 +385,32 @@ __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_mis_queue(MisArgs a
         });
 }
 
+// The remaining bounces of every path of one level's queue in one launch: a lane runs its path's
+// bounces until it stops (bounce_one returns false once the path is inactive or past maxDepth, so
+// at most maxDepth + 1 of them). For the last levels, where the few paths that Russian roulette
+// leaves cost less than one launch sequence per level.
+template <bool kStats, bool kAll>
+__global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_mis_drain(MisArgs a, GzPath *__restrict__ paths,
+                                                                       const int *__restrict__ qin, const unsigned *nptr,
+                                                                       unsigned *head, unsigned chunk,
+                                                                       uint32_t *__restrict__ stats) {
+        const unsigned nin = *nptr;
+        const int lane = (int)(threadIdx.x & 63);
+        for (;;) {
+                unsigned c = 0;
+                if (lane == 0) c = atomicAdd(head, chunk);
+                c = __shfl(c, 0);
+                if (c >= nin) break;
+                const unsigned end = nin - c < chunk ? nin : c + chunk;
+                for (unsigned j = c + (unsigned)lane; j < end; j += 64) {
+                        const int i = qin[j];
+                        for (int b = 0; b <= a.maxDepth; ++b)
+                                if (!bounce_one<kStats, kAll, false>(a, paths, i, stats, AdjArgs{nullptr, nullptr}, nullptr))
+                                        break;
+                }
+        }
+}
+
 // The replayed bounce over the same queues; a persistent block flushes its LDS gradient
 // accumulators once, when the level is drained.
 template <bool kAll>
@@ -441,17 +532,6 @@ __global__ __launch_bounds__(kBlock) void k_wf_rays(MisArgs a, const Ray_C *__re
         }
 }
 
-// Appends i to q at the counter n for the lanes with `want`: one ballot, one atomic per wave.
-__device__ __forceinline__ void wave_append(bool want, int i, int *__restrict__ q, unsigned *n, uint64_t below,
-                                            int lane) {
-        const uint64_t m = __ballot(want);
-        if (m == 0ull) return;
-        unsigned b = 0;
-        if (lane == 0) b = atomicAdd(n, (unsigned)__popcll(m));
-        b = __shfl(b, 0);
-        if (want) q[b + (unsigned)__popcll(m & below)] = i;
-}
-
 template <bool kStats, bool kAll>
 __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_wf_phase(
         MisArgs a, GzPath *__restrict__ paths, const GzHit *__restrict__ hits, const int *__restrict__ qin,
@@ -459,71 +539,45 @@ __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_wf_phase(
         Pending *__restrict__ pend, Ray_C *__restrict__ srays, Ray_C *__restrict__ mrays, unsigned chunk,
         uint32_t *__restrict__ stats) {
         WfLevel &lv = ctl[level];
-        const unsigned nin = kAll ? lv.an : lv.dn;
-        unsigned *head = kAll ? &lv.ahead : &lv.dhead;
-        const int lane = (int)(threadIdx.x & 63);
-        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        for (;;) {
-                unsigned c = 0;
-                if (lane == 0) c = atomicAdd(head, chunk);
-                c = __shfl(c, 0);
-                if (c >= nin) break;
-                const unsigned end = nin - c < chunk ? nin : c + chunk;
-                for (unsigned j0 = c; j0 < end; j0 += 64) {
-                        const unsigned j = j0 + (unsigned)lane;
-                        int i = -1;
-                        if (j < end) i = qin[j];
-                        bool go = false, shadow = false, mis = false;
-                        if (i >= 0) {
-                                const GzPath p = paths[i];
-                                Pending pd;
-                                pd.flags = 0;
-                                if (!(p.flags & GZ_PATH_ACTIVE)) {
-                                } else if (p.bounce > a.maxDepth) {
-                                        paths[i].flags = p.flags & ~GZ_PATH_ACTIVE;
-                                } else {
-                                        const GzHit h = hits[i];
-                                        Ray_C sr, mr;
-                                        go = shade_phase<kStats, kAll>(a, paths, i, p, h.prim >= 0, h.prim, h.u, h.v, pd,
-                                                                       sr, mr, stats);
-                                        shadow = (pd.flags & kPendShadow) != 0;
-                                        mis = (pd.flags & kPendMis) != 0;
-                                        if (shadow) srays[i] = sr;
-                                        if (mis) mrays[i] = mr;
-                                }
-                                pend[i] = pd;
-                        }
-                        wave_append(go, i, qnext, &ctl[level + 1].qn, below, lane);
-                        wave_append(shadow, i, qs, &lv.sn, below, lane);
-                        wave_append(mis, i, qm, &lv.mn, below, lane);
+        int *const qo[3] = {qnext, qs, qm};
+        unsigned *const on[3] = {&ctl[level + 1].qn, &lv.sn, &lv.mn};
+        queue_run_n<3>(kAll ? lv.an : lv.dn, qin, kAll ? &lv.ahead : &lv.dhead, chunk, qo, on, [&](int i) {
+                const GzPath p = paths[i];
+                Pending pd;
+                pd.flags = 0;
+                unsigned mask = 0;
+                if (!(p.flags & GZ_PATH_ACTIVE)) {
+                } else if (p.bounce > a.maxDepth) {
+                        paths[i].flags = p.flags & ~GZ_PATH_ACTIVE;
+                } else {
+                        const GzHit h = hits[i];
+                        Ray_C sr, mr;
+                        const bool go = shade_phase<kStats, kAll>(a, paths, i, p, h.prim >= 0, h.prim, h.u, h.v, pd, sr,
+                                                                  mr, stats);
+                        if (pd.flags & kPendShadow) srays[i] = sr;
+                        if (pd.flags & kPendMis) mrays[i] = mr;
+                        mask = (go ? 1u : 0u) | ((pd.flags & kPendShadow) ? 2u : 0u) | ((pd.flags & kPendMis) ? 4u : 0u);
                 }
-        }
+                pend[i] = pd;
+                return mask;
+        });
 }
 
 __global__ __launch_bounds__(kMisBlock) void k_wf_finish(MisArgs a, GzPath *__restrict__ paths,
                                                          const int *__restrict__ qin, const unsigned *nptr,
-                                                         unsigned *head, const Pending *__restrict__ pend,
+                                                         const Pending *__restrict__ pend,
                                                          const uint8_t *__restrict__ occ,
                                                          const Ray_C *__restrict__ mrays,
                                                          const GzHit *__restrict__ mhits) {
-        const unsigned nin = *nptr;
-        const unsigned chunk = 256;
-        const int lane = (int)(threadIdx.x & 63);
-        for (;;) {
-                unsigned c = 0;
-                if (lane == 0) c = atomicAdd(head, chunk);
-                c = __shfl(c, 0);
-                if (c >= nin) break;
-                const unsigned end = nin - c < chunk ? nin : c + chunk;
-                for (unsigned j = c + (unsigned)lane; j < end; j += 64) {
-                        const int i = qin[j];
-                        const Pending pd = pend[i];
-                        if (!(pd.flags & kPendWork)) continue;
-                        const bool o = (pd.flags & kPendShadow) ? occ[i] != 0 : false;
-                        const Ray_C mr = (pd.flags & kPendMis) ? mrays[i] : Ray_C{0, 0, 0, 0, 0, 0};
-                        const GzHit mh = (pd.flags & kPendMis) ? mhits[i] : GzHit{-1, 0.0f, 0.0f, 0.0f};
-                        finish_phase(a, paths, i, pd, o, mr, mh);
-                }
+        const size_t nin = *nptr;
+        for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < nin; j += (size_t)gridDim.x * blockDim.x) {
+                const int i = qin[j];
+                const Pending pd = pend[i];
+                if (!(pd.flags & kPendWork)) continue;
+                const bool o = (pd.flags & kPendShadow) ? occ[i] != 0 : false;
+                const Ray_C mr = (pd.flags & kPendMis) ? mrays[i] : Ray_C{0, 0, 0, 0, 0, 0};
+                const GzHit mh = (pd.flags & kPendMis) ? mhits[i] : GzHit{-1, 0.0f, 0.0f, 0.0f};
+                finish_phase(a, paths, i, pd, o, mr, mh);
         }
 }
 
@@ -863,7 +917,8 @@ bool gz::mis_args(Scene *s, const GzMisParams *params, MisArgs &a) {
 // enqueues one level's kernel.
 template <typename K, typename Launch>
 static int queued_levels(Scene *s, K kern, int64_t count, int maxDepth, hipStream_t st, const char *who,
-                         Launch &&launch) {
+                         Launch &&launch, const std::function<void(unsigned, const int *, unsigned *, unsigned *,
+                                                                   unsigned)> &drain = {}) {
         const int levels = maxDepth + 2;
         MisQueue *q = mis_queue(s, st, (size_t)count, levels);
         if (!q) return GZ_E_ALLOC;
@@ -875,6 +930,10 @@ static int queued_levels(Scene *s, K kern, int64_t count, int maxDepth, hipStrea
         for (int level = 0; level <= maxDepth; ++level) {
                 const int *qin = level == 0 ? nullptr : q->q[level & 1];
                 int *qout = q->q[(level + 1) & 1];
+                if (drain && level >= c.drainLevel) {
+                        drain(grid, qin, &q->ctl[level].n, &q->ctl[level].head, chunk);
+                        return hip_ok(hipGetLastError(), who) ? GZ_OK : GZ_E_HIP;
+                }
                 launch(level, grid, qin, qout, q->ctl, chunk);
                 if (!hip_ok(hipGetLastError(), who)) return GZ_E_HIP;
         }
@@ -934,11 +993,19 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                 const unsigned sgrid = (unsigned)std::max<int64_t>(
                         1, std::min<int64_t>((int64_t)s->cuCount * 8, (count + kMisBlock - 1) / kMisBlock));
                 const unsigned spgrid = split_grid(s, count);
+                auto kdrain = d_stats ? (all ? k_mis_drain<true, true> : k_mis_drain<true, false>)
+                                      : (all ? k_mis_drain<false, true> : k_mis_drain<false, false>);
+                const unsigned gdrain = queue_grid(s, kdrain, c.block, count, chunk);
                 hipLaunchKernelGGL(k_wf_clear, dim3(1), dim3(256), 0, st, q->wctl, levels);
                 for (int level = 0; level <= params->maxDepth; ++level) {
                         const int *qin = level == 0 ? nullptr : q->q[level & 1];
                         int *qout = q->q[(level + 1) & 1];
                         WfLevel *lv = q->wctl + level;
+                        if (c.drainLevel > 0 && level >= c.drainLevel) {
+                                hipLaunchKernelGGL(kdrain, dim3(gdrain), dim3(c.block), 0, st, a, d_paths, qin, &lv->qn,
+                                                   &lv->qhead, chunk, stats);
+                                return hip_ok(hipGetLastError(), who) ? GZ_OK : GZ_E_HIP;
+                        }
                         hipLaunchKernelGGL(ktrace, dim3(tgrid), dim3(kBlock), 0, st, a, d_paths, (unsigned)count, qin,
                                            q->wctl, level, q->hits, refill, tchunk, stats);
                         hipLaunchKernelGGL(k_wf_split, dim3(spgrid), dim3(kMisBlock), 0, st, a, (unsigned)count, qin,
@@ -953,10 +1020,10 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                         hipLaunchKernelGGL(kclose, dim3(cgrid), dim3(kBlock), 0, st, a, q->mrays, INFINITY, q->qm,
                                            &lv->mn, &lv->mhead, nullptr, q->mhits, refill, chunk, stats);
                         hipLaunchKernelGGL(k_wf_finish, dim3(sgrid), dim3(kMisBlock), 0, st, a, d_paths, q->qd, &lv->dn,
-                                           &lv->fdhead, q->pend, q->occ, q->mrays, q->mhits);
+                                           q->pend, q->occ, q->mrays, q->mhits);
                         if (all)
                                 hipLaunchKernelGGL(k_wf_finish, dim3(sgrid), dim3(kMisBlock), 0, st, a, d_paths, q->qa,
-                                                   &lv->an, &lv->fahead, q->pend, q->occ, q->mrays, q->mhits);
+                                                   &lv->an, q->pend, q->occ, q->mrays, q->mhits);
                         if (!hip_ok(hipGetLastError(), who)) return GZ_E_HIP;
                 }
                 return GZ_OK;
@@ -984,10 +1051,18 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                 const unsigned chunk = (unsigned)c.chunk;
                 const unsigned gd = queue_grid(s, kd, c.block, count, chunk);
                 const unsigned ga = queue_grid(s, ka, c.block, count, chunk);
+                auto kdrain = d_stats ? (all ? k_mis_drain<true, true> : k_mis_drain<true, false>)
+                                      : (all ? k_mis_drain<false, true> : k_mis_drain<false, false>);
+                const unsigned gdrain = queue_grid(s, kdrain, c.block, count, chunk);
                 hipLaunchKernelGGL(k_wf_clear, dim3(1), dim3(256), 0, st, q->wctl, levels);
                 for (int level = 0; level <= params->maxDepth; ++level) {
                         const int *qin = level == 0 ? nullptr : q->q[level & 1];
                         int *qout = q->q[(level + 1) & 1];
+                        if (c.drainLevel > 0 && level >= c.drainLevel) {
+                                hipLaunchKernelGGL(kdrain, dim3(gdrain), dim3(c.block), 0, st, a, d_paths, qin,
+                                                   &q->wctl[level].qn, &q->wctl[level].qhead, chunk, stats);
+                                return hip_ok(hipGetLastError(), who) ? GZ_OK : GZ_E_HIP;
+                        }
                         hipLaunchKernelGGL(ktrace, dim3(tgrid), dim3(kBlock), 0, st, a, d_paths, (unsigned)count, qin,
                                            q->wctl, level, q->hits, refill, tchunk, stats);
                         hipLaunchKernelGGL(k_wf_split, dim3(split_grid(s, count)), dim3(kMisBlock), 0, st, a,
@@ -1008,13 +1083,22 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                 }
                 auto kern = d_stats ? (all ? k_mis_queue<true, true> : k_mis_queue<true, false>)
                                     : (all ? k_mis_queue<false, true> : k_mis_queue<false, false>);
+                auto kdrain = d_stats ? (all ? k_mis_drain<true, true> : k_mis_drain<true, false>)
+                                      : (all ? k_mis_drain<false, true> : k_mis_drain<false, false>);
                 const int block = s->misLaunch.block;
-                return queued_levels(s, kern, count, params->maxDepth, st, who,
-                                     [&](int level, unsigned grid, const int *qin, int *qout, QueueLevel *ctl,
-                                         unsigned chunk) {
-                                             hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, a, d_paths,
-                                                                (unsigned)count, qin, qout, ctl, level, chunk, stats);
-                                     });
+                std::function<void(unsigned, const int *, unsigned *, unsigned *, unsigned)> drain;
+                if (s->misLaunch.drainLevel > 0)
+                        drain = [&](unsigned grid, const int *qin, unsigned *n, unsigned *head, unsigned chunk) {
+                                hipLaunchKernelGGL(kdrain, dim3(grid), dim3(block), 0, st, a, d_paths, qin, n, head, chunk,
+                                                   stats);
+                        };
+                return queued_levels(
+                        s, kern, count, params->maxDepth, st, who,
+                        [&](int level, unsigned grid, const int *qin, int *qout, QueueLevel *ctl, unsigned chunk) {
+                                hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, a, d_paths, (unsigned)count, qin,
+                                                   qout, ctl, level, chunk, stats);
+                        },
+                        drain);
         }
         // A path makes at most maxDepth + 1 bounces (bounce 0 .. maxDepth); a launch after the
         // last finds every path inactive.
@@ -1036,9 +1120,12 @@ extern "C" int gz_gpu_set_mis_launch(void *handle, const GzMisLaunch *launch) {
         }
         const GzMisLaunch &c = *launch;
         if (c.queued < 0 || c.queued > 3 || (c.block != 64 && c.block != 128 && c.block != 256) ||
-            c.gridScale < 1 || c.gridScale > 16 || c.chunk < 64 || c.chunk > 4096 || c.chunk % 64 != 0) {
-                set_error("gz_gpu_set_mis_launch: queued %d block %d gridScale %d chunk %d out of range", c.queued,
-                          c.block, c.gridScale, c.chunk);
+            c.gridScale < 1 || c.gridScale > 16 || c.chunk < 64 || c.chunk > 4096 || c.chunk % 64 != 0 ||
+            c.drainLevel < 0 || c.drainLevel > 64 || (c.queued == 0 && c.drainLevel != 0) || c.reserved[0] != 0 ||
+            c.reserved[1] != 0 || c.reserved[2] != 0) {
+                set_error("gz_gpu_set_mis_launch: queued %d block %d gridScale %d chunk %d drainLevel %d out of range "
+                          "(drainLevel needs a queued mode; reserved must be zero)",
+                          c.queued, c.block, c.gridScale, c.chunk, c.drainLevel);
                 return GZ_E_INVALID;
         }
         Scene *s = static_cast<Scene *>(handle);

@@ -440,6 +440,22 @@ __device__ __forceinline__ void adjoint_vertex(const MisArgs &a, const Bsdf &b, 
 // One bounce of path i. kStats: also adds the bounce's traversal work (all its rays) to
 // stats[i] and counts the bounce (interior, tris, bounces). kAdj: replays the bounce of the
 // forward pass and adds the gradient of the path's radiance (lds: per-block accumulator).
+// The light strategy's term of estimateDirect for an unoccluded light sample: f |cos| L weighted
+// by the power heuristic against the BSDF's pdf of the light direction, over the light pdf
+// (VolumePathIntegrator.swift sampleLightSource / MultipleImportanceSampler).
+template <bool kAll>
+__device__ __forceinline__ f3 light_term(const MisArgs &a, const GzPath &p, const Bsdf &b, const Interaction &si,
+                                         const LightSample &ls) {
+        LayerRng r1 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightF);
+        const f3 scatter = mul(bsdf_eval_world<kAll>(b, si.outgoing, ls.dir, r1), absdot(ls.dir, si.shadingNormal));
+        const f3 Lest = cmul(scatter, ls.L);
+        const float lpdf = ls.pdf;
+        LayerRng r2 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightPdf);
+        const float bpdfL = bsdf_pdf_world<kAll>(b, si.outgoing, ls.dir, r2);
+        const float wl = power_heuristic(lpdf, bpdfL);
+        return lpdf == 0.0f ? mk(0.0f, 0.0f, 0.0f) : divs(mul(Lest, wl), lpdf);
+}
+
 // Whether a path record takes part in a bounce (VolumePathIntegrator.swift:400-402 stops it past
 // maxDepth).
 __device__ __forceinline__ bool path_live(const MisArgs &a, const GzPath &p) {
@@ -493,19 +509,32 @@ __device__ __forceinline__ bool bounce_rest(const MisArgs &a, GzPath *__restrict
                 LightSample ls;
                 light_sample(a, choice, si.position, u0, u1, ls);
                 f3 Lest = mk(0.0f, 0.0f, 0.0f), linc = mk(0.0f, 0.0f, 0.0f);
-                float lpdf = 0.0f;
-                if (!cblack(ls.L) && !isinf(ls.pdf) && !occluded<kStats>(a, si.position, ls.target, w)) {
-                        LayerRng r1 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightF);
-                        const f3 scatter =
-                                mul(bsdf_eval_world<kAll>(b, si.outgoing, ls.dir, r1), absdot(ls.dir, si.shadingNormal));
-                        Lest = cmul(scatter, ls.L);
-                        lpdf = ls.pdf;
-                        linc = ls.dir;
+                float lpdf = 0.0f, wl;
+                f3 lc = mk(0.0f, 0.0f, 0.0f);
+                if (kAdj) {
+                        // the reference's order: the shadow ray first (the gradient of a zero term
+                        // can be non-zero, so the adjoint needs its visibility)
+                        if (!cblack(ls.L) && !isinf(ls.pdf) && !occluded<kStats>(a, si.position, ls.target, w)) {
+                                LayerRng r1 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightF);
+                                const f3 scatter = mul(bsdf_eval_world<kAll>(b, si.outgoing, ls.dir, r1),
+                                                       absdot(ls.dir, si.shadingNormal));
+                                Lest = cmul(scatter, ls.L);
+                                lpdf = ls.pdf;
+                                linc = ls.dir;
+                        }
+                        LayerRng r2 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightPdf);
+                        const float bpdfL = bsdf_pdf_world<kAll>(b, si.outgoing, linc, r2);
+                        wl = power_heuristic(lpdf, bpdfL);
+                        lc = lpdf == 0.0f ? mk(0.0f, 0.0f, 0.0f) : divs(mul(Lest, wl), lpdf);
+                } else {
+                        // the term as if unoccluded, then the shadow ray only where that term is not
+                        // exactly zero: an occluded light gives the zero term as well
+                        wl = 0.0f;
+                        if (!cblack(ls.L) && !isinf(ls.pdf)) {
+                                const f3 lcU = light_term<kAll>(a, p, b, si, ls);
+                                if (!cblack(lcU) && !occluded<kStats>(a, si.position, ls.target, w)) lc = lcU;
+                        }
                 }
-                LayerRng r2 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightPdf);
-                const float bpdfL = bsdf_pdf_world<kAll>(b, si.outgoing, linc, r2);
-                const float wl = power_heuristic(lpdf, bpdfL);
-                const f3 lc = lpdf == 0.0f ? mk(0.0f, 0.0f, 0.0f) : divs(mul(Lest, wl), lpdf);
                 float v0, v1, v2;
                 zs_get3d(a.zs, zs, a.sobol, v0, v1, v2);
                 f3 bf, bwi;
@@ -663,21 +692,15 @@ __device__ __forceinline__ bool shade_phase(const MisArgs &a, GzPath *__restrict
                 pd.flags = kPendWork;
                 f3 lcU = mk(0.0f, 0.0f, 0.0f);
                 if (!cblack(ls.L) && !isinf(ls.pdf)) {
-                        // the unoccluded case of bounce_rest; the shadow ray decides it later
-                        LayerRng r1 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightF);
-                        const f3 scatter =
-                                mul(bsdf_eval_world<kAll>(b, si.outgoing, ls.dir, r1), absdot(ls.dir, si.shadingNormal));
-                        const f3 Lest = cmul(scatter, ls.L);
-                        const float lpdf = ls.pdf;
-                        LayerRng r2 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightPdf);
-                        const float bpdfL = bsdf_pdf_world<kAll>(b, si.outgoing, ls.dir, r2);
-                        const float wl = power_heuristic(lpdf, bpdfL);
-                        lcU = lpdf == 0.0f ? mk(0.0f, 0.0f, 0.0f) : divs(mul(Lest, wl), lpdf);
-                        // occluded(): origin from + 1e-4 (target - from), direction target - origin
-                        const f3 o = add(si.position, mul(sub(ls.target, si.position), 0.0001f));
-                        const f3 d = sub(ls.target, o);
-                        sray = Ray_C{o.x, o.y, o.z, d.x, d.y, d.z};
-                        pd.flags |= kPendShadow;
+                        // bounce_rest's unoccluded term; the shadow ray, where it is not zero, decides
+                        lcU = light_term<kAll>(a, p, b, si, ls);
+                        if (!cblack(lcU)) {
+                                // occluded(): origin from + 1e-4 (target - from), direction target - origin
+                                const f3 o = add(si.position, mul(sub(ls.target, si.position), 0.0001f));
+                                const f3 d = sub(ls.target, o);
+                                sray = Ray_C{o.x, o.y, o.z, d.x, d.y, d.z};
+                                pd.flags |= kPendShadow;
+                        }
                 }
                 float v0, v1, v2;
                 zs_get3d(a.zs, zs, a.sobol, v0, v1, v2);

@@ -175,16 +175,18 @@ class VolumePathIntegrator:
                                                        ctypes.byref(self._params), abi.stream_ptr(stream)),
                   "gz_gpu_mis_bounce_device")
 
-    def set_launch(self, queued: int = 1, block: int = 256, grid_scale: int = 1, chunk: int = 256) -> None:
+    def set_launch(self, queued: int = 1, block: int = 256, grid_scale: int = 1, chunk: int = 256,
+                   drain_level: int = 0) -> None:
         """Launch configuration of trace/trace_stats and the adjoint on this scene handle (the
         results never depend on it)."""
-        c = abi.GzMisLaunch(queued, block, grid_scale, chunk)
+        c = abi.GzMisLaunch(queued, block, grid_scale, chunk, drain_level)
         abi.check(abi.load().gz_gpu_set_mis_launch(self.bh.handle, ctypes.byref(c)), "gz_gpu_set_mis_launch")
 
-    def launch(self) -> tuple[int, int, int, int]:
+    def launch(self) -> tuple[int, int, int, int, int]:
+        """(queued, block, grid_scale, chunk, drain_level) of this scene handle."""
         c = abi.GzMisLaunch()
         abi.check(abi.load().gz_gpu_get_mis_launch(self.bh.handle, ctypes.byref(c)), "gz_gpu_get_mis_launch")
-        return c.queued, c.block, c.gridScale, c.chunk
+        return c.queued, c.block, c.gridScale, c.chunk, c.drainLevel
 
     def trace(self, paths_t, count: int | None = None, stream=None) -> None:
         """All bounces of every active path (maxDepth + 1 levels), in place."""

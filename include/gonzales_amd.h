@@ -368,13 +368,18 @@ int gz_gpu_mis_trace_device(void *handle, struct GzPath *d_paths, int64_t count,
  *               `count` (stopped paths idle in their waves). The adjoint replays queue 1 for 1-3;
  *   block     : threads per workgroup, 64, 128 or 256;
  *   gridScale : persistent grid = gridScale x the workgroups resident per CU x CUs (1..16);
- *   chunk     : queue entries a wave takes per atomic, a multiple of 64 in 64..4096. */
+ *   chunk     : queue entries a wave takes per atomic, a multiple of 64 in 64..4096;
+ *   drainLevel: 0, or 1..64 with a queued mode: the paths left at that bounce level run all their
+ *               remaining bounces in one launch (a lane per path), in place of one launch
+ *               sequence per level for the few paths that Russian roulette leaves;
+ *   reserved  : zero. */
 struct GzMisLaunch {
         int32_t queued;
         int32_t block;
         int32_t gridScale;
         int32_t chunk;
-        int32_t reserved[4];
+        int32_t drainLevel;
+        int32_t reserved[3];
 };
 
 int gz_gpu_set_mis_launch(void *handle, const struct GzMisLaunch *launch);
@@ -497,7 +502,8 @@ struct GzBatchInfo {
  * failure, with gz_last_error() set. */
 void *gz_batch_create(const struct GzBatchScene *scenes, int32_t sceneCount, int64_t roundPaths);
 int gz_batch_info(void *batch, struct GzBatchInfo *info);
-/* Launch shape of the batch's persistent bounce kernel (queued must be 1; see GzMisLaunch). */
+/* Launch shape of the batch's persistent bounce kernel (queued must be 1 and drainLevel 0; see
+ * GzMisLaunch). */
 int gz_batch_set_launch(void *batch, const struct GzMisLaunch *launch);
 /* Renders every path of every scene once and adds it to the scenes' films, asynchronously on
  * `stream`. Each round is the same launch sequence (camera rays, device path queues per bounce

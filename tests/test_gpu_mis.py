@@ -206,8 +206,9 @@ def test_mixed_materials_match_oracle(gpu, sky):
     _check(got, ref, min_frac=0.98)
 
 
-LAUNCHES = [(0, 256, 1, 256), (1, 256, 1, 256), (1, 64, 1, 64), (1, 128, 2, 1024), (1, 256, 4, 4096),
-            (2, 256, 1, 64), (2, 128, 2, 256), (3, 256, 1, 64), (3, 128, 1, 256)]
+LAUNCHES = [(0, 256, 1, 256, 0), (1, 256, 1, 256, 0), (1, 64, 1, 64, 0), (1, 128, 2, 1024, 0), (1, 256, 4, 4096, 0),
+            (2, 256, 1, 64, 0), (2, 128, 2, 256, 0), (3, 256, 1, 64, 0), (3, 128, 1, 256, 0), (1, 256, 1, 64, 1),
+            (1, 256, 1, 256, 3), (2, 256, 1, 64, 2), (3, 256, 1, 64, 3), (3, 64, 1, 128, 6)]
 
 
 @pytest.mark.parametrize("mixed", [False, True])
@@ -291,12 +292,16 @@ def test_bad_launch_configurations_are_rejected(gpu):
     lib = abi.load()
     for cfg in [(4, 256, 1, 256), (-1, 256, 1, 256), (1, 96, 1, 256), (1, 512, 1, 256), (1, 256, 0, 256),
                 (1, 256, 17, 256),
-                (1, 256, 1, 32), (1, 256, 1, 100), (1, 256, 1, 8192)]:
+                (1, 256, 1, 32), (1, 256, 1, 100), (1, 256, 1, 8192), (1, 256, 1, 256, -1), (1, 256, 1, 256, 65),
+                (0, 256, 1, 256, 2)]:
         c = abi.GzMisLaunch(*cfg)
         assert lib.gz_gpu_set_mis_launch(bh.handle, ctypes.byref(c)) == abi.GZ_E_INVALID, cfg
+    c = abi.GzMisLaunch(1, 256, 1, 256)
+    c.reserved[1] = 5
+    assert lib.gz_gpu_set_mis_launch(bh.handle, ctypes.byref(c)) == abi.GZ_E_INVALID
     out = abi.GzMisLaunch()
     assert lib.gz_gpu_get_mis_launch(bh.handle, ctypes.byref(out)) == abi.GZ_OK
-    assert (out.queued, out.block, out.gridScale, out.chunk) == (1, 256, 1, 256)
+    assert (out.queued, out.block, out.gridScale, out.chunk, out.drainLevel) == (1, 256, 1, 256, 0)
 
 
 def test_ingested_regression_scene_matches_oracle(gpu):
