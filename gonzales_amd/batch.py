@@ -52,8 +52,10 @@ class SceneBatch:
         abi.check(abi.load().gz_batch_info(self.handle, ctypes.byref(out)), "gz_batch_info")
         return out
 
-    def set_launch(self, block: int = 256, grid_scale: int = 1, chunk: int = 256) -> None:
-        c = abi.GzMisLaunch(1, block, grid_scale, chunk)
+    def set_launch(self, block: int = 256, grid_scale: int = 1, chunk: int = 256, drain_level: int = 3) -> None:
+        """Launch shape of the batch's bounce kernel; drain_level > 0 runs the bounce levels from
+        that one on in a single launch (a lane per path), 0 keeps one launch per level."""
+        c = abi.GzMisLaunch(1, block, grid_scale, chunk, drain_level)
         abi.check(abi.load().gz_batch_set_launch(self.handle, ctypes.byref(c)), "gz_batch_set_launch")
 
     def clear_films(self) -> None:
@@ -82,6 +84,7 @@ class BatchLaunch:
     block: int = 256
     grid_scale: int = 1
     chunk: int = 256
+    drain_level: int = 3
 
 
 BATCH_CANDIDATES = (
@@ -90,6 +93,8 @@ BATCH_CANDIDATES = (
     BatchLaunch(256, 1, 1024),
     BatchLaunch(256, 2, 256),
     BatchLaunch(128, 1, 256),
+    BatchLaunch(256, 1, 256, 0),
+    BatchLaunch(256, 1, 256, 2),
 )
 
 
@@ -101,7 +106,7 @@ def autotune_batch(batch: SceneBatch, candidates=BATCH_CANDIDATES, reps: int = 1
     s = torch.cuda.current_stream()
     timings = {}
     for cfg in candidates:
-        batch.set_launch(cfg.block, cfg.grid_scale, cfg.chunk)
+        batch.set_launch(cfg.block, cfg.grid_scale, cfg.chunk, cfg.drain_level)
         batch.render(True)  # captures the graph for this shape
         torch.cuda.synchronize()
         a = torch.cuda.Event(enable_timing=True)
@@ -113,7 +118,7 @@ def autotune_batch(batch: SceneBatch, candidates=BATCH_CANDIDATES, reps: int = 1
         torch.cuda.synchronize()
         timings[cfg] = a.elapsed_time(b) / reps
     best = min(timings, key=timings.get)
-    batch.set_launch(best.block, best.grid_scale, best.chunk)
+    batch.set_launch(best.block, best.grid_scale, best.chunk, best.drain_level)
     batch.clear_films()
     return best, timings
 

@@ -117,7 +117,11 @@ __global__ void k_batch_plan(const int *__restrict__ cursor, const BatchRound *_
         base[R.nItems] = sum;
 }
 
-template <bool kAll>
+// One bounce level of a round: waves take chunks of the level's work items through one global
+// counter; survivors go to the item's next-level queue (appends held over four runs of 64 and
+// written with one atomic). kDrain: a lane runs each path's remaining bounces instead, and no
+// queue is written (the last levels of a round, in one launch).
+template <bool kAll, bool kDrain>
 __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_batch_bounce(const int *__restrict__ cursor,
                                                             const BatchRound *__restrict__ rounds,
                                                             const BatchItem *__restrict__ items,
@@ -144,20 +148,36 @@ __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_batch_bounce(const 
                 const unsigned n = level == 0 ? (unsigned)it.count : nIn[k];
                 const unsigned e0 = (g - base[k]) * chunk;
                 const unsigned e1 = n - e0 < chunk ? n : e0 + chunk;
+                if (kDrain) {
+                        for (unsigned j = e0 + (unsigned)lane; j < e1; j += 64) {
+                                const int idx = level == 0 ? it.offset + (int)j : qin[it.offset + (int)j];
+                                for (int b = 0; b <= a.maxDepth; ++b)
+                                        if (!bounce_one<false, kAll, false>(a, paths, idx, nullptr, AdjArgs{nullptr, nullptr},
+                                                                           nullptr))
+                                                break;
+                        }
+                        continue;
+                }
+                int *const qo[1] = {qout + it.offset};
+                unsigned *const on[1] = {&nOut[k]};
+                Keep4 keep;
+                unsigned bits[1] = {0u};
+                int runs = 0;
                 for (unsigned j0 = e0; j0 < e1; j0 += 64) {
                         const unsigned j = j0 + (unsigned)lane;
                         int idx = -1;
                         if (j < e1) idx = level == 0 ? it.offset + (int)j : qin[it.offset + (int)j];
                         const bool go = idx >= 0 && bounce_one<false, kAll, false>(a, paths, idx, nullptr,
                                                                                    AdjArgs{nullptr, nullptr}, nullptr);
-                        const uint64_t m = __ballot(go);
-                        if (m != 0ull) {
-                                unsigned b = 0;
-                                if (lane == 0) b = atomicAdd(&nOut[k], (unsigned)__popcll(m));
-                                b = __shfl(b, 0);
-                                if (go) qout[it.offset + (int)(b + (unsigned)__popcll(m & below))] = idx;
+                        keep.set(runs, idx);
+                        bits[0] |= (go ? 1u : 0u) << runs;
+                        if (++runs == 4) {
+                                flush_appends<1>(keep, bits, runs, qo, on, lane, below);
+                                runs = 0;
+                                bits[0] = 0u;
                         }
                 }
+                if (runs > 0) flush_appends<1>(keep, bits, runs, qo, on, lane, below);
         }
 }
 
@@ -189,7 +209,7 @@ struct Batch {
         int maxItems = 0;
         std::vector<BatchRound> hRounds;
         std::vector<BatchItem> hItems;
-        GzMisLaunch launch{1, 256, 1, 256, 0, {0, 0, 0}};
+        GzMisLaunch launch{1, 256, 1, 256, 3, {0, 0, 0}};
         int cuCount = 256;
 
         int *dCursor = nullptr;
@@ -238,7 +258,9 @@ static bool upload(T **dst, const T *src, size_t count, const char *what) {
 static bool enqueue_round(Batch *b, hipStream_t st) {
         const unsigned chunk = (unsigned)b->launch.chunk;
         const int block = b->launch.block;
-        auto kern = b->all ? k_batch_bounce<true> : k_batch_bounce<false>;
+        auto kern = b->all ? k_batch_bounce<true, false> : k_batch_bounce<false, false>;
+        auto kdrain = b->all ? k_batch_bounce<true, true> : k_batch_bounce<false, true>;
+        const int drainLevel = b->launch.drainLevel;
         int perCU = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, block, 0) != hipSuccess || perCU < 1) perCU = 1;
         const int64_t want = (int64_t)b->cuCount * perCU * b->launch.gridScale;
@@ -252,8 +274,11 @@ static bool enqueue_round(Batch *b, hipStream_t st) {
         for (int level = 0; level < b->levels; ++level) {
                 hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(64), 0, st, b->dCursor, b->dRounds, b->dItems, b->ctr,
                                    level, chunk);
-                hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, b->dCursor, b->dRounds, b->dItems, b->dArgs,
-                                   b->dPaths, b->dQ[level & 1], b->dQ[(level + 1) & 1], b->ctr, level, chunk);
+                const bool drain = drainLevel > 0 && level >= drainLevel;
+                hipLaunchKernelGGL(drain ? kdrain : kern, dim3(grid), dim3(block), 0, st, b->dCursor, b->dRounds,
+                                   b->dItems, b->dArgs, b->dPaths, b->dQ[level & 1], b->dQ[(level + 1) & 1], b->ctr,
+                                   level, chunk);
+                if (drain) break;
         }
         hipLaunchKernelGGL(k_batch_film, dim3(filmBlocks), dim3(kBatchBlock), 0, st, b->dCursor, b->dRounds, b->dItems,
                            b->dFilms, b->dPaths);
@@ -423,9 +448,10 @@ extern "C" int gz_batch_set_launch(void *batch, const GzMisLaunch *launch) {
         }
         const GzMisLaunch &c = *launch;
         if (c.queued != 1 || (c.block != 64 && c.block != 128 && c.block != 256) || c.gridScale < 1 ||
-            c.gridScale > 16 || c.chunk < 64 || c.chunk > 4096 || c.chunk % 64 != 0 || c.drainLevel != 0) {
+            c.gridScale > 16 || c.chunk < 64 || c.chunk > 4096 || c.chunk % 64 != 0 || c.drainLevel < 0 ||
+            c.drainLevel > 64 || c.reserved[0] != 0 || c.reserved[1] != 0 || c.reserved[2] != 0) {
                 set_error("gz_batch_set_launch: queued %d block %d gridScale %d chunk %d drainLevel %d out of range "
-                          "(queued must be 1, drainLevel 0)",
+                          "(queued must be 1)",
                           c.queued, c.block, c.gridScale, c.chunk, c.drainLevel);
                 return GZ_E_INVALID;
         }

@@ -830,6 +830,52 @@ __device__ __forceinline__ bool hit_needs_all_bsdfs(const MisArgs &a, int prim) 
         return a.mats[m].type != GZ_MAT_DIFFUSE;
 }
 
+// ---- Queue appends ----------------------------------------------------------------------------
+
+// Entries of up to four runs of 64 queue entries held per lane until their appends are written
+// (k is wave-uniform, so the selects stay in registers).
+struct Keep4 {
+        int e0 = -1, e1 = -1, e2 = -1, e3 = -1;
+        __device__ __forceinline__ void set(int k, int v) {
+                if (k == 0)
+                        e0 = v;
+                else if (k == 1)
+                        e1 = v;
+                else if (k == 2)
+                        e2 = v;
+                else
+                        e3 = v;
+        }
+        __device__ __forceinline__ int get(int k) const { return k == 0 ? e0 : k == 1 ? e1 : k == 2 ? e2 : e3; }
+};
+
+// Writes the appends of `runs` held runs: bits[q] bit k = the lane's entry of run k goes to queue q.
+// One atomic per queue and flush: a counter that every wave of the chip hits serialises them.
+template <int kQ>
+__device__ __forceinline__ void flush_appends(const Keep4 &keep, const unsigned (&bits)[kQ], int runs,
+                                              int *const (&qout)[kQ], unsigned *const (&outN)[kQ], int lane,
+                                              uint64_t below) {
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+                unsigned total = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                        if (k < runs) total += (unsigned)__popcll(__ballot((bits[q] >> k) & 1u));
+                if (total == 0) continue;
+                unsigned base = 0;
+                if (lane == 0) base = atomicAdd(outN[q], total);
+                base = __shfl(base, 0);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                        if (k >= runs) break;
+                        const bool want = (bits[q] >> k) & 1u;
+                        const uint64_t m = __ballot(want);
+                        if (want) qout[q][base + (unsigned)__popcll(m & below)] = keep.get(k);
+                        base += (unsigned)__popcll(m);
+                }
+        }
+}
+
 // Host side (gz_mis.hip): the device arguments of a scene's MIS tables and parameters (false
 // with the error set when the handle has none or the parameters are out of range), and whether
 // every material is diffuse (the kernels built without the other BSDFs then suffice).

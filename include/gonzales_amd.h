@@ -372,7 +372,8 @@ int gz_gpu_mis_trace_device(void *handle, struct GzPath *d_paths, int64_t count,
  *   drainLevel: 0, or 1..64 with a queued mode: the paths left at that bounce level run all their
  *               remaining bounces in one launch (a lane per path), in place of one launch
  *               sequence per level for the few paths that Russian roulette leaves;
- *   reserved  : zero. */
+ *   reserved  : zero.
+ * The default of a handle: {3, 256, 1, 64, 3} above 65536 primitives, else {1, 256, 1, 256, 3}. */
 struct GzMisLaunch {
         int32_t queued;
         int32_t block;
@@ -502,8 +503,8 @@ struct GzBatchInfo {
  * failure, with gz_last_error() set. */
 void *gz_batch_create(const struct GzBatchScene *scenes, int32_t sceneCount, int64_t roundPaths);
 int gz_batch_info(void *batch, struct GzBatchInfo *info);
-/* Launch shape of the batch's persistent bounce kernel (queued must be 1 and drainLevel 0; see
- * GzMisLaunch). */
+/* Launch shape of the batch's persistent bounce kernel (queued must be 1; see GzMisLaunch; the
+ * default is {1, 256, 1, 256, 3}). */
 int gz_batch_set_launch(void *batch, const struct GzMisLaunch *launch);
 /* Renders every path of every scene once and adds it to the scenes' films, asynchronously on
  * `stream`. Each round is the same launch sequence (camera rays, device path queues per bounce

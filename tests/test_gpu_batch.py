@@ -64,12 +64,13 @@ def test_batch_films_equal_single_scene_renders(gpu):
         assert float(junk) == float(junk)
         for k, (got, ref) in enumerate(zip(_films(entries), refs)):
             assert got.tobytes() == ref.tobytes(), (mode, k)
-    batch.set_launch(128, 2, 64)
-    batch.clear_films()
-    batch.render(use_graph=True)
-    torch.cuda.synchronize()
-    for k, (got, ref) in enumerate(zip(_films(entries), refs)):
-        assert got.tobytes() == ref.tobytes(), ("reshaped", k)
+    for shape in [(128, 2, 64, 3), (256, 1, 256, 0), (256, 1, 128, 1)]:
+        batch.set_launch(*shape)
+        batch.clear_films()
+        batch.render(use_graph=True)
+        torch.cuda.synchronize()
+        for k, (got, ref) in enumerate(zip(_films(entries), refs)):
+            assert got.tobytes() == ref.tobytes(), ("reshaped", shape, k)
     best, timings = autotune_batch(batch, reps=1)
     assert best in timings
     batch.render(use_graph=True)
@@ -89,7 +90,7 @@ def test_batch_rejects_bad_input(gpu):
         SceneBatch(entries, round_paths=4)
     batch = SceneBatch(entries, round_paths=1 << 14)
     import ctypes
-    for cfg in [(0, 256, 1, 256), (1, 256, 1, 100), (1, 512, 1, 256)]:
+    for cfg in [(0, 256, 1, 256), (1, 256, 1, 100), (1, 512, 1, 256), (1, 256, 1, 256, -1), (1, 256, 1, 256, 65)]:
         c = abi.GzMisLaunch(*cfg)
         assert abi.load().gz_batch_set_launch(batch.handle, ctypes.byref(c)) == abi.GZ_E_INVALID
     batch.close()

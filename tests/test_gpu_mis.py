@@ -301,7 +301,8 @@ def test_bad_launch_configurations_are_rejected(gpu):
     assert lib.gz_gpu_set_mis_launch(bh.handle, ctypes.byref(c)) == abi.GZ_E_INVALID
     out = abi.GzMisLaunch()
     assert lib.gz_gpu_get_mis_launch(bh.handle, ctypes.byref(out)) == abi.GZ_OK
-    assert (out.queued, out.block, out.gridScale, out.chunk, out.drainLevel) == (1, 256, 1, 256, 0)
+    # the default of a small scene: the queued megakernel, the last levels drained
+    assert (out.queued, out.block, out.gridScale, out.chunk, out.drainLevel) == (1, 256, 1, 256, 3)
 
 
 def test_ingested_regression_scene_matches_oracle(gpu):
