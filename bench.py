@@ -420,7 +420,7 @@ def batch_leg(args, dev, stream) -> dict:
     best, timings = autotune_batch(tune, BATCH_CANDIDATES, reps=1)
     tune.close()
     batch = SceneBatch(entries, round_paths=1 << 23)
-    batch.set_launch(best.block, best.grid_scale, best.chunk, best.drain_level)
+    batch.set_launch(best.block, best.grid_scale, best.chunk, best.drain_level, best.queued)
     info = batch.info()
     n = info.totalPaths
     batch.render(True)  # capture + warm-up

@@ -52,10 +52,13 @@ class SceneBatch:
         abi.check(abi.load().gz_batch_info(self.handle, ctypes.byref(out)), "gz_batch_info")
         return out
 
-    def set_launch(self, block: int = 256, grid_scale: int = 1, chunk: int = 256, drain_level: int = 3) -> None:
-        """Launch shape of the batch's bounce kernel; drain_level > 0 runs the bounce levels from
-        that one on in a single launch (a lane per path), 0 keeps one launch per level."""
-        c = abi.GzMisLaunch(1, block, grid_scale, chunk, drain_level)
+    def set_launch(self, block: int = 256, grid_scale: int = 1, chunk: int = 256, drain_level: int = 3,
+                   queued: int = 1) -> None:
+        """Launch shape of the batch's bounce levels: queued 1 = one persistent bounce kernel per
+        level, 3 = the full wavefront per level (closest hits sorted by BSDF class, shade kernels,
+        shadow and BSDF rays traced by their own kernels, finish kernels); drain_level > 0 runs
+        the bounce levels from that one on in a single launch (a lane per path)."""
+        c = abi.GzMisLaunch(queued, block, grid_scale, chunk, drain_level)
         abi.check(abi.load().gz_batch_set_launch(self.handle, ctypes.byref(c)), "gz_batch_set_launch")
 
     def clear_films(self) -> None:
@@ -85,16 +88,19 @@ class BatchLaunch:
     grid_scale: int = 1
     chunk: int = 256
     drain_level: int = 3
+    queued: int = 1
 
 
 BATCH_CANDIDATES = (
+    BatchLaunch(256, 1, 64, 3, 3),
+    BatchLaunch(256, 1, 256, 3, 3),
+    BatchLaunch(256, 1, 64, 2, 3),
     BatchLaunch(256, 1, 64),
     BatchLaunch(256, 1, 256),
     BatchLaunch(256, 1, 1024),
     BatchLaunch(256, 2, 256),
     BatchLaunch(128, 1, 256),
     BatchLaunch(256, 1, 256, 0),
-    BatchLaunch(256, 1, 256, 2),
 )
 
 
@@ -106,7 +112,7 @@ def autotune_batch(batch: SceneBatch, candidates=BATCH_CANDIDATES, reps: int = 1
     s = torch.cuda.current_stream()
     timings = {}
     for cfg in candidates:
-        batch.set_launch(cfg.block, cfg.grid_scale, cfg.chunk, cfg.drain_level)
+        batch.set_launch(cfg.block, cfg.grid_scale, cfg.chunk, cfg.drain_level, cfg.queued)
         batch.render(True)  # captures the graph for this shape
         torch.cuda.synchronize()
         a = torch.cuda.Event(enable_timing=True)
@@ -118,7 +124,7 @@ def autotune_batch(batch: SceneBatch, candidates=BATCH_CANDIDATES, reps: int = 1
         torch.cuda.synchronize()
         timings[cfg] = a.elapsed_time(b) / reps
     best = min(timings, key=timings.get)
-    batch.set_launch(best.block, best.grid_scale, best.chunk, best.drain_level)
+    batch.set_launch(best.block, best.grid_scale, best.chunk, best.drain_level, best.queued)
     batch.clear_films()
     return best, timings
 

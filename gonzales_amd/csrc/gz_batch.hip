@@ -33,6 +33,8 @@ struct BatchItem {
         int64_t first;      // first path of the item in the scene's crop numbering
         int32_t offset;     // first slot in the round's path buffer
         int32_t pixOffset;  // first pixel of the item among the round's pixels
+        int32_t wave;       // wavefront rounds: 1 = the wavefront kernels bounce it, 0 = the bounce kernel
+        int32_t pad;
 };
 
 struct BatchRound {
@@ -71,6 +73,8 @@ __global__ void k_batch_next(int *cursor) {
 }
 
 constexpr int kBatchBlock = 256;
+// Scenes above this many primitives take the wavefront kernels in wavefront rounds.
+constexpr int64_t kWaveMinPrims = 65536;
 
 __global__ __launch_bounds__(kBatchBlock) void k_batch_camera(const int *__restrict__ cursor,
                                                               const BatchRound *__restrict__ rounds,
@@ -103,15 +107,19 @@ __global__ void k_batch_clear(BatchCounters c) {
         for (int k = threadIdx.x; k < c.levels; k += blockDim.x) c.head[k] = 0u;
 }
 
+// One lane: each item's chunk count at `level` and their running sum, into `base`. select -1
+// counts every item, 0 and 1 only the items whose `wave` flag equals it (the others get no chunks).
 __global__ void k_batch_plan(const int *__restrict__ cursor, const BatchRound *__restrict__ rounds,
-                             const BatchItem *__restrict__ items, BatchCounters c, int level, unsigned chunk) {
+                             const BatchItem *__restrict__ items, BatchCounters c, int level, unsigned chunk,
+                             unsigned *base, int select) {
         if (threadIdx.x != 0 || blockIdx.x != 0) return;
         const BatchRound R = rounds[cursor[0]];
-        unsigned *base = c.base + (size_t)level * (c.maxItems + 1);
         unsigned sum = 0;
         for (int i = 0; i < R.nItems; ++i) {
                 base[i] = sum;
-                const unsigned n = level == 0 ? (unsigned)items[R.firstItem + i].count : c.n[level * c.maxItems + i];
+                const BatchItem &it = items[R.firstItem + i];
+                if (select >= 0 && it.wave != select) continue;
+                const unsigned n = level == 0 ? (unsigned)it.count : c.n[level * c.maxItems + i];
                 sum += (n + chunk - 1) / chunk;
         }
         base[R.nItems] = sum;
@@ -197,11 +205,302 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_film(const int *__restric
                        f.maxComponent, f.light, f.weight, f.albedo, f.normal);
 }
 
+// ---- Wavefront rounds (launch queued = 3) ------------------------------------------------------
+//
+// Each bounce level of a round as the single-scene full wavefront does it (gz_mis.hip): a
+// closest-hit kernel that also sorts the hits by BSDF class, one shade kernel per class that
+// queues each path's shadow ray and BSDF ray, an any-hit and a closest-hit kernel over those ray
+// queues and one finish kernel per class. Every queue is kept per item (at the item's slot
+// offset), so every chunk a wave takes belongs to one scene and its MisArgs stay uniform. The
+// level queues and their counters are the megakernel rounds' own (BatchCounters), so the drain
+// launch serves both.
+// kWLevel: only the chunk sums of the wavefront items' level queues (their counts are the
+// BatchCounters').
+enum : int { kWDiff = 0, kWAll = 1, kWShadow = 2, kWBsdf = 3, kWLevel = 4, kWKinds = 5 };
+
+// n[level][kind][item] entries, base[level][kind][item] the chunk running sum
+// (base[...][nItems] = the chunks of that kind).
+struct WaveCounters {
+        unsigned *n;
+        unsigned *base;
+        int maxItems;
+        int levels;
+        __device__ __forceinline__ unsigned *count(int level, int kind) const {
+                return n + ((size_t)level * kWKinds + kind) * maxItems;
+        }
+        __device__ __forceinline__ unsigned *bases(int level, int kind) const {
+                return base + ((size_t)level * kWKinds + kind) * (maxItems + 1);
+        }
+        unsigned *baseOf(int level, int kind) const { return base + ((size_t)level * kWKinds + kind) * (maxItems + 1); }
+};
+
+// Per-slot state of the wavefront rounds.
+struct WaveSlots {
+        GzHit *hits;
+        Pending *pend;
+        Ray_C *srays, *mrays;
+        uint8_t *occ;
+        GzHit *mhits;
+        int *q[4];  // kWDiff .. kWBsdf
+};
+
+__global__ void k_bw_clear(WaveCounters w) {
+        const size_t nn = (size_t)w.levels * kWKinds * w.maxItems;
+        for (size_t k = threadIdx.x; k < nn; k += blockDim.x) w.n[k] = 0u;
+}
+
+// One lane: the chunk running sums of the kinds in `mask` at `level`.
+__global__ void k_bw_plan(const int *__restrict__ cursor, const BatchRound *__restrict__ rounds, WaveCounters w,
+                          int level, int mask, unsigned chunk) {
+        if (threadIdx.x != 0 || blockIdx.x != 0) return;
+        const BatchRound R = rounds[cursor[0]];
+        for (int kind = 0; kind < kWLevel; ++kind) {
+                if (!((mask >> kind) & 1)) continue;
+                const unsigned *n = w.count(level, kind);
+                unsigned *base = w.bases(level, kind);
+                unsigned sum = 0;
+                for (int i = 0; i < R.nItems; ++i) {
+                        base[i] = sum;
+                        sum += (n[i] + chunk - 1) / chunk;
+                }
+                base[R.nItems] = sum;
+        }
+}
+
+// The next chunk of a queue kind for the wave: its item k and entries [e0, e1) of the item's
+// queue; false once the kind's chunks are done. Wave w takes the chunks w, w + waves, ... (no
+// atomic: a counter that every wave hits serialises). Wave-uniform.
+struct Chunk {
+        int k;
+        unsigned e0, e1;
+        unsigned g;  // the wave's next chunk
+};
+__device__ __forceinline__ void first_chunk(Chunk &c) {
+        c.g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+}
+__device__ __forceinline__ bool next_chunk(const BatchRound &R, const unsigned *base, const unsigned *n, unsigned chunk,
+                                           Chunk &c) {
+        const unsigned g = (unsigned)__builtin_amdgcn_readfirstlane((int)c.g);
+        c.g = g + gridDim.x * (blockDim.x >> 6);
+        if (g >= base[R.nItems]) return false;
+        c.k = __builtin_amdgcn_readfirstlane(find_item(R.nItems, (int)g, [&](int i) { return (int)base[i]; }));
+        const unsigned cnt = n[c.k];
+        c.e0 = (g - base[c.k]) * chunk;
+        c.e1 = cnt - c.e0 < chunk ? cnt : c.e0 + chunk;
+        return true;
+}
+
+// Closest hit of a path's ray through the traversal step loop (trace_ray's semantics: an
+// overflowing stack reports a miss and sets the scene's error flag).
+template <bool kAny>
+__device__ __forceinline__ void batch_trace(const MisArgs &a, const Ray_C &ray, float tMax, Stack<kLdsMax> &S,
+                                            Lane &L) {
+        lane_init(L, ray, tMax, a.rootDesc, a.sceneSafe);
+        if (a.empty) return;
+        int st;
+        do {
+                st = step<kAny, false, kLdsMax>(L, S, a.nodes, a.tris);
+        } while (st == kStepGo);
+        if (st == kStepOverflow) {
+                atomicOr(a.err, 1);
+                L.hitPrim = -1;
+        }
+}
+
+// Level `level`: the closest hit of each path of the items' level queues, sorted into the
+// items' class queues (misses and stopped paths to the diffuse one, whose shade kernel clears
+// them).
+__global__ __launch_bounds__(kMisBlock) void k_bw_trace(const int *__restrict__ cursor,
+                                                        const BatchRound *__restrict__ rounds,
+                                                        const BatchItem *__restrict__ items,
+                                                        const MisArgs *__restrict__ args,
+                                                        const GzPath *__restrict__ paths,
+                                                        const int *__restrict__ qin, BatchCounters c,
+                                                        WaveCounters w, WaveSlots ws, int level, unsigned chunk) {
+        Stack<kLdsMax> S;
+        const BatchRound R = rounds[cursor[0]];
+        const BatchItem *its = items + R.firstItem;
+        const unsigned *base = w.bases(level, kWLevel);
+        const unsigned *nIn = c.n + (size_t)level * c.maxItems;
+        const int lane = (int)(threadIdx.x & 63);
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        Chunk ch;
+        first_chunk(ch);
+        for (;;) {
+                const unsigned g = (unsigned)__builtin_amdgcn_readfirstlane((int)ch.g);
+                ch.g = g + gridDim.x * (blockDim.x >> 6);
+                if (g >= base[R.nItems]) break;
+                ch.k = __builtin_amdgcn_readfirstlane(find_item(R.nItems, (int)g, [&](int i) { return (int)base[i]; }));
+                const BatchItem it = its[ch.k];
+                const unsigned n = level == 0 ? (unsigned)it.count : nIn[ch.k];
+                ch.e0 = (g - base[ch.k]) * chunk;
+                ch.e1 = n - ch.e0 < chunk ? n : ch.e0 + chunk;
+                const MisArgs &a = args[__builtin_amdgcn_readfirstlane(it.scene)];
+                int *const qo[2] = {ws.q[kWDiff] + it.offset, ws.q[kWAll] + it.offset};
+                unsigned *const on[2] = {w.count(level, kWDiff) + ch.k, w.count(level, kWAll) + ch.k};
+                Keep4 keep;
+                unsigned bits[2] = {0u, 0u};
+                int runs = 0;
+                for (unsigned j0 = ch.e0; j0 < ch.e1; j0 += 64) {
+                        const unsigned j = j0 + (unsigned)lane;
+                        int idx = -1;
+                        if (j < ch.e1) idx = level == 0 ? it.offset + (int)j : qin[it.offset + (int)j];
+                        unsigned mask = 0u;
+                        if (idx >= 0) {
+                                const GzPath &p = paths[idx];
+                                GzHit h{-1, INFINITY, 0.0f, 0.0f};
+                                if (path_live(a, p)) {
+                                        Lane L;
+                                        batch_trace<false>(a, p.ray, INFINITY, S, L);
+                                        if (L.hitPrim >= 0) h = GzHit{L.hitPrim, L.tBest, L.bu, L.bv};
+                                }
+                                ws.hits[idx] = h;
+                                mask = hit_needs_all_bsdfs(a, h.prim) ? 2u : 1u;
+                        }
+                        keep.set(runs, idx);
+                        bits[0] |= (mask & 1u) << runs;
+                        bits[1] |= ((mask >> 1) & 1u) << runs;
+                        if (++runs == 4) {
+                                flush_appends<2>(keep, bits, runs, qo, on, lane, below);
+                                runs = 0;
+                                bits[0] = bits[1] = 0u;
+                        }
+                }
+                if (runs > 0) flush_appends<2>(keep, bits, runs, qo, on, lane, below);
+        }
+}
+
+// shade_phase over one class queue of the items; survivors to the items' next-level queues,
+// shadow and BSDF rays to the items' ray queues.
+template <bool kAll>
+__global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_bw_phase(const int *__restrict__ cursor,
+                                                                      const BatchRound *__restrict__ rounds,
+                                                                      const BatchItem *__restrict__ items,
+                                                                      const MisArgs *__restrict__ args,
+                                                                      GzPath *__restrict__ paths, int *__restrict__ qout,
+                                                                      BatchCounters c, WaveCounters w, WaveSlots ws,
+                                                                      int level, unsigned chunk) {
+        const int kind = kAll ? kWAll : kWDiff;
+        const BatchRound R = rounds[cursor[0]];
+        const BatchItem *its = items + R.firstItem;
+        const int lane = (int)(threadIdx.x & 63);
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        const int *qin = ws.q[kind];
+        unsigned *nNext = c.n + (size_t)(level + 1) * c.maxItems;
+        Chunk ch;
+        first_chunk(ch);
+        while (next_chunk(R, w.bases(level, kind), w.count(level, kind), chunk, ch)) {
+                const BatchItem it = its[ch.k];
+                const MisArgs &a = args[__builtin_amdgcn_readfirstlane(it.scene)];
+                int *const qo[3] = {qout + it.offset, ws.q[kWShadow] + it.offset, ws.q[kWBsdf] + it.offset};
+                unsigned *const on[3] = {nNext + ch.k, w.count(level, kWShadow) + ch.k, w.count(level, kWBsdf) + ch.k};
+                Keep4 keep;
+                unsigned bits[3] = {0u, 0u, 0u};
+                int runs = 0;
+                for (unsigned j0 = ch.e0; j0 < ch.e1; j0 += 64) {
+                        const unsigned j = j0 + (unsigned)lane;
+                        int idx = -1;
+                        if (j < ch.e1) idx = qin[it.offset + (int)j];
+                        unsigned mask = 0u;
+                        if (idx >= 0) {
+                                const GzPath p = paths[idx];
+                                Pending pd;
+                                pd.flags = 0;
+                                if (!(p.flags & GZ_PATH_ACTIVE)) {
+                                } else if (p.bounce > a.maxDepth) {
+                                        paths[idx].flags = p.flags & ~GZ_PATH_ACTIVE;
+                                } else {
+                                        const GzHit h = ws.hits[idx];
+                                        Ray_C sr, mr;
+                                        const bool go = shade_phase<false, kAll>(a, paths, idx, p, h.prim >= 0, h.prim, h.u,
+                                                                                 h.v, pd, sr, mr, nullptr);
+                                        if (pd.flags & kPendShadow) ws.srays[idx] = sr;
+                                        if (pd.flags & kPendMis) ws.mrays[idx] = mr;
+                                        mask = (go ? 1u : 0u) | ((pd.flags & kPendShadow) ? 2u : 0u) |
+                                               ((pd.flags & kPendMis) ? 4u : 0u);
+                                }
+                                ws.pend[idx] = pd;
+                        }
+                        keep.set(runs, idx);
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) bits[q] |= ((mask >> q) & 1u) << runs;
+                        if (++runs == 4) {
+                                flush_appends<3>(keep, bits, runs, qo, on, lane, below);
+                                runs = 0;
+                                bits[0] = bits[1] = bits[2] = 0u;
+                        }
+                }
+                if (runs > 0) flush_appends<3>(keep, bits, runs, qo, on, lane, below);
+        }
+}
+
+// The items' shadow rays (kAny: any hit below tMax 1 - 1e-5) or BSDF rays (closest hit).
+template <bool kAny>
+__global__ __launch_bounds__(kMisBlock) void k_bw_rays(const int *__restrict__ cursor,
+                                                       const BatchRound *__restrict__ rounds,
+                                                       const BatchItem *__restrict__ items,
+                                                       const MisArgs *__restrict__ args, WaveCounters w, WaveSlots ws,
+                                                       int level, unsigned chunk) {
+        Stack<kLdsMax> S;
+        const int kind = kAny ? kWShadow : kWBsdf;
+        const BatchRound R = rounds[cursor[0]];
+        const BatchItem *its = items + R.firstItem;
+        const int lane = (int)(threadIdx.x & 63);
+        const int *qin = ws.q[kind];
+        Chunk ch;
+        first_chunk(ch);
+        while (next_chunk(R, w.bases(level, kind), w.count(level, kind), chunk, ch)) {
+                const BatchItem it = its[ch.k];
+                const MisArgs &a = args[__builtin_amdgcn_readfirstlane(it.scene)];
+                for (unsigned j = ch.e0 + (unsigned)lane; j < ch.e1; j += 64) {
+                        const int idx = qin[it.offset + (int)j];
+                        Lane L;
+                        if (kAny) {
+                                batch_trace<true>(a, ws.srays[idx], 1.0f - 0.00001f, S, L);
+                                ws.occ[idx] = L.hitPrim >= 0 ? 1 : 0;
+                        } else {
+                                batch_trace<false>(a, ws.mrays[idx], INFINITY, S, L);
+                                ws.mhits[idx] = L.hitPrim >= 0 ? GzHit{L.hitPrim, L.tBest, L.bu, L.bv}
+                                                               : GzHit{-1, INFINITY, 0.0f, 0.0f};
+                        }
+                }
+        }
+}
+
+// finish_phase over one class queue of the items.
+__global__ __launch_bounds__(kMisBlock) void k_bw_finish(const int *__restrict__ cursor,
+                                                         const BatchRound *__restrict__ rounds,
+                                                         const BatchItem *__restrict__ items,
+                                                         const MisArgs *__restrict__ args, GzPath *__restrict__ paths,
+                                                         WaveCounters w, WaveSlots ws, int level, int kind,
+                                                         unsigned chunk) {
+        const BatchRound R = rounds[cursor[0]];
+        const BatchItem *its = items + R.firstItem;
+        const int lane = (int)(threadIdx.x & 63);
+        const int *qin = ws.q[kind];
+        Chunk ch;
+        first_chunk(ch);
+        while (next_chunk(R, w.bases(level, kind), w.count(level, kind), chunk, ch)) {
+                const BatchItem it = its[ch.k];
+                const MisArgs &a = args[__builtin_amdgcn_readfirstlane(it.scene)];
+                for (unsigned j = ch.e0 + (unsigned)lane; j < ch.e1; j += 64) {
+                        const int idx = qin[it.offset + (int)j];
+                        const Pending pd = ws.pend[idx];
+                        if (!(pd.flags & kPendWork)) continue;
+                        const bool o = (pd.flags & kPendShadow) ? ws.occ[idx] != 0 : false;
+                        const Ray_C mr = (pd.flags & kPendMis) ? ws.mrays[idx] : Ray_C{0, 0, 0, 0, 0, 0};
+                        const GzHit mh = (pd.flags & kPendMis) ? ws.mhits[idx] : GzHit{-1, 0.0f, 0.0f, 0.0f};
+                        finish_phase(a, paths, idx, pd, o, mr, mh);
+                }
+        }
+}
+
 struct Batch {
         int device = 0;
         int nScenes = 0;
         std::vector<Scene *> scenes;
         bool all = false;  // some scene has a non-diffuse material
+        bool anyWave = false, anyMega = false;  // wavefront rounds: items of each kind exist
         int levels = 0;    // maxDepth + 1 over the scenes
         int64_t totalPaths = 0;
         int roundPaths = 0;
@@ -222,6 +521,9 @@ struct Batch {
         int *dQ[2] = {nullptr, nullptr};
         BatchCounters ctr{nullptr, nullptr, nullptr, 0, 0};
         const uint32_t *sobol = nullptr;
+        // wavefront rounds (allocated by the first launch shape that asks for them)
+        WaveCounters wctr{nullptr, nullptr, 0, 0};
+        WaveSlots ws{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, {nullptr, nullptr, nullptr, nullptr}};
 
         hipStream_t captureStream = nullptr;
         hipGraph_t graph = nullptr;
@@ -241,11 +543,53 @@ static void free_batch(Batch *b) {
         if (hipGetDevice(&cur) == hipSuccess && cur != b->device) (void)hipSetDevice(b->device);
         release_graph(b);
         if (b->captureStream) (void)hipStreamDestroy(b->captureStream);
-        void *ptrs[] = {b->dCursor, b->dRounds, b->dItems, b->dArgs, b->dCams, b->dFilms, b->dPaths,
-                        b->dQ[0],   b->dQ[1],  b->ctr.n,  b->ctr.head, b->ctr.base};
+        void *ptrs[] = {b->dCursor,   b->dRounds,  b->dItems,   b->dArgs,     b->dCams,     b->dFilms,
+                        b->dPaths,    b->dQ[0],    b->dQ[1],    b->ctr.n,     b->ctr.head,  b->ctr.base,
+                        b->wctr.n,    b->wctr.base, b->ws.hits,  b->ws.pend,   b->ws.srays,
+                        b->ws.mrays,  b->ws.occ,   b->ws.mhits, b->ws.q[0],  b->ws.q[1],   b->ws.q[2],
+                        b->ws.q[3]};
         for (void *p : ptrs)
                 if (p) (void)hipFree(p);
         delete b;
+}
+
+// The wavefront rounds' counters and per-slot state (once per batch).
+static bool ensure_wave(Batch *b) {
+        if (b->wctr.n) return true;
+        const size_t slots = (size_t)b->roundPaths;
+        WaveCounters w{nullptr, nullptr, b->ctr.maxItems, b->levels};
+        WaveSlots ws{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, {nullptr, nullptr, nullptr, nullptr}};
+        bool ok = hip_ok(hipMalloc(&w.n, (size_t)w.levels * kWKinds * w.maxItems * sizeof(unsigned)), "batch wave counters") &&
+                  hip_ok(hipMalloc(&w.base, (size_t)w.levels * kWKinds * (w.maxItems + 1) * sizeof(unsigned)),
+                         "batch wave counters") &&
+                  hip_ok(hipMalloc(&ws.hits, slots * sizeof(GzHit)), "batch hits") &&
+                  hip_ok(hipMalloc(&ws.pend, slots * sizeof(Pending)), "batch pending terms") &&
+                  hip_ok(hipMalloc(&ws.srays, slots * sizeof(Ray_C)), "batch shadow rays") &&
+                  hip_ok(hipMalloc(&ws.mrays, slots * sizeof(Ray_C)), "batch BSDF rays") &&
+                  hip_ok(hipMalloc(&ws.occ, slots), "batch shadow results") &&
+                  hip_ok(hipMalloc(&ws.mhits, slots * sizeof(GzHit)), "batch BSDF ray hits");
+        for (int k = 0; ok && k < 4; ++k) ok = hip_ok(hipMalloc(&ws.q[k], slots * sizeof(int)), "batch wave queues");
+        b->wctr = w;  // freed with the batch, also after a failure
+        b->ws = ws;
+        if (!ok) {
+                void *ptrs[] = {w.n, w.base, ws.hits, ws.pend, ws.srays, ws.mrays, ws.occ, ws.mhits,
+                                ws.q[0], ws.q[1], ws.q[2], ws.q[3]};
+                for (void *p : ptrs)
+                        if (p) (void)hipFree(p);
+                b->wctr = WaveCounters{nullptr, nullptr, 0, 0};
+                b->ws = WaveSlots{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, {nullptr, nullptr, nullptr, nullptr}};
+        }
+        return ok;
+}
+
+// Persistent grid of a batch kernel: gridScale x its resident blocks per CU x CUs.
+template <typename K>
+static unsigned batch_grid(const Batch *b, K kern, int block) {
+        int perCU = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, block, 0) != hipSuccess || perCU < 1) perCU = 1;
+        const int64_t want = (int64_t)b->cuCount * perCU * b->launch.gridScale;
+        const int64_t need = ((int64_t)b->roundPaths + b->launch.chunk - 1) / b->launch.chunk * 64 / block + 1;
+        return (unsigned)std::max<int64_t>(1, std::min(want, need));
 }
 
 template <class T>
@@ -271,9 +615,67 @@ static bool enqueue_round(Batch *b, hipStream_t st) {
         hipLaunchKernelGGL(k_batch_camera, dim3(camBlocks), dim3(kBatchBlock), 0, st, b->dCursor, b->dRounds, b->dItems,
                            b->dCams, b->sobol, b->dPaths);
         hipLaunchKernelGGL(k_batch_clear, dim3(1), dim3(256), 0, st, b->ctr);
+        if (b->launch.queued == 3) {
+                hipLaunchKernelGGL(k_bw_clear, dim3(1), dim3(256), 0, st, b->wctr);
+                auto kd = k_bw_phase<false>;
+                auto ka = k_bw_phase<true>;
+                const unsigned gt = batch_grid(b, k_bw_trace, block), gd = batch_grid(b, kd, block),
+                               ga = batch_grid(b, ka, block), gany = batch_grid(b, k_bw_rays<true>, block),
+                               gclose = batch_grid(b, k_bw_rays<false>, block), gfin = batch_grid(b, k_bw_finish, block);
+                const int *c = b->dCursor;
+                // the shade and finish kernels over chunks of at least 256 entries: their work per
+                // entry is short next to the per-chunk item lookup and queue appends
+                const unsigned pchunk = std::max(256u, chunk);
+                for (int level = 0; level < b->levels; ++level) {
+                        unsigned *lbase = b->ctr.base + (size_t)level * (b->ctr.maxItems + 1);
+                        if (drainLevel > 0 && level >= drainLevel) {
+                                hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(64), 0, st, c, b->dRounds, b->dItems, b->ctr,
+                                                   level, chunk, lbase, -1);
+                                hipLaunchKernelGGL(kdrain, dim3(grid), dim3(block), 0, st, c, b->dRounds, b->dItems, b->dArgs,
+                                                   b->dPaths, b->dQ[level & 1], b->dQ[(level + 1) & 1], b->ctr, level,
+                                                   chunk);
+                                break;
+                        }
+                        // the small scenes' items: the bounce kernel; the others: the wavefront kernels
+                        if (b->anyMega) {
+                                hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(64), 0, st, c, b->dRounds, b->dItems, b->ctr,
+                                                   level, chunk, lbase, 0);
+                                hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, c, b->dRounds, b->dItems, b->dArgs,
+                                                   b->dPaths, b->dQ[level & 1], b->dQ[(level + 1) & 1], b->ctr, level,
+                                                   chunk);
+                        }
+                        if (!b->anyWave) continue;
+                        hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(64), 0, st, c, b->dRounds, b->dItems, b->ctr, level,
+                                           chunk, b->wctr.baseOf(level, kWLevel), 1);
+                        hipLaunchKernelGGL(k_bw_trace, dim3(gt), dim3(block), 0, st, c, b->dRounds, b->dItems, b->dArgs,
+                                           b->dPaths, b->dQ[level & 1], b->ctr, b->wctr, b->ws, level, chunk);
+                        hipLaunchKernelGGL(k_bw_plan, dim3(1), dim3(64), 0, st, c, b->dRounds, b->wctr, level,
+                                           (1 << kWDiff) | (1 << kWAll), pchunk);
+                        hipLaunchKernelGGL(kd, dim3(gd), dim3(block), 0, st, c, b->dRounds, b->dItems, b->dArgs, b->dPaths,
+                                           b->dQ[(level + 1) & 1], b->ctr, b->wctr, b->ws, level, pchunk);
+                        if (b->all)
+                                hipLaunchKernelGGL(ka, dim3(ga), dim3(block), 0, st, c, b->dRounds, b->dItems, b->dArgs,
+                                                   b->dPaths, b->dQ[(level + 1) & 1], b->ctr, b->wctr, b->ws, level, pchunk);
+                        hipLaunchKernelGGL(k_bw_plan, dim3(1), dim3(64), 0, st, c, b->dRounds, b->wctr, level,
+                                           (1 << kWShadow) | (1 << kWBsdf), chunk);
+                        hipLaunchKernelGGL(k_bw_rays<true>, dim3(gany), dim3(block), 0, st, c, b->dRounds, b->dItems,
+                                           b->dArgs, b->wctr, b->ws, level, chunk);
+                        hipLaunchKernelGGL(k_bw_rays<false>, dim3(gclose), dim3(block), 0, st, c, b->dRounds, b->dItems,
+                                           b->dArgs, b->wctr, b->ws, level, chunk);
+                        hipLaunchKernelGGL(k_bw_finish, dim3(gfin), dim3(block), 0, st, c, b->dRounds, b->dItems, b->dArgs,
+                                           b->dPaths, b->wctr, b->ws, level, (int)kWDiff, pchunk);
+                        if (b->all)
+                                hipLaunchKernelGGL(k_bw_finish, dim3(gfin), dim3(block), 0, st, c, b->dRounds, b->dItems,
+                                                   b->dArgs, b->dPaths, b->wctr, b->ws, level, (int)kWAll, pchunk);
+                }
+                hipLaunchKernelGGL(k_batch_film, dim3(filmBlocks), dim3(kBatchBlock), 0, st, b->dCursor, b->dRounds,
+                                   b->dItems, b->dFilms, b->dPaths);
+                hipLaunchKernelGGL(k_batch_next, dim3(1), dim3(64), 0, st, b->dCursor);
+                return hip_ok(hipGetLastError(), "batch round launch");
+        }
         for (int level = 0; level < b->levels; ++level) {
                 hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(64), 0, st, b->dCursor, b->dRounds, b->dItems, b->ctr,
-                                   level, chunk);
+                                   level, chunk, b->ctr.base + (size_t)level * (b->ctr.maxItems + 1), -1);
                 const bool drain = drainLevel > 0 && level >= drainLevel;
                 hipLaunchKernelGGL(drain ? kdrain : kern, dim3(grid), dim3(block), 0, st, b->dCursor, b->dRounds,
                                    b->dItems, b->dArgs, b->dPaths, b->dQ[level & 1], b->dQ[(level + 1) & 1], b->ctr,
@@ -324,6 +726,7 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
         std::vector<MisArgs> args((size_t)sceneCount);
         std::vector<GzCamera> cams((size_t)sceneCount);
         std::vector<BatchFilm> films((size_t)sceneCount);
+        std::vector<char> waveScene((size_t)sceneCount, 0);
         int maxDepth = 0;
         for (int32_t s = 0; s < sceneCount; ++s) {
                 const GzBatchScene &bs = scenes[s];
@@ -362,6 +765,12 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
                         return nullptr;
                 }
                 if (!mis_all_diffuse(sc)) b->all = true;
+                // the wavefront kernels pay off on large scenes (the single-scene launch's default)
+                waveScene[(size_t)s] = sc->primCount > kWaveMinPrims;
+                if (waveScene[(size_t)s])
+                        b->anyWave = true;
+                else
+                        b->anyMega = true;
                 maxDepth = std::max(maxDepth, (int)bs.params.maxDepth);
                 cams[(size_t)s] = bs.camera;
                 films[(size_t)s] = BatchFilm{bs.d_light,       bs.d_weight,         bs.d_albedo,  bs.d_normal,
@@ -392,7 +801,8 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
                                 continue;
                         }
                         const int64_t take = std::min(room, cropPixels - done);
-                        b->hItems.push_back(BatchItem{s, (int)(take * c.spp), done * c.spp, used, pixels});
+                        b->hItems.push_back(BatchItem{s, (int)(take * c.spp), done * c.spp, used, pixels,
+                                                      waveScene[(size_t)s] ? 1 : 0, 0});
                         used += (int)(take * c.spp);
                         pixels += (int)take;
                         done += take;
@@ -447,15 +857,21 @@ extern "C" int gz_batch_set_launch(void *batch, const GzMisLaunch *launch) {
                 return GZ_E_INVALID;
         }
         const GzMisLaunch &c = *launch;
-        if (c.queued != 1 || (c.block != 64 && c.block != 128 && c.block != 256) || c.gridScale < 1 ||
+        if ((c.queued != 1 && c.queued != 3) || (c.block != 64 && c.block != 128 && c.block != 256) || c.gridScale < 1 ||
             c.gridScale > 16 || c.chunk < 64 || c.chunk > 4096 || c.chunk % 64 != 0 || c.drainLevel < 0 ||
             c.drainLevel > 64 || c.reserved[0] != 0 || c.reserved[1] != 0 || c.reserved[2] != 0) {
                 set_error("gz_batch_set_launch: queued %d block %d gridScale %d chunk %d drainLevel %d out of range "
-                          "(queued must be 1)",
+                          "(queued must be 1 or 3)",
                           c.queued, c.block, c.gridScale, c.chunk, c.drainLevel);
                 return GZ_E_INVALID;
         }
         Batch *b = static_cast<Batch *>(batch);
+        if (c.queued == 3) {
+                int cur = -1;
+                if (!hip_ok(hipGetDevice(&cur), "get device") ||
+                    (cur != b->device && !hip_ok(hipSetDevice(b->device), "set device")) || !ensure_wave(b))
+                        return GZ_E_ALLOC;
+        }
         b->launch = c;
         release_graph(b);  // captured with the old shape
         return GZ_OK;

@@ -63,7 +63,8 @@ __global__ void k_queue_clear(QueueLevel *ctl, int levels) {
 }
 
 // Runs body(i) for every entry i of a queue of `nin` entries (the identity 0 .. nin-1 when qin is
-// null), taken `chunk` (a multiple of 64) at a time through the atomic `head`; body returns a mask
+// null), taken `chunk` (a multiple of 64) at a time through the atomic `head` (null: wave w takes
+// chunks w, w + waves, w + 2 waves, ... with no atomic); body returns a mask
 // of the kQ output queues i goes to, appended through their atomic counters outN after every four
 // runs of 64. Every lane of a wave reaches each ballot: the loop bounds are wave-uniform.
 template <int kQ, typename Body>
@@ -77,14 +78,22 @@ __device__ __forceinline__ void queue_run_n(unsigned nin, const int *__restrict_
         for (int q = 0; q < kQ; ++q) bits[q] = 0u;
         int runs = 0;
         unsigned c = 0, end = 0;
+        const uint64_t nWaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+        uint64_t nextChunk = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
         for (;;) {
                 if (c >= end) {
-                        unsigned t = 0;
-                        if (lane == 0) t = atomicAdd(head, chunk);
-                        t = __shfl(t, 0);
+                        uint64_t t = 0;
+                        if (head) {
+                                unsigned h = 0;
+                                if (lane == 0) h = atomicAdd(head, chunk);
+                                t = __shfl(h, 0);
+                        } else {
+                                t = nextChunk * chunk;
+                                nextChunk += nWaves;
+                        }
                         if (t >= nin) break;
-                        c = t;
-                        end = nin - t < chunk ? nin : t + chunk;
+                        c = (unsigned)t;
+                        end = nin - c < chunk ? nin : c + chunk;
                 }
                 const unsigned j = c + (unsigned)lane;
                 int i = -1;
@@ -497,7 +506,7 @@ __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_wf_phase(
         WfLevel &lv = ctl[level];
         int *const qo[3] = {qnext, qs, qm};
         unsigned *const on[3] = {&ctl[level + 1].qn, &lv.sn, &lv.mn};
-        queue_run_n<3>(kAll ? lv.an : lv.dn, qin, kAll ? &lv.ahead : &lv.dhead, chunk, qo, on, [&](int i) {
+        queue_run_n<3>(kAll ? lv.an : lv.dn, qin, nullptr, chunk, qo, on, [&](int i) {
                 const GzPath p = paths[i];
                 Pending pd;
                 pd.flags = 0;

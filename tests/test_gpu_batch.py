@@ -33,6 +33,13 @@ def _scenes():
     mb.upload_to_gpu()
     cam3 = PerspectiveCamera(look_at((0.0, 1.4, 3.5), (0.1, 0.5, -0.2), (0, 1, 0)), 35.0, Film(32, 24), 8, 5)
     out.append((VolumePathIntegrator(mb, _options(cam3, 6), sky=(0.6, 0.7, 0.8), materials=gz), cam3))
+    # two scenes above 65536 primitives: the wavefront kernels bounce their items in wavefront rounds
+    for k, kind in enumerate(("sphere", "soup")):
+        bh, gz, (eye, at, up, fov) = scenes.batch_scene({"index": 90 + k, "kind": kind, "tris": 70000, "res": 20,
+                                                          "spp": 4, "max_depth": 6, "seed": 4242 + k})
+        bh.upload_to_gpu()
+        cam = PerspectiveCamera(look_at(eye, at, up), fov, Film(20, 18), 4, 11 + k)
+        out.append((VolumePathIntegrator(bh, _options(cam, 6), sky=(0.4, 0.45, 0.5), materials=gz), cam))
     return out
 
 
@@ -64,7 +71,8 @@ def test_batch_films_equal_single_scene_renders(gpu):
         assert float(junk) == float(junk)
         for k, (got, ref) in enumerate(zip(_films(entries), refs)):
             assert got.tobytes() == ref.tobytes(), (mode, k)
-    for shape in [(128, 2, 64, 3), (256, 1, 256, 0), (256, 1, 128, 1)]:
+    for shape in [(128, 2, 64, 3), (256, 1, 256, 0), (256, 1, 128, 1), (256, 1, 64, 3, 3), (128, 1, 256, 0, 3),
+                  (64, 2, 128, 2, 3)]:
         batch.set_launch(*shape)
         batch.clear_films()
         batch.render(use_graph=True)
@@ -90,7 +98,8 @@ def test_batch_rejects_bad_input(gpu):
         SceneBatch(entries, round_paths=4)
     batch = SceneBatch(entries, round_paths=1 << 14)
     import ctypes
-    for cfg in [(0, 256, 1, 256), (1, 256, 1, 100), (1, 512, 1, 256), (1, 256, 1, 256, -1), (1, 256, 1, 256, 65)]:
+    for cfg in [(0, 256, 1, 256), (2, 256, 1, 256), (1, 256, 1, 100), (1, 512, 1, 256), (1, 256, 1, 256, -1),
+                (1, 256, 1, 256, 65)]:
         c = abi.GzMisLaunch(*cfg)
         assert abi.load().gz_batch_set_launch(batch.handle, ctypes.byref(c)) == abi.GZ_E_INVALID
     batch.close()
