@@ -830,6 +830,14 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
                 free_batch(b);
                 return nullptr;
         }
+        // Default launch: wavefront rounds when some scene is large enough for the wavefront kernels.
+        if (b->anyWave) {
+                if (!ensure_wave(b)) {
+                        free_batch(b);
+                        return nullptr;
+                }
+                b->launch = GzMisLaunch{3, 256, 1, 64, 3, {0, 0, 0}};
+        }
         return b;
 }
 

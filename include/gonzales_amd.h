@@ -503,8 +503,10 @@ struct GzBatchInfo {
  * failure, with gz_last_error() set. */
 void *gz_batch_create(const struct GzBatchScene *scenes, int32_t sceneCount, int64_t roundPaths);
 int gz_batch_info(void *batch, struct GzBatchInfo *info);
-/* Launch shape of the batch's bounce levels (queued 1 or 3, as in GzMisLaunch; the default is
- * {1, 256, 1, 256, 3}). */
+/* Launch shape of the batch's bounce levels (queued 1 or 3, as in GzMisLaunch). queued 3 bounces
+ * the items of scenes above 65536 primitives with the wavefront kernels and the others with the
+ * bounce kernel, level by level. The default is {3, 256, 1, 64, 3} when some scene is that large,
+ * else {1, 256, 1, 256, 3}. */
 int gz_batch_set_launch(void *batch, const struct GzMisLaunch *launch);
 /* Renders every path of every scene once and adds it to the scenes' films, asynchronously on
  * `stream`. Each round is the same launch sequence (camera rays, device path queues per bounce
