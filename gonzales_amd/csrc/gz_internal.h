@@ -60,7 +60,8 @@ struct Scene {
         std::mutex mu;
         MisScene *mis = nullptr;
         MisQueues *misQueues = nullptr;
-        GzMisLaunch misLaunch{1, 256, 1, 256, 0, {0, 0, 0}};
+        GzMisLaunch misLaunch{1, 256, 1, 256, 3, {0, 0, 0}};
+        bool misLaunchSet = false;  // false: misLaunch follows the scene (default_mis_launch)
         int device = 0;
         hipStream_t stream = nullptr;  // used by the synchronous host-buffer entry points
 

@@ -905,6 +905,22 @@ static int queued_levels(Scene *s, K kern, int64_t count, int maxDepth, hipStrea
         return GZ_OK;
 }
 
+// The launch a handle uses until gz_gpu_set_mis_launch sets one. The full wavefront keeps each
+// traversal at the traversal kernels' occupancy where the megakernel, with every BSDF compiled
+// in, runs at 2 waves per SIMD; the megakernel wins only on small all-diffuse scenes (4 waves per
+// SIMD, the BVH in L2), where the wavefront's extra passes cost more than they save. Both drain
+// the bounce levels from 3 on.
+bool gz::default_wavefront(const Scene *s) {
+        const bool allDiffuse = s->mis ? s->mis->allDiffuse : true;
+        return !(allDiffuse && s->primCount <= kDefaultMegakernelPrims);
+}
+
+static void refresh_default_launch(Scene *s) {
+        if (s->misLaunchSet) return;
+        s->misLaunch = default_wavefront(s) ? GzMisLaunch{3, 256, 1, 64, 3, {0, 0, 0}}
+                                            : GzMisLaunch{1, 256, 1, 256, 3, {0, 0, 0}};
+}
+
 static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisParams *params,
                       GzPathStats *d_stats, void *stream, bool toEnd, const char *who) {
         clear_error();
@@ -923,6 +939,7 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                 set_error("%s: %lld paths exceed one launch", who, (long long)count);
                 return GZ_E_INVALID;
         }
+        refresh_default_launch(s);
         const hipStream_t st = static_cast<hipStream_t>(stream);
         uint32_t *stats = reinterpret_cast<uint32_t *>(d_stats);
         const bool all = !s->mis->allDiffuse;
@@ -1096,6 +1113,7 @@ extern "C" int gz_gpu_set_mis_launch(void *handle, const GzMisLaunch *launch) {
         Scene *s = static_cast<Scene *>(handle);
         std::lock_guard<std::mutex> lock(s->mu);
         s->misLaunch = c;
+        s->misLaunchSet = true;
         return GZ_OK;
 }
 
@@ -1107,6 +1125,7 @@ extern "C" int gz_gpu_get_mis_launch(void *handle, GzMisLaunch *launch) {
         }
         Scene *s = static_cast<Scene *>(handle);
         std::lock_guard<std::mutex> lock(s->mu);
+        refresh_default_launch(s);
         *launch = s->misLaunch;
         return GZ_OK;
 }
@@ -1150,6 +1169,7 @@ extern "C" int gz_gpu_mis_adjoint_device(void *handle, GzPath *d_paths, const Gz
         }
         const hipStream_t st = static_cast<hipStream_t>(stream);
         const AdjArgs adj{d_adjoint, d_grad};
+        refresh_default_launch(s);
         if (s->misLaunch.queued) {
                 if (count > 0x7fffffffLL) {
                         set_error("gz_gpu_mis_adjoint_device: %lld paths exceed one queue (2^31 - 1)", (long long)count);

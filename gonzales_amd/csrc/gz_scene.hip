@@ -103,10 +103,6 @@ bool scene_upload_common(Scene *s, const PrimId_C *prims, int64_t nPrims, const 
                   hip_ok(hipDeviceGetAttribute(&s->cuCount, hipDeviceAttributeMultiprocessorCount, s->device),
                          "hipDeviceGetAttribute");
         s->deviceBytes += (int64_t)(sizeof(unsigned long long) * kCounterRing + sizeof(int));
-        // Default MIS launch by scene size: on large scenes the full wavefront keeps each
-        // traversal at the traversal kernels' occupancy; on small ones its launches cost more
-        // than they save. Both drain the last levels from bounce level 3.
-        s->misLaunch = nPrims > 65536 ? GzMisLaunch{3, 256, 1, 64, 3, {0, 0, 0}} : GzMisLaunch{1, 256, 1, 256, 3, {0, 0, 0}};
         if (ok && nMeshes > 0) {
                 // Per-mesh arrays for the shading kernels (kernel.mojo:488-531 uploads the same).
                 std::vector<DevMesh> dm((size_t)nMeshes);

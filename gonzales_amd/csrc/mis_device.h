@@ -881,5 +881,9 @@ __device__ __forceinline__ void flush_appends(const Keep4 &keep, const unsigned 
 // every material is diffuse (the kernels built without the other BSDFs then suffice).
 bool mis_args(Scene *s, const GzMisParams *params, MisArgs &a);
 bool mis_all_diffuse(const Scene *s);
+// Whether a scene's default launch is the full wavefront: every scene but the small all-diffuse
+// ones (at most kDefaultMegakernelPrims primitives), which the queued megakernel serves better.
+constexpr int64_t kDefaultMegakernelPrims = 65536;
+bool default_wavefront(const Scene *s);
 
 }  // namespace gz

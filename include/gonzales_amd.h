@@ -373,7 +373,9 @@ int gz_gpu_mis_trace_device(void *handle, struct GzPath *d_paths, int64_t count,
  *               remaining bounces in one launch (a lane per path), in place of one launch
  *               sequence per level for the few paths that Russian roulette leaves;
  *   reserved  : zero.
- * The default of a handle: {3, 256, 1, 64, 3} above 65536 primitives, else {1, 256, 1, 256, 3}. */
+ * Until gz_gpu_set_mis_launch sets one, a handle uses {1, 256, 1, 256, 3} if every material is
+ * diffuse and it has at most 65536 primitives, else {3, 256, 1, 64, 3} (gz_gpu_get_mis_launch
+ * returns the one in use). */
 struct GzMisLaunch {
         int32_t queued;
         int32_t block;
@@ -504,9 +506,9 @@ struct GzBatchInfo {
 void *gz_batch_create(const struct GzBatchScene *scenes, int32_t sceneCount, int64_t roundPaths);
 int gz_batch_info(void *batch, struct GzBatchInfo *info);
 /* Launch shape of the batch's bounce levels (queued 1 or 3, as in GzMisLaunch). queued 3 bounces
- * the items of scenes above 65536 primitives with the wavefront kernels and the others with the
- * bounce kernel, level by level. The default is {3, 256, 1, 64, 3} when some scene is that large,
- * else {1, 256, 1, 256, 3}. */
+ * the items of the scenes whose own default launch is the wavefront with the wavefront kernels and
+ * the others with the bounce kernel, level by level. The default is {3, 256, 1, 64, 3} when some
+ * scene's default is the wavefront, else {1, 256, 1, 256, 3}. */
 int gz_batch_set_launch(void *batch, const struct GzMisLaunch *launch);
 /* Renders every path of every scene once and adds it to the scenes' films, asynchronously on
  * `stream`. Each round is the same launch sequence (camera rays, device path queues per bounce
