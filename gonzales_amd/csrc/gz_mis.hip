@@ -134,12 +134,11 @@ __device__ __forceinline__ void queue_loop(unsigned count, const int *__restrict
 // ---- Split wavefront: one trace kernel and two shade kernels per bounce level ----------------
 //
 // k_wf_trace: persistent lanes with refill trace the closest hit of every path of the level's
-// queue (the batch traversal's refill scheme: idle lanes take new paths when `refill` of them
-// are idle), store it, and sort the path by the BSDF its hit shades with into the diffuse queue
-// (misses and stopped paths too) or the queue of the other BSDFs: one ballot per queue and
-// step, one atomic per wave and queue. k_wf_shade<kAll> then runs the rest of the bounce
-// (bounce_rest) over one class queue: the diffuse kernel is the small one (4 waves/SIMD), only
-// the other class pays for the 256-register kernel. Survivors go to the next level's queue.
+// queue (idle lanes take new paths from the queue when `refill` of them are idle) and store it;
+// k_wf_split sorts the paths by the BSDF their hit shades with into the diffuse queue (misses and
+// stopped paths too) or the queue of the other BSDFs. k_wf_shade<kAll> then runs the rest of the
+// bounce (bounce_rest) over one class queue: the diffuse kernel is the small one (4 waves/SIMD),
+// only the other class pays for the 256-register kernel. Survivors go to the next level's queue.
 struct WfLevel {
         unsigned qn, qhead;    // the level's queue (appended by the previous level's shade kernels)
         unsigned dn, dhead;    // diffuse class
@@ -405,95 +404,61 @@ __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 3) void k_mis_adjoint_queue(M
 //
 // Per level: k_wf_trace (closest hits of the paths' rays), k_wf_split (class queues), then per
 // class k_wf_phase (shade_phase: the bounce up to its shadow and BSDF rays, which it appends to
-// two ray queues, and the continuation), k_wf_rays twice (any hit of the shadow rays, closest hit
-// of the BSDF rays: lanes refilled from the queue, 7 to 8 waves per SIMD) and k_wf_finish over the
-// class queues (finish_phase). The shade kernels trace nothing, so every traversal runs at the
-// traversal kernels' occupancy.
+// two ray queues, and the continuation), k_wf_rays<1> and <2> (any hit of the shadow rays,
+// closest hit of the BSDF rays, 7 to 8 waves per SIMD) and k_wf_finish over the class queues
+// (finish_phase). The shade kernels trace nothing, so every traversal runs at the traversal
+// kernels' occupancy.
 
-// Lanes refilled from a queue of path indices; ray `rays[i]` with tMax `tmax`; the any hit
-// into occ[i] or the closest hit into hits[i].
-template <bool kAny, bool kStats>
-__global__ __launch_bounds__(kBlock) void k_wf_rays(MisArgs a, const Ray_C *__restrict__ rays, float tmax,
-                                                    const int *__restrict__ queue, const unsigned *nptr,
-                                                    unsigned *head, uint8_t *__restrict__ occ, GzHit *__restrict__ hits,
-                                                    int refill, unsigned chunk, uint32_t *__restrict__ stats) {
+// The rays of a queue of path indices, 64 per wave and chunk by static striding (no atomic),
+// each lane tracing its ray to the end: kMode 0 the paths' own rays (queue of level `level`, the
+// identity over `count` at level 0; stopped paths get a miss) into hits, 1 the shadow rays
+// `rays[i]` (any hit below 1 - 1e-5) into occ, 2 the BSDF rays (closest hit) into hits. On the
+// 5M-triangle frame, per 8M-path batch: shadow rays 2.7 ms here against 4.1 ms with k_wf_trace's
+// lane refill, BSDF rays 2.8 against 3.1 ms, but the paths' own rays 7.8 against 6.4 ms; so
+// k_wf_trace takes those.
+template <int kMode, bool kStats>
+__global__ __launch_bounds__(kBlock) void k_wf_rays(MisArgs a, const GzPath *__restrict__ paths, unsigned count,
+                                                    const int *__restrict__ qin, const unsigned *nptr,
+                                                    const Ray_C *__restrict__ rays, uint8_t *__restrict__ occ,
+                                                    GzHit *__restrict__ hits, uint32_t *__restrict__ stats) {
+        constexpr bool kAny = kMode == 1;
         Stack<kLdsMax> S;
-        Lane L;
-        int cur = -1;
-        bool done = false;
-        unsigned resBase = 0, resCount = 0;
-        bool counterDone = false, exhausted = false;
-        const int laneId = (int)(threadIdx.x & 63);
-        const uint64_t below = laneId == 0 ? 0ull : (~0ull >> (64 - laneId));
-        const unsigned nin = *nptr;
-        for (;;) {
-                if (!exhausted) {
-                        const uint64_t idle = __ballot(cur < 0);
-                        const int nIdle = __popcll(idle);
-                        if (nIdle >= refill || nIdle == 64) {
-                                const unsigned rank = (unsigned)__popcll(idle & below);
-                                const unsigned need = (unsigned)nIdle;
-                                unsigned fresh = 0, take = 0;
-                                if (resCount < need && !counterDone) {
-                                        take = need - resCount > chunk ? need - resCount : chunk;
-                                        const int leader = __ffsll((unsigned long long)idle) - 1;
-                                        if (laneId == leader) fresh = atomicAdd(head, take);
-                                        fresh = __shfl(fresh, leader);
-                                        if (fresh >= nin || nin - fresh <= take) counterDone = true;
-                                }
-                                if (cur < 0) {
-                                        unsigned mine = 0xffffffffu;
-                                        if (rank < resCount)
-                                                mine = resBase + rank;
-                                        else if (rank - resCount < take)
-                                                mine = fresh + (rank - resCount);
-                                        if (mine < nin) {
-                                                cur = queue[mine];
-                                                lane_init(L, rays[cur], tmax, a.rootDesc, a.sceneSafe);
-                                                done = a.empty;
-                                        }
-                                }
-                                if (take > 0) {
-                                        const unsigned used = need - resCount;
-                                        resBase = fresh + used;
-                                        resCount = take - used;
-                                } else {
-                                        const unsigned used = need < resCount ? need : resCount;
-                                        resBase += used;
-                                        resCount -= used;
-                                }
-                                if (counterDone && (resCount == 0 || resBase >= nin)) exhausted = true;
+        const unsigned nin = (kMode == 0 && !qin) ? count : *nptr;
+        const int lane = (int)(threadIdx.x & 63);
+        const unsigned waves = gridDim.x * (blockDim.x >> 6);
+        for (unsigned g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); (size_t)g * 64 < nin; g += waves) {
+                const unsigned j = g * 64 + (unsigned)lane;
+                if (j >= nin) continue;
+                const int idx = qin ? qin[j] : (int)j;
+                Lane L;
+                L.hitPrim = -1;
+                L.nInterior = L.nTris = 0;
+                bool trace = true;
+                if (kMode == 0) {
+                        const GzPath &p = paths[idx];
+                        trace = path_live(a, p);
+                        if (trace) lane_init(L, p.ray, INFINITY, a.rootDesc, a.sceneSafe);
+                } else {
+                        lane_init(L, rays[idx], kAny ? 1.0f - 0.00001f : INFINITY, a.rootDesc, a.sceneSafe);
+                }
+                if (trace && !a.empty) {
+                        int st;
+                        do {
+                                st = step<kAny, kStats, kLdsMax>(L, S, a.nodes, a.tris);
+                        } while (st == kStepGo);
+                        if (st == kStepOverflow) {
+                                atomicOr(a.err, 1);
+                                L.hitPrim = -1;
                         }
                 }
-                if (cur >= 0 && !done) {
-                        const int st = step<kAny, kStats, kLdsMax>(L, S, a.nodes, a.tris);
-                        if (st != kStepGo) {
-                                if (st == kStepOverflow) {
-                                        atomicOr(a.err, 1);
-                                        L.hitPrim = -1;
-                                }
-                                done = true;
-                        }
+                if (kAny)
+                        occ[idx] = L.hitPrim >= 0 ? 1 : 0;
+                else
+                        hits[idx] = L.hitPrim >= 0 ? GzHit{L.hitPrim, L.tBest, L.bu, L.bv} : GzHit{-1, INFINITY, 0.0f, 0.0f};
+                if (kStats) {
+                        stats[3 * (size_t)idx] += L.nInterior;
+                        stats[3 * (size_t)idx + 1] += L.nTris;
                 }
-                if (cur >= 0 && done) {
-                        if (kAny) {
-                                occ[cur] = L.hitPrim >= 0 ? 1 : 0;
-                        } else {
-                                GzHit h;
-                                h.prim = L.hitPrim;
-                                h.t = L.hitPrim >= 0 ? L.tBest : tmax;
-                                h.u = L.hitPrim >= 0 ? L.bu : 0.0f;
-                                h.v = L.hitPrim >= 0 ? L.bv : 0.0f;
-                                hits[cur] = h;
-                        }
-                        if (kStats) {
-                                stats[3 * (size_t)cur] += L.nInterior;
-                                stats[3 * (size_t)cur + 1] += L.nTris;
-                        }
-                        cur = -1;
-                        done = false;
-                }
-                if (exhausted && __ballot(cur >= 0) == 0ull) break;
         }
 }
 
@@ -956,8 +921,8 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                 const int refill = tc.refillLanes > 0 && tc.refillLanes <= 64 ? tc.refillLanes : 56;
                 const unsigned tchunk = (unsigned)(tc.chunk >= 1 && tc.chunk <= 4096 ? tc.chunk : 64);
                 auto ktrace = d_stats ? k_wf_trace<true> : k_wf_trace<false>;
-                auto kany = d_stats ? k_wf_rays<true, true> : k_wf_rays<true, false>;
-                auto kclose = d_stats ? k_wf_rays<false, true> : k_wf_rays<false, false>;
+                auto kany = d_stats ? k_wf_rays<1, true> : k_wf_rays<1, false>;
+                auto kclose = d_stats ? k_wf_rays<2, true> : k_wf_rays<2, false>;
                 auto kd = d_stats ? k_wf_phase<true, false> : k_wf_phase<false, false>;
                 auto ka = d_stats ? k_wf_phase<true, true> : k_wf_phase<false, true>;
                 auto tgrid_of = [&](auto kern) {
@@ -997,10 +962,10 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                         if (all)
                                 hipLaunchKernelGGL(ka, dim3(ga), dim3(c.block), 0, st, a, d_paths, q->hits, q->qa, q->wctl,
                                                    level, qout, q->qs, q->qm, q->pend, q->srays, q->mrays, chunk, stats);
-                        hipLaunchKernelGGL(kany, dim3(agrid), dim3(kBlock), 0, st, a, q->srays, 1.0f - 0.00001f, q->qs,
-                                           &lv->sn, &lv->shead, q->occ, nullptr, refill, chunk, stats);
-                        hipLaunchKernelGGL(kclose, dim3(cgrid), dim3(kBlock), 0, st, a, q->mrays, INFINITY, q->qm,
-                                           &lv->mn, &lv->mhead, nullptr, q->mhits, refill, chunk, stats);
+                        hipLaunchKernelGGL(kany, dim3(agrid), dim3(kBlock), 0, st, a, d_paths, (unsigned)count, q->qs,
+                                           &lv->sn, q->srays, q->occ, nullptr, stats);
+                        hipLaunchKernelGGL(kclose, dim3(cgrid), dim3(kBlock), 0, st, a, d_paths, (unsigned)count, q->qm,
+                                           &lv->mn, q->mrays, nullptr, q->mhits, stats);
                         hipLaunchKernelGGL(k_wf_finish, dim3(sgrid), dim3(kMisBlock), 0, st, a, d_paths, q->qd, &lv->dn,
                                            q->pend, q->occ, q->mrays, q->mhits);
                         if (all)
