@@ -149,6 +149,42 @@ class DeviceFilm:
                                                          abi.stream_ptr(stream)), "gz_gpu_film_resolve_device")
         return self.out
 
+    def resolve_aov(self, which: str, stream=None):
+        """The first-hit albedo ("albedo") or normal ("normal") image: each pixel's sum over its
+        samples divided by their filter weight, as Film.writeImages normalizes the two before
+        the denoiser takes them (Film.swift:41-58). A new (pixels * 3) tensor."""
+        import torch
+        src = {"albedo": self.albedo, "normal": self.normal}[which]
+        out = torch.empty_like(self.out)
+        bad = torch.zeros(1, dtype=torch.int32, device=out.device)
+        abi.check(abi.load().gz_gpu_film_resolve_device(abi.ptr(src), abi.ptr(self.weight), self.pixels,
+                                                         abi.ptr(out), abi.ptr(bad), abi.stream_ptr(stream)),
+                  "gz_gpu_film_resolve_device")
+        return out
+
+    def write_exrs(self, directory, name: str = "gonzales.exr") -> list[str]:
+        """Film.writeImages' three files (Film.swift:40-78): the beauty under `name`, and
+        albedo.exr and normal.exr beside it, the auxiliary inputs of a denoiser. The reference
+        first runs Open Image Denoise's RT filter on the beauty with those two (Denoiser.swift);
+        that library is not part of this build, so the beauty is written as rendered and the
+        three files are the hand-off to a denoiser run afterwards. Returns the three paths."""
+        import os
+
+        import torch
+
+        from .exr import write_exr
+        f = self.camera.film
+        shape = (f.res_y, f.res_x, 3)
+        images = [(name, self.resolve()), ("albedo.exr", self.resolve_aov("albedo")),
+                  ("normal.exr", self.resolve_aov("normal"))]
+        torch.cuda.synchronize()
+        paths = []
+        for fname, img in images:
+            path = os.path.join(str(directory), fname)
+            write_exr(path, img.cpu().numpy().reshape(shape))
+            paths.append(path)
+        return paths
+
 
 def cornell_camera(res: int = 512, spp: int = 16, seed: int = 0, **kw) -> PerspectiveCamera:
     """The Cornell box camera: `Transform [1 -0 -0 -0 -0 1 -0 -0 -0 -0 -1 -0 -0 -1 6.8 1]`,

@@ -80,6 +80,46 @@ def test_film_matches_oracle(gpu):
     ref, bad = oracle.film_resolve(light, weight)
     assert bad == 0 and int(film.bad.item()) == 0
     assert out.cpu().numpy().tobytes() == ref.tobytes()
+    # the denoiser hand-off: albedo and normal normalized by the same filter weights
+    for which, sums in (("albedo", albedo), ("normal", normal)):
+        aov = film.resolve_aov(which)
+        ref_aov, bad = oracle.film_resolve(sums, weight)
+        assert bad == 0
+        assert aov.cpu().numpy().tobytes() == ref_aov.tobytes(), which
+
+
+def test_film_exrs_are_written_for_a_denoiser(gpu, tmp_path):
+    """DeviceFilm.write_exrs writes the beauty, albedo.exr and normal.exr (Film.writeImages'
+    three files); each reads back as the resolved image."""
+    import torch
+    from gonzales_amd.exr import read_rgb
+    cam = cornell_camera(res=24, spp=4)
+    bh, _ = scenes.cornell_hierarchy()
+    bh.upload_to_gpu()
+    opt = MisOptions(max_depth=3, zsobol_seed=cam.c.zsSeed, log2_spp=cam.c.zsLog2Spp,
+                     base4_digits=cam.c.zsBase4Digits)
+    integ = VolumePathIntegrator(bh, opt)
+    n = cam.path_count
+    d = torch.empty(n * abi.GZ_PATH.itemsize, dtype=torch.uint8, device="cuda")
+    cam.paths_device(d)
+    integ.trace(d)
+    film = DeviceFilm(cam)
+    film.add(d, n)
+    paths = film.write_exrs(tmp_path, "beauty.exr")
+    assert [p.split("/")[-1] for p in paths] == ["beauty.exr", "albedo.exr", "normal.exr"]
+    want = [film.resolve(), film.resolve_aov("albedo"), film.resolve_aov("normal")]
+    for path, img in zip(paths, want):
+        got = read_rgb(path)
+        assert got.shape == (24, 24, 3)
+        assert np.array_equal(got.reshape(-1), img.cpu().numpy())
+    alb = read_rgb(paths[1])
+    nrm = read_rgb(paths[2])
+    # the film sums a pixel's samples unweighted and divides by the sum of their filter weights
+    # (Film.swift:120-133, Image.addPixel), so a pixel of one wall has albedo and normal length
+    # near the wall's, off by the ratio of sample count to weight sum
+    assert alb.min() >= 0.0 and alb.max() <= 0.8
+    ln = np.linalg.norm(nrm, axis=2)
+    assert 0.95 < np.median(ln) < 1.05
 
 
 def test_cornell_render_matches_oracle(gpu):
