@@ -143,9 +143,8 @@ struct WfLevel {
         unsigned qn, qhead;    // the level's queue (appended by the previous level's shade kernels)
         unsigned dn, dhead;    // diffuse class
         unsigned an, ahead;    // other BSDFs
-        unsigned sn, shead;    // shadow rays (full wavefront)
-        unsigned mn, mhead;    // BSDF rays (full wavefront)
-        unsigned pad[6];
+        unsigned sn, mn;       // shadow rays and BSDF rays (full wavefront; taken by static striding)
+        unsigned pad[8];
 };
 static_assert(sizeof(WfLevel) == 64, "WfLevel");
 
@@ -391,6 +390,41 @@ __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 3) void k_mis_adjoint_queue(M
         queue_loop(count, qin, qout, ctl, level, chunk, [&](int i) {
                 return bounce_one<false, kAll, true>(a, paths, i, nullptr, adj, useLds ? sg : nullptr);
         });
+        if (useLds) {
+                __syncthreads();
+                for (int k = threadIdx.x; k < a.nMats * 4; k += blockDim.x) {
+                        const double v = sg[k];
+                        if (v != 0.0) atomicAdd(&adj.grad[k], v);
+                }
+        }
+}
+
+// The replayed bounces of the levels from one on in one launch (k_mis_drain's form): a lane
+// replays its path's remaining bounces; the block flushes its LDS accumulators once at the end.
+template <bool kAll>
+__global__ __launch_bounds__(kMisBlock, kAll ? 2 : 3) void k_mis_adjoint_drain(MisArgs a, GzPath *__restrict__ paths,
+                                                                 const int *__restrict__ qin, const unsigned *nptr,
+                                                                 unsigned *head, unsigned chunk, AdjArgs adj) {
+        __shared__ double sg[kAdjLdsMats * 4];
+        const bool useLds = a.nMats <= kAdjLdsMats;
+        if (useLds) {
+                for (int k = threadIdx.x; k < a.nMats * 4; k += blockDim.x) sg[k] = 0.0;
+                __syncthreads();
+        }
+        const unsigned nin = *nptr;
+        const int lane = (int)(threadIdx.x & 63);
+        for (;;) {
+                unsigned c = 0;
+                if (lane == 0) c = atomicAdd(head, chunk);
+                c = __shfl(c, 0);
+                if (c >= nin) break;
+                const unsigned end = nin - c < chunk ? nin : c + chunk;
+                for (unsigned j = c + (unsigned)lane; j < end; j += 64) {
+                        const int i = qin[j];
+                        for (int b = 0; b <= a.maxDepth; ++b)
+                                if (!bounce_one<false, kAll, true>(a, paths, i, nullptr, adj, useLds ? sg : nullptr)) break;
+                }
+        }
         if (useLds) {
                 __syncthreads();
                 for (int k = threadIdx.x; k < a.nMats * 4; k += blockDim.x) {
@@ -1141,13 +1175,21 @@ extern "C" int gz_gpu_mis_adjoint_device(void *handle, GzPath *d_paths, const Gz
                         return GZ_E_INVALID;
                 }
                 auto kern = s->mis->allDiffuse ? k_mis_adjoint_queue<false> : k_mis_adjoint_queue<true>;
+                auto kdrain = s->mis->allDiffuse ? k_mis_adjoint_drain<false> : k_mis_adjoint_drain<true>;
                 const int block = s->misLaunch.block;
-                return queued_levels(s, kern, count, params->maxDepth, st, "MIS adjoint launch",
-                                     [&](int level, unsigned grid, const int *qin, int *qout, QueueLevel *ctl,
-                                         unsigned chunk) {
-                                             hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, a, d_paths,
-                                                                (unsigned)count, qin, qout, ctl, level, chunk, adj);
-                                     });
+                std::function<void(unsigned, const int *, unsigned *, unsigned *, unsigned)> drain;
+                if (s->misLaunch.drainLevel > 0)
+                        drain = [&](unsigned grid, const int *qin, unsigned *n, unsigned *head, unsigned chunk) {
+                                hipLaunchKernelGGL(kdrain, dim3(grid), dim3(block), 0, st, a, d_paths, qin, n, head, chunk,
+                                                   adj);
+                        };
+                return queued_levels(
+                        s, kern, count, params->maxDepth, st, "MIS adjoint launch",
+                        [&](int level, unsigned grid, const int *qin, int *qout, QueueLevel *ctl, unsigned chunk) {
+                                hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, a, d_paths, (unsigned)count, qin,
+                                                   qout, ctl, level, chunk, adj);
+                        },
+                        drain);
         }
         auto kernel = s->mis->allDiffuse ? k_mis_adjoint<false> : k_mis_adjoint<true>;
         for (int b = 0; b <= params->maxDepth; ++b) {

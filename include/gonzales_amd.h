@@ -371,7 +371,8 @@ int gz_gpu_mis_trace_device(void *handle, struct GzPath *d_paths, int64_t count,
  *   chunk     : queue entries a wave takes per atomic, a multiple of 64 in 64..4096;
  *   drainLevel: 0, or 1..64 with a queued mode: the paths left at that bounce level run all their
  *               remaining bounces in one launch (a lane per path), in place of one launch
- *               sequence per level for the few paths that Russian roulette leaves;
+ *               sequence per level for the few paths that Russian roulette leaves (the
+ *               adjoint's replay drains from the same level);
  *   reserved  : zero.
  * Until gz_gpu_set_mis_launch sets one, a handle uses {1, 256, 1, 256, 3} if every material is
  * diffuse and it has at most 65536 primitives, else {3, 256, 1, 64, 3} (gz_gpu_get_mis_launch

@@ -188,23 +188,28 @@ def test_autograd_descent_reduces_the_loss(gpu):
 
 
 def test_queued_adjoint_equals_the_full_relaunch(gpu):
-    """The replay over device path queues and the one-lane-per-path replay give the same
-    gradient (to the order of the double atomics) and the same path records."""
+    """The replay over device path queues (with and without the drain launch) and the
+    one-lane-per-path replay give the same gradient (to the order of the double atomics) and the
+    same path records."""
     import torch
     r, ta, tr = _setup(32, 8, 4)
     albedo, rough = _params(ta, tr, np.random.default_rng(5))
     r.set_parameters(albedo, rough)
     target = torch.zeros(32, 32, 3, dtype=torch.float64, device="cuda")
     out = []
-    for queued in (1, 0):
-        r.integrator.set_launch(queued, 256, 1, 256)
+    # relaunch, queued without and with the drain from level 2, the full wavefront forward
+    # (the adjoint then replays the queued form) drained from level 3
+    launches = [(0, 256, 1, 256, 0), (1, 256, 1, 256, 0), (1, 256, 1, 64, 2), (3, 256, 1, 64, 3)]
+    for cfg in launches:
+        r.integrator.set_launch(*cfg)
         img = r.render()
         ga, gr = r.backward(2 * (img.double() - target))
         out.append((img.cpu().numpy(), ga.cpu().numpy(), gr.cpu().numpy(), r.d_paths.cpu().numpy().tobytes()))
     r.integrator.set_launch()
-    assert np.array_equal(out[0][0], out[1][0])
-    assert out[0][3] == out[1][3]
-    for k in (1, 2):
-        a, b = out[0][k], out[1][k]
-        assert np.all(np.abs(a - b) <= 1e-9 * np.abs(b).max()), k
+    for cfg, o in zip(launches[1:], out[1:]):
+        assert np.array_equal(o[0], out[0][0]), cfg
+        assert o[3] == out[0][3], cfg
+        for k in (1, 2):
+            a, b = o[k], out[0][k]
+            assert np.all(np.abs(a - b) <= 1e-9 * np.abs(b).max()), (cfg, k)
     assert np.abs(out[0][1]).max() > 0 and np.abs(out[0][2]).max() > 0
