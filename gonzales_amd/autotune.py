@@ -110,18 +110,17 @@ MIS_CANDIDATES = (
     MisLaunch(3, 256, 1, 64),
     MisLaunch(3, 256, 1, 64, 2),
     MisLaunch(3, 256, 1, 64, 3),
-    MisLaunch(3, 256, 1, 256),
+    MisLaunch(3, 256, 1, 64, 4),
+    MisLaunch(3, 256, 1, 256, 3),
     MisLaunch(1, 256, 1, 64, 3),
+    MisLaunch(1, 256, 1, 256, 1),
+    MisLaunch(1, 256, 1, 256, 2),
     MisLaunch(1, 256, 1, 256, 3),
-    MisLaunch(2, 256, 1, 64),
-    MisLaunch(2, 256, 1, 256),
+    MisLaunch(2, 256, 1, 64, 3),
     MisLaunch(0, 256, 1, 256),
     MisLaunch(1, 256, 1, 64),
     MisLaunch(1, 256, 1, 256),
-    MisLaunch(1, 256, 1, 1024),
-    MisLaunch(1, 256, 2, 256),
-    MisLaunch(1, 128, 1, 256),
-    MisLaunch(1, 64, 1, 256),
+    MisLaunch(1, 128, 1, 256, 3),
 )
 
 
