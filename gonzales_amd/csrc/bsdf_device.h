@@ -700,19 +700,25 @@ __device__ __noinline__ float lay_pdf(const Bx &b, f3 wo, f3 wi, LayerRng &r) {
 }
 
 // ---- dispatch -----------------------------------------------------------------------------------
-// kAll = false compiles the diffuse BSDF only: the launcher picks it when every material of the
-// scene is diffuse, so such scenes do not pay the registers of the other BSDFs.
-template <bool kAll>
+// The BSDF set a kernel is compiled with (its kAll parameter): kBsdfDiffuse only the diffuse BSDF,
+// for scenes or class queues without other materials; kBsdfAll every BSDF; kBsdfNoLayer every
+// BSDF but the layered one, whose noinline random walk sets the register count of any kernel that
+// can call it (256 against 143 for the wavefront's shade kernel). A kernel only sees materials of
+// its set; the others evaluate to black.
+enum : int { kBsdfDiffuse = 0, kBsdfAll = 1, kBsdfNoLayer = 2 };
+template <int kAll>
 __device__ __forceinline__ f3 bx_f(const Bx &b, f3 wo, f3 wi, LayerRng &r) {
         if (!kAll || b.type == GZ_MAT_DIFFUSE) return divs(b.R, kPi);
         switch (b.type) {
         case GZ_MAT_CONDUCTOR: return mfr_f(bx_conductor(b), wo, wi);
         case GZ_MAT_DIELECTRIC: return diel_f(bx_dielectric(b), wo, wi);
-        case GZ_MAT_COATED_CONDUCTOR: return lay_f(b, wo, wi, r);
+        case GZ_MAT_COATED_CONDUCTOR:
+                if constexpr (kAll == kBsdfAll) return lay_f(b, wo, wi, r);
+                break;
         }
         return mk1(0.0f);
 }
-template <bool kAll>
+template <int kAll>
 __device__ __forceinline__ float bx_pdf(const Bx &b, f3 wo, f3 wi, LayerRng &r) {
         if (!kAll || b.type == GZ_MAT_DIFFUSE) {
                 if (!same_hemisphere(wo, wi)) return 0.0f;
@@ -721,11 +727,13 @@ __device__ __forceinline__ float bx_pdf(const Bx &b, f3 wo, f3 wi, LayerRng &r) 
         switch (b.type) {
         case GZ_MAT_CONDUCTOR: return mfr_pdf(bx_conductor(b), wo, wi);
         case GZ_MAT_DIELECTRIC: return diel_pdf(bx_dielectric(b), wo, wi);
-        case GZ_MAT_COATED_CONDUCTOR: return lay_pdf(b, wo, wi, r);
+        case GZ_MAT_COATED_CONDUCTOR:
+                if constexpr (kAll == kBsdfAll) return lay_pdf(b, wo, wi, r);
+                break;
         }
         return 0.0f;
 }
-template <bool kAll>
+template <int kAll>
 __device__ __forceinline__ BSample bx_sample(const Bx &b, f3 wo, float u0, float u1, float u2, LayerRng &r) {
         if (!kAll || b.type == GZ_MAT_DIFFUSE) {
                 f3 wi = cosine_hemisphere(u0, u1);
@@ -735,15 +743,17 @@ __device__ __forceinline__ BSample bx_sample(const Bx &b, f3 wo, float u0, float
         switch (b.type) {
         case GZ_MAT_CONDUCTOR: return mfr_sample(bx_conductor(b), wo, u0, u1);
         case GZ_MAT_DIELECTRIC: return diel_sample(bx_dielectric(b), wo, u0, u1, u2, true);
-        case GZ_MAT_COATED_CONDUCTOR: return lay_sample(b, wo, u0, u1, u2, r);
+        case GZ_MAT_COATED_CONDUCTOR:
+                if constexpr (kAll == kBsdfAll) return lay_sample(b, wo, u0, u1, u2, r);
+                break;
         }
         return bs_none();
 }
-template <bool kAll>
+template <int kAll>
 __device__ __forceinline__ bool bx_specular(const Bx &b) {
         return kAll && b.type == GZ_MAT_DIELECTRIC && tr_smooth(b.d);
 }
-template <bool kAll>
+template <int kAll>
 __device__ __forceinline__ f3 bx_albedo(const Bx &b) { return (!kAll || b.type == GZ_MAT_DIFFUSE) ? b.R : mk1(1.0f); }
 
 }  // namespace gz

@@ -127,8 +127,8 @@ __global__ void k_batch_plan(const int *__restrict__ cursor, const BatchRound *_
 // counter; survivors go to the item's next-level queue (appends held over four runs of 64 and
 // written with one atomic). kDrain: a lane runs each path's remaining bounces instead, and no
 // queue is written (the last levels of a round, in one launch).
-template <bool kAll, bool kDrain>
-__global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_batch_bounce(const int *__restrict__ cursor,
+template <int kAll, bool kDrain>
+__global__ __launch_bounds__(kMisBlock, mis_min_waves(kAll)) void k_batch_bounce(const int *__restrict__ cursor,
                                                             const BatchRound *__restrict__ rounds,
                                                             const BatchItem *__restrict__ items,
                                                             const MisArgs *__restrict__ args, GzPath *__restrict__ paths,
@@ -440,8 +440,8 @@ __global__ __launch_bounds__(kMisBlock) void k_bw_split(const int *__restrict__ 
 
 // shade_phase over one class queue of the items; survivors to the items' next-level queues,
 // shadow and BSDF rays to the items' ray queues.
-template <bool kAll>
-__global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_bw_phase(const int *__restrict__ cursor,
+template <int kAll>
+__global__ __launch_bounds__(kMisBlock, mis_min_waves(kAll)) void k_bw_phase(const int *__restrict__ cursor,
                                                                       const BatchRound *__restrict__ rounds,
                                                                       const BatchItem *__restrict__ items,
                                                                       const MisArgs *__restrict__ args,

@@ -12,7 +12,7 @@
 
 namespace gz {
 
-template <bool kStats, bool kAll>
+template <bool kStats, int kAll>
 __global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__restrict__ paths, int64_t n,
                                                           uint32_t *__restrict__ stats) {
         const int64_t i = (int64_t)blockIdx.x * kMisBlock + threadIdx.x;
@@ -21,7 +21,7 @@ __global__ __launch_bounds__(kMisBlock) void k_mis_bounce(MisArgs a, GzPath *__r
 
 // The replayed bounce with gradients; per-block LDS accumulators when the material table fits
 // (one global atomic per touched entry per block instead of one per vertex).
-template <bool kAll>
+template <int kAll>
 __global__ __launch_bounds__(kMisBlock) void k_mis_adjoint(MisArgs a, GzPath *__restrict__ paths, int64_t n,
                                                            AdjArgs adj) {
         __shared__ double sg[kAdjLdsMats * 4];
@@ -144,7 +144,8 @@ struct WfLevel {
         unsigned dn, dhead;    // diffuse class
         unsigned an, ahead;    // other BSDFs
         unsigned sn, mn;       // shadow rays and BSDF rays (full wavefront; taken by static striding)
-        unsigned pad[8];
+        unsigned nn;           // BSDFs but the layered one (full wavefront)
+        unsigned pad[7];
 };
 static_assert(sizeof(WfLevel) == 64, "WfLevel");
 
@@ -253,14 +254,19 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(MisArgs a, const GzPath *__
 // bytes it moves.
 constexpr int kSplitPer = 8;
 
+// classes: 1 every path to the diffuse queue (scenes without other materials), 2 the diffuse and
+// the all-BSDF queues, 3 the diffuse, all-BSDF (layered) and other-BSDF queues.
 __global__ __launch_bounds__(kMisBlock) void k_wf_split(MisArgs a, unsigned count, const int *__restrict__ qin,
                                                         WfLevel *ctl, int level, const GzHit *__restrict__ hits,
-                                                        int *__restrict__ qd, int *__restrict__ qa, bool split) {
+                                                        int *__restrict__ qd, int *__restrict__ qa, int *__restrict__ qn,
+                                                        int classes) {
         constexpr int kWaves = kMisBlock / 64;
         constexpr size_t kTile = (size_t)kMisBlock * kSplitPer;
-        __shared__ unsigned sCnt[kWaves][2];
-        __shared__ unsigned sBase[2];
+        __shared__ unsigned sCnt[kWaves][3];
+        __shared__ unsigned sBase[3];
         WfLevel &lv = ctl[level];
+        unsigned *counts[3] = {&lv.dn, &lv.an, &lv.nn};
+        int *queues[3] = {qd, qa, qn};
         const size_t nin = qin ? lv.qn : count;
         const int lane = (int)(threadIdx.x & 63);
         const int wave = (int)(threadIdx.x >> 6);
@@ -268,56 +274,58 @@ __global__ __launch_bounds__(kMisBlock) void k_wf_split(MisArgs a, unsigned coun
         for (size_t t0 = (size_t)blockIdx.x * kTile; t0 < nin; t0 += (size_t)gridDim.x * kTile) {
                 const size_t w0 = t0 + (size_t)wave * 64 * kSplitPer;
                 int idx[kSplitPer];
-                unsigned bitsD = 0, bitsA = 0;
-                unsigned cd = 0, ca = 0;
+                unsigned bits[3] = {0u, 0u, 0u};
+                unsigned cnt[3] = {0u, 0u, 0u};
 #pragma unroll
                 for (int k = 0; k < kSplitPer; ++k) {
                         const size_t j = w0 + (size_t)k * 64 + (size_t)lane;
                         int i = -1;
                         if (j < nin) i = qin ? qin[j] : (int)j;
                         idx[k] = i;
-                        const bool toA = i >= 0 && split && hit_needs_all_bsdfs(a, hits[i].prim);
-                        const bool toD = i >= 0 && !toA;
-                        bitsA |= (unsigned)toA << k;
-                        bitsD |= (unsigned)toD << k;
-                        cd += (unsigned)__popcll(__ballot(toD));
-                        ca += (unsigned)__popcll(__ballot(toA));
+                        int cls = -1;
+                        if (i >= 0) {
+                                const int set = classes > 1 ? hit_bsdf_set(a, hits[i].prim) : kBsdfDiffuse;
+                                cls = set == kBsdfDiffuse ? 0 : (set == kBsdfNoLayer && classes == 3) ? 2 : 1;
+                        }
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                                bits[c] |= (unsigned)(cls == c) << k;
+                                cnt[c] += (unsigned)__popcll(__ballot(cls == c));
+                        }
                 }
-                if (lane == 0) {
-                        sCnt[wave][0] = cd;
-                        sCnt[wave][1] = ca;
-                }
+                if (lane == 0)
+                        for (int c = 0; c < 3; ++c) sCnt[wave][c] = cnt[c];
                 __syncthreads();
                 if (threadIdx.x == 0) {
-                        unsigned td = 0, ta = 0;
-                        for (int w = 0; w < kWaves; ++w) {
-                                td += sCnt[w][0];
-                                ta += sCnt[w][1];
+                        for (int c = 0; c < 3; ++c) {
+                                unsigned t = 0;
+                                for (int w = 0; w < kWaves; ++w) t += sCnt[w][c];
+                                sBase[c] = t ? atomicAdd(counts[c], t) : 0u;
                         }
-                        sBase[0] = td ? atomicAdd(&lv.dn, td) : 0u;
-                        sBase[1] = ta ? atomicAdd(&lv.an, ta) : 0u;
                 }
                 __syncthreads();
-                unsigned od = sBase[0], oa = sBase[1];
-                for (int w = 0; w < wave; ++w) {
-                        od += sCnt[w][0];
-                        oa += sCnt[w][1];
+                unsigned off[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                        off[c] = sBase[c];
+                        for (int w = 0; w < wave; ++w) off[c] += sCnt[w][c];
                 }
 #pragma unroll
                 for (int k = 0; k < kSplitPer; ++k) {
-                        const bool toD = (bitsD >> k) & 1u, toA = (bitsA >> k) & 1u;
-                        const uint64_t md = __ballot(toD), ma = __ballot(toA);
-                        if (toD) qd[od + (unsigned)__popcll(md & below)] = idx[k];
-                        if (toA) qa[oa + (unsigned)__popcll(ma & below)] = idx[k];
-                        od += (unsigned)__popcll(md);
-                        oa += (unsigned)__popcll(ma);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                                const bool mine = (bits[c] >> k) & 1u;
+                                const uint64_t m = __ballot(mine);
+                                if (mine) queues[c][off[c] + (unsigned)__popcll(m & below)] = idx[k];
+                                off[c] += (unsigned)__popcll(m);
+                        }
                 }
                 __syncthreads();  // sCnt and sBase are the next tile's
         }
 }
 
-template <bool kStats, bool kAll>
-__global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_wf_shade(MisArgs a, GzPath *__restrict__ paths,
+template <bool kStats, int kAll>
+__global__ __launch_bounds__(kMisBlock, mis_min_waves(kAll)) void k_wf_shade(MisArgs a, GzPath *__restrict__ paths,
                                                                       const GzHit *__restrict__ hits,
                                                                       const int *__restrict__ qin, WfLevel *ctl,
                                                                       int level, int *__restrict__ qout,
@@ -338,8 +346,8 @@ __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_wf_shade(MisArgs a,
                   });
 }
 
-template <bool kStats, bool kAll>
-__global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_mis_queue(MisArgs a, GzPath *__restrict__ paths, unsigned count,
+template <bool kStats, int kAll>
+__global__ __launch_bounds__(kMisBlock, mis_min_waves(kAll)) void k_mis_queue(MisArgs a, GzPath *__restrict__ paths, unsigned count,
                                                          const int *__restrict__ qin, int *__restrict__ qout,
                                                          QueueLevel *ctl, int level, unsigned chunk,
                                                          uint32_t *__restrict__ stats) {
@@ -352,8 +360,8 @@ __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_mis_queue(MisArgs a
 // bounces until it stops (bounce_one returns false once the path is inactive or past maxDepth, so
 // at most maxDepth + 1 of them). For the last levels, where the few paths that Russian roulette
 // leaves cost less than one launch sequence per level.
-template <bool kStats, bool kAll>
-__global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_mis_drain(MisArgs a, GzPath *__restrict__ paths,
+template <bool kStats, int kAll>
+__global__ __launch_bounds__(kMisBlock, mis_min_waves(kAll)) void k_mis_drain(MisArgs a, GzPath *__restrict__ paths,
                                                                        const int *__restrict__ qin, const unsigned *nptr,
                                                                        unsigned *head, unsigned chunk,
                                                                        uint32_t *__restrict__ stats) {
@@ -376,7 +384,7 @@ __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_mis_drain(MisArgs a
 
 // The replayed bounce over the same queues; a persistent block flushes its LDS gradient
 // accumulators once, when the level is drained.
-template <bool kAll>
+template <int kAll>
 __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 3) void k_mis_adjoint_queue(MisArgs a, GzPath *__restrict__ paths,
                                                                  unsigned count, const int *__restrict__ qin,
                                                                  int *__restrict__ qout, QueueLevel *ctl, int level,
@@ -401,7 +409,7 @@ __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 3) void k_mis_adjoint_queue(M
 
 // The replayed bounces of the levels from one on in one launch (k_mis_drain's form): a lane
 // replays its path's remaining bounces; the block flushes its LDS accumulators once at the end.
-template <bool kAll>
+template <int kAll>
 __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 3) void k_mis_adjoint_drain(MisArgs a, GzPath *__restrict__ paths,
                                                                  const int *__restrict__ qin, const unsigned *nptr,
                                                                  unsigned *head, unsigned chunk, AdjArgs adj) {
@@ -496,8 +504,8 @@ __global__ __launch_bounds__(kBlock) void k_wf_rays(MisArgs a, const GzPath *__r
         }
 }
 
-template <bool kStats, bool kAll>
-__global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_wf_phase(
+template <bool kStats, int kAll>
+__global__ __launch_bounds__(kMisBlock, mis_min_waves(kAll)) void k_wf_phase(
         MisArgs a, GzPath *__restrict__ paths, const GzHit *__restrict__ hits, const int *__restrict__ qin,
         WfLevel *ctl, int level, int *__restrict__ qnext, int *__restrict__ qs, int *__restrict__ qm,
         Pending *__restrict__ pend, Ray_C *__restrict__ srays, Ray_C *__restrict__ mrays, unsigned chunk,
@@ -505,7 +513,8 @@ __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 4) void k_wf_phase(
         WfLevel &lv = ctl[level];
         int *const qo[3] = {qnext, qs, qm};
         unsigned *const on[3] = {&ctl[level + 1].qn, &lv.sn, &lv.mn};
-        queue_run_n<3>(kAll ? lv.an : lv.dn, qin, nullptr, chunk, qo, on, [&](int i) {
+        const unsigned nin = kAll == kBsdfAll ? lv.an : kAll == kBsdfNoLayer ? lv.nn : lv.dn;
+        queue_run_n<3>(nin, qin, nullptr, chunk, qo, on, [&](int i) {
                 const GzPath p = paths[i];
                 Pending pd;
                 pd.flags = 0;
@@ -556,7 +565,8 @@ struct MisQueue {
         int *qd = nullptr, *qa = nullptr;
         GzHit *hits = nullptr;
         size_t wfCap = 0;
-        // full wavefront: ray queues, pending terms, rays and their results
+        // full wavefront: the other-BSDF class queue, ray queues, pending terms, rays and results
+        int *qn = nullptr;
         int *qs = nullptr, *qm = nullptr;
         Pending *pend = nullptr;
         Ray_C *srays = nullptr, *mrays = nullptr;
@@ -575,8 +585,8 @@ void release_mis_queues(MisQueues *m) {
         if (!m) return;
         for (auto &kv : m->perStream) {
                 MisQueue &q = kv.second;
-                void *ptrs[] = {q.q[0], q.q[1], q.ctl, q.qd, q.qa, q.hits, q.wctl, q.qs, q.qm, q.pend, q.srays, q.mrays,
-                                q.occ, q.mhits};
+                void *ptrs[] = {q.q[0], q.q[1], q.ctl, q.qd, q.qa, q.hits, q.wctl, q.qn, q.qs, q.qm, q.pend, q.srays,
+                                q.mrays, q.occ, q.mhits};
                 for (void *p : ptrs)
                         if (p) (void)hipFree(p);
         }
@@ -640,16 +650,17 @@ static MisQueue *full_queue(Scene *s, hipStream_t stream, size_t count, int leve
         MisQueue *q = wf_queue(s, stream, count, levels);
         if (!q) return nullptr;
         if (q->fullCap < count) {
-                void *old[] = {q->qs, q->qm, q->pend, q->srays, q->mrays, q->occ, q->mhits};
+                void *old[] = {q->qn, q->qs, q->qm, q->pend, q->srays, q->mrays, q->occ, q->mhits};
                 for (void *p : old)
                         if (p) (void)hipFree(p);
-                q->qs = q->qm = nullptr;
+                q->qn = q->qs = q->qm = nullptr;
                 q->pend = nullptr;
                 q->srays = q->mrays = nullptr;
                 q->occ = nullptr;
                 q->mhits = nullptr;
                 q->fullCap = 0;
-                if (!hip_ok(hipMalloc(&q->qs, count * sizeof(int)), "shadow queue") ||
+                if (!hip_ok(hipMalloc(&q->qn, count * sizeof(int)), "other-BSDF queue") ||
+                    !hip_ok(hipMalloc(&q->qs, count * sizeof(int)), "shadow queue") ||
                     !hip_ok(hipMalloc(&q->qm, count * sizeof(int)), "BSDF ray queue") ||
                     !hip_ok(hipMalloc(&q->pend, count * sizeof(Pending)), "pending terms") ||
                     !hip_ok(hipMalloc(&q->srays, count * sizeof(Ray_C)), "shadow rays") ||
@@ -694,6 +705,8 @@ struct MisScene {
         bool hasSky = false;
         float sky[3] = {0, 0, 0};
         bool allDiffuse = true;  // picks the kernel built with the diffuse BSDF only
+        bool anyLayered = false;  // some material is the layered (coated conductor) BSDF
+        bool anyOther = false;    // some material is non-diffuse and not layered
 };
 
 void release_mis(MisScene *m) {
@@ -780,8 +793,13 @@ static int set_mis_scene(Scene *s, const GzMaterial *materials, int32_t material
         m->nChoices = (int)power.size();
         m->totalPower = total;
         m->hasSky = skyRadiance != nullptr;
-        for (int32_t i = 0; i < materialCount; ++i)
+        for (int32_t i = 0; i < materialCount; ++i) {
                 if (materials[i].type != GZ_MAT_DIFFUSE) m->allDiffuse = false;
+                if (materials[i].type == GZ_MAT_COATED_CONDUCTOR)
+                        m->anyLayered = true;
+                else if (materials[i].type != GZ_MAT_DIFFUSE)
+                        m->anyOther = true;
+        }
         if (skyRadiance)
                 for (int k = 0; k < 3; ++k) m->sky[k] = skyRadiance[k];
         const bool ok = put(&m->dMats, materials, (size_t)materialCount, "upload materials") &&
@@ -957,8 +975,13 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                 auto ktrace = d_stats ? k_wf_trace<true> : k_wf_trace<false>;
                 auto kany = d_stats ? k_wf_rays<1, true> : k_wf_rays<1, false>;
                 auto kclose = d_stats ? k_wf_rays<2, true> : k_wf_rays<2, false>;
-                auto kd = d_stats ? k_wf_phase<true, false> : k_wf_phase<false, false>;
-                auto ka = d_stats ? k_wf_phase<true, true> : k_wf_phase<false, true>;
+                auto kd = d_stats ? k_wf_phase<true, kBsdfDiffuse> : k_wf_phase<false, kBsdfDiffuse>;
+                auto ka = d_stats ? k_wf_phase<true, kBsdfAll> : k_wf_phase<false, kBsdfAll>;
+                auto kn = d_stats ? k_wf_phase<true, kBsdfNoLayer> : k_wf_phase<false, kBsdfNoLayer>;
+                // class queues: the layered BSDF's paths and the other non-diffuse ones apart, so
+                // the latter shade at 3 waves per SIMD instead of 2
+                const bool layered = s->mis->anyLayered, other = s->mis->anyOther;
+                const int classes = !all ? 1 : 3;
                 auto tgrid_of = [&](auto kern) {
                         int perCU = 0;
                         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, kBlock, 0) != hipSuccess ||
@@ -971,6 +994,7 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                 const unsigned chunk = (unsigned)c.chunk;
                 const unsigned gd = queue_grid(s, kd, c.block, count, chunk);
                 const unsigned ga = queue_grid(s, ka, c.block, count, chunk);
+                const unsigned gn = queue_grid(s, kn, c.block, count, chunk);
                 const unsigned sgrid = (unsigned)std::max<int64_t>(
                         1, std::min<int64_t>((int64_t)s->cuCount * 8, (count + kMisBlock - 1) / kMisBlock));
                 const unsigned spgrid = split_grid(s, count);
@@ -990,11 +1014,14 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                         hipLaunchKernelGGL(ktrace, dim3(tgrid), dim3(kBlock), 0, st, a, d_paths, (unsigned)count, qin,
                                            q->wctl, level, q->hits, refill, tchunk, stats);
                         hipLaunchKernelGGL(k_wf_split, dim3(spgrid), dim3(kMisBlock), 0, st, a, (unsigned)count, qin,
-                                           q->wctl, level, q->hits, q->qd, q->qa, all);
+                                           q->wctl, level, q->hits, q->qd, q->qa, q->qn, classes);
                         hipLaunchKernelGGL(kd, dim3(gd), dim3(c.block), 0, st, a, d_paths, q->hits, q->qd, q->wctl, level,
                                            qout, q->qs, q->qm, q->pend, q->srays, q->mrays, chunk, stats);
-                        if (all)
+                        if (all && layered)
                                 hipLaunchKernelGGL(ka, dim3(ga), dim3(c.block), 0, st, a, d_paths, q->hits, q->qa, q->wctl,
+                                                   level, qout, q->qs, q->qm, q->pend, q->srays, q->mrays, chunk, stats);
+                        if (all && other)
+                                hipLaunchKernelGGL(kn, dim3(gn), dim3(c.block), 0, st, a, d_paths, q->hits, q->qn, q->wctl,
                                                    level, qout, q->qs, q->qm, q->pend, q->srays, q->mrays, chunk, stats);
                         hipLaunchKernelGGL(kany, dim3(agrid), dim3(kBlock), 0, st, a, d_paths, (unsigned)count, q->qs,
                                            &lv->sn, q->srays, q->occ, nullptr, stats);
@@ -1002,9 +1029,12 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                                            &lv->mn, q->mrays, nullptr, q->mhits, stats);
                         hipLaunchKernelGGL(k_wf_finish, dim3(sgrid), dim3(kMisBlock), 0, st, a, d_paths, q->qd, &lv->dn,
                                            q->pend, q->occ, q->mrays, q->mhits);
-                        if (all)
+                        if (all && layered)
                                 hipLaunchKernelGGL(k_wf_finish, dim3(sgrid), dim3(kMisBlock), 0, st, a, d_paths, q->qa,
                                                    &lv->an, q->pend, q->occ, q->mrays, q->mhits);
+                        if (all && other)
+                                hipLaunchKernelGGL(k_wf_finish, dim3(sgrid), dim3(kMisBlock), 0, st, a, d_paths, q->qn,
+                                                   &lv->nn, q->pend, q->occ, q->mrays, q->mhits);
                         if (!hip_ok(hipGetLastError(), who)) return GZ_E_HIP;
                 }
                 return GZ_OK;
@@ -1047,7 +1077,8 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                         hipLaunchKernelGGL(ktrace, dim3(tgrid), dim3(kBlock), 0, st, a, d_paths, (unsigned)count, qin,
                                            q->wctl, level, q->hits, refill, tchunk, stats);
                         hipLaunchKernelGGL(k_wf_split, dim3(split_grid(s, count)), dim3(kMisBlock), 0, st, a,
-                                           (unsigned)count, qin, q->wctl, level, q->hits, q->qd, q->qa, all);
+                                           (unsigned)count, qin, q->wctl, level, q->hits, q->qd, q->qa, q->qa,
+                                           all ? 2 : 1);
                         hipLaunchKernelGGL(kd, dim3(gd), dim3(c.block), 0, st, a, d_paths, q->hits, q->qd, q->wctl,
                                            level, qout, chunk, stats);
                         if (all)

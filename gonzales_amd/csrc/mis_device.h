@@ -196,19 +196,19 @@ __device__ __forceinline__ f3 to_local(const Bsdf &b, f3 w) { return normalized(
 __device__ __forceinline__ f3 to_world(const Bsdf &b, f3 l) { return add(add(mul(b.x, l.x), mul(b.y, l.y)), mul(b.z, l.z)); }
 // evaluateWorldSpace: every BSDF keeps the protocol's isReflective = true and
 // isTransmissive = false, so only the reflection side is evaluated.
-template <bool kAll>
+template <int kAll>
 __device__ __forceinline__ f3 bsdf_eval_world(const Bsdf &b, f3 wo, f3 wi, LayerRng &r) {
         const f3 ol = to_local(b, wo), il = to_local(b, wi);
         const bool reflect = dot(wi, b.ng) * dot(wo, b.ng) > 0.0f;
         return reflect ? bx_f<kAll>(b.b, ol, il, r) : mk(0.0f, 0.0f, 0.0f);
 }
-template <bool kAll>
+template <int kAll>
 __device__ __forceinline__ float bsdf_pdf_world(const Bsdf &b, f3 wo, f3 wi, LayerRng &r) {
         const f3 il = to_local(b, wi), ol = to_local(b, wo);
         if (ol.z == 0.0f) return 0.0f;
         return bx_pdf<kAll>(b.b, ol, il, r);
 }
-template <bool kAll>
+template <int kAll>
 __device__ __forceinline__ void bsdf_sample_world(const Bsdf &b, f3 wo, float u0, float u1, float u2, f3 &f, f3 &wi,
                                                   float &pdf, LayerRng &r) {
         const f3 ol = to_local(b, wo);
@@ -443,7 +443,7 @@ __device__ __forceinline__ void adjoint_vertex(const MisArgs &a, const Bsdf &b, 
 // The light strategy's term of estimateDirect for an unoccluded light sample: f |cos| L weighted
 // by the power heuristic against the BSDF's pdf of the light direction, over the light pdf
 // (VolumePathIntegrator.swift sampleLightSource / MultipleImportanceSampler).
-template <bool kAll>
+template <int kAll>
 __device__ __forceinline__ f3 light_term(const MisArgs &a, const GzPath &p, const Bsdf &b, const Interaction &si,
                                          const LightSample &ls) {
         LayerRng r1 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightF);
@@ -464,7 +464,7 @@ __device__ __forceinline__ bool path_live(const MisArgs &a, const GzPath &p) {
 
 // The bounce of path i (record p, already loaded and live) after its closest hit: `traced` and
 // (prim, hu, hv) are the closest hit of p.ray. Returns whether the path goes on.
-template <bool kStats, bool kAll, bool kAdj>
+template <bool kStats, int kAll, bool kAdj>
 __device__ __forceinline__ bool bounce_rest(const MisArgs &a, GzPath *__restrict__ paths, int64_t i, GzPath p,
                                             bool traced, int prim, float hu, float hv, Work &w,
                                             uint32_t *__restrict__ stats, const AdjArgs &adj, double *lds) {
@@ -645,7 +645,7 @@ struct Pending {
 };
 static_assert(sizeof(Pending) == 64, "Pending");
 
-template <bool kStats, bool kAll>
+template <bool kStats, int kAll>
 __device__ __forceinline__ bool shade_phase(const MisArgs &a, GzPath *__restrict__ paths, int64_t i, GzPath p,
                                             bool traced, int prim, float hu, float hv, Pending &pd, Ray_C &sray,
                                             Ray_C &mray, uint32_t *__restrict__ stats) {
@@ -804,7 +804,7 @@ __device__ __forceinline__ void finish_phase(const MisArgs &a, GzPath *__restric
 }
 
 // One bounce of path i; returns whether the path goes on to another bounce.
-template <bool kStats, bool kAll, bool kAdj>
+template <bool kStats, int kAll, bool kAdj>
 __device__ __forceinline__ bool bounce_one(const MisArgs &a, GzPath *__restrict__ paths, int64_t i,
                                            uint32_t *__restrict__ stats, const AdjArgs &adj, double *lds) {
         const GzPath p = paths[i];
@@ -820,15 +820,24 @@ __device__ __forceinline__ bool bounce_one(const MisArgs &a, GzPath *__restrict_
         return bounce_rest<kStats, kAll, kAdj>(a, paths, i, p, traced, prim, hu, hv, w, stats, adj, lds);
 }
 
-// The material class a hit shades with: true when its BSDF is not the diffuse one (area-light
-// surfaces shade with materials[0], as in surface_interaction and bsdf_make).
-__device__ __forceinline__ bool hit_needs_all_bsdfs(const MisArgs &a, int prim) {
-        if (prim < 0 || prim >= a.primCount) return false;
+// The BSDF set a hit shades with (area-light surfaces shade with materials[0], as in
+// surface_interaction and bsdf_make): kBsdfDiffuse for a diffuse material or a miss, kBsdfAll for
+// the layered BSDF, kBsdfNoLayer for the other BSDFs.
+__device__ __forceinline__ int hit_bsdf_set(const MisArgs &a, int prim) {
+        if (prim < 0 || prim >= a.primCount) return kBsdfDiffuse;
         const PrimId_C pr = a.prims[prim];
         int m = pr.type == 3 ? 0 : (int)pr.materialIndex;
         if (m < 0 || m >= a.nMats) m = 0;
-        return a.mats[m].type != GZ_MAT_DIFFUSE;
+        const int t = a.mats[m].type;
+        return t == GZ_MAT_DIFFUSE ? kBsdfDiffuse : t == GZ_MAT_COATED_CONDUCTOR ? kBsdfAll : kBsdfNoLayer;
 }
+// Whether the hit needs a kernel with more than the diffuse BSDF.
+__device__ __forceinline__ bool hit_needs_all_bsdfs(const MisArgs &a, int prim) {
+        return hit_bsdf_set(a, prim) != kBsdfDiffuse;
+}
+
+// Waves per SIMD a bounce kernel compiled with BSDF set kAll asks of the compiler.
+constexpr int mis_min_waves(int kAll) { return kAll == kBsdfAll ? 2 : kAll == kBsdfNoLayer ? 3 : 4; }
 
 // ---- Queue appends ----------------------------------------------------------------------------
 
