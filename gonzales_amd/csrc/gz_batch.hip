@@ -535,6 +535,7 @@ struct Batch {
         int nScenes = 0;
         std::vector<Scene *> scenes;
         bool all = false;  // some scene has a non-diffuse material
+        int bsdfSet = kBsdfDiffuse;  // the BSDF set of the bounce kernels: the widest of the scenes' sets
         bool anyWave = false, anyMega = false;  // wavefront rounds: items of each kind exist
         int levels = 0;    // maxDepth + 1 over the scenes
         int64_t totalPaths = 0;
@@ -637,8 +638,8 @@ static bool upload(T **dst, const T *src, size_t count, const char *what) {
 static bool enqueue_round(Batch *b, hipStream_t st) {
         const unsigned chunk = (unsigned)b->launch.chunk;
         const int block = b->launch.block;
-        auto kern = b->all ? k_batch_bounce<true, false> : k_batch_bounce<false, false>;
-        auto kdrain = b->all ? k_batch_bounce<true, true> : k_batch_bounce<false, true>;
+        auto kern = pick_set(b->bsdfSet, k_batch_bounce<0, false>, k_batch_bounce<1, false>, k_batch_bounce<2, false>);
+        auto kdrain = pick_set(b->bsdfSet, k_batch_bounce<0, true>, k_batch_bounce<1, true>, k_batch_bounce<2, true>);
         const int drainLevel = b->launch.drainLevel;
         int perCU = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, block, 0) != hipSuccess || perCU < 1) perCU = 1;
@@ -652,8 +653,8 @@ static bool enqueue_round(Batch *b, hipStream_t st) {
         hipLaunchKernelGGL(k_batch_clear, dim3(1), dim3(256), 0, st, b->ctr);
         if (b->launch.queued == 3) {
                 hipLaunchKernelGGL(k_bw_clear, dim3(1), dim3(256), 0, st, b->wctr);
-                auto kd = k_bw_phase<false>;
-                auto ka = k_bw_phase<true>;
+                auto kd = k_bw_phase<kBsdfDiffuse>;
+                auto ka = b->bsdfSet == kBsdfNoLayer ? k_bw_phase<kBsdfNoLayer> : k_bw_phase<kBsdfAll>;
                 const unsigned gt = batch_grid(b, k_bw_rays<0>, block),
                                gsplit = (unsigned)std::max<int64_t>(
                                        1, std::min<int64_t>((int64_t)b->cuCount * 8,
@@ -812,6 +813,8 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
                         return nullptr;
                 }
                 if (!mis_all_diffuse(sc)) b->all = true;
+                const int set = mis_bsdf_set(sc);
+                if (set == kBsdfAll || (set == kBsdfNoLayer && b->bsdfSet == kBsdfDiffuse)) b->bsdfSet = set;
                 // in wavefront rounds, the scenes whose single-scene default is the wavefront take
                 // the wavefront kernels, the small all-diffuse ones the bounce kernel
                 waveScene[(size_t)s] = default_wavefront(sc);

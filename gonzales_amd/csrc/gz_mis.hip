@@ -385,7 +385,7 @@ __global__ __launch_bounds__(kMisBlock, mis_min_waves(kAll)) void k_mis_drain(Mi
 // The replayed bounce over the same queues; a persistent block flushes its LDS gradient
 // accumulators once, when the level is drained.
 template <int kAll>
-__global__ __launch_bounds__(kMisBlock, kAll ? 2 : 3) void k_mis_adjoint_queue(MisArgs a, GzPath *__restrict__ paths,
+__global__ __launch_bounds__(kMisBlock, kAll == kBsdfAll ? 2 : 3) void k_mis_adjoint_queue(MisArgs a, GzPath *__restrict__ paths,
                                                                  unsigned count, const int *__restrict__ qin,
                                                                  int *__restrict__ qout, QueueLevel *ctl, int level,
                                                                  unsigned chunk, AdjArgs adj) {
@@ -410,7 +410,7 @@ __global__ __launch_bounds__(kMisBlock, kAll ? 2 : 3) void k_mis_adjoint_queue(M
 // The replayed bounces of the levels from one on in one launch (k_mis_drain's form): a lane
 // replays its path's remaining bounces; the block flushes its LDS accumulators once at the end.
 template <int kAll>
-__global__ __launch_bounds__(kMisBlock, kAll ? 2 : 3) void k_mis_adjoint_drain(MisArgs a, GzPath *__restrict__ paths,
+__global__ __launch_bounds__(kMisBlock, kAll == kBsdfAll ? 2 : 3) void k_mis_adjoint_drain(MisArgs a, GzPath *__restrict__ paths,
                                                                  const int *__restrict__ qin, const unsigned *nptr,
                                                                  unsigned *head, unsigned chunk, AdjArgs adj) {
         __shared__ double sg[kAdjLdsMats * 4];
@@ -818,6 +818,14 @@ static int set_mis_scene(Scene *s, const GzMaterial *materials, int32_t material
 
 bool mis_all_diffuse(const Scene *s) { return !s->mis || s->mis->allDiffuse; }
 
+// The BSDF set a scene's bounce kernels are compiled with: the diffuse BSDF only, every BSDF, or
+// every BSDF but the layered one (a scene without coated conductors keeps its megakernel and drain
+// at 143 instead of 256 registers).
+int gz::mis_bsdf_set(const Scene *s) {
+        if (!s->mis || s->mis->allDiffuse) return kBsdfDiffuse;
+        return s->mis->anyLayered ? kBsdfAll : kBsdfNoLayer;
+}
+
 }  // namespace gz
 
 using namespace gz;
@@ -960,6 +968,7 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
         const hipStream_t st = static_cast<hipStream_t>(stream);
         uint32_t *stats = reinterpret_cast<uint32_t *>(d_stats);
         const bool all = !s->mis->allDiffuse;
+        const int bset = mis_bsdf_set(s);
         if (toEnd && s->misLaunch.queued == 3) {
                 if (count > 0x7fffffffLL) {
                         set_error("%s: %lld paths exceed one queue (2^31 - 1)", who, (long long)count);
@@ -998,8 +1007,8 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                 const unsigned sgrid = (unsigned)std::max<int64_t>(
                         1, std::min<int64_t>((int64_t)s->cuCount * 8, (count + kMisBlock - 1) / kMisBlock));
                 const unsigned spgrid = split_grid(s, count);
-                auto kdrain = d_stats ? (all ? k_mis_drain<true, true> : k_mis_drain<true, false>)
-                                      : (all ? k_mis_drain<false, true> : k_mis_drain<false, false>);
+                auto kdrain = d_stats ? pick_set(bset, k_mis_drain<true, 0>, k_mis_drain<true, 1>, k_mis_drain<true, 2>)
+                                      : pick_set(bset, k_mis_drain<false, 0>, k_mis_drain<false, 1>, k_mis_drain<false, 2>);
                 const unsigned gdrain = queue_grid(s, kdrain, c.block, count, chunk);
                 hipLaunchKernelGGL(k_wf_clear, dim3(1), dim3(256), 0, st, q->wctl, levels);
                 for (int level = 0; level <= params->maxDepth; ++level) {
@@ -1052,8 +1061,9 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                 const int refill = tc.refillLanes > 0 && tc.refillLanes <= 64 ? tc.refillLanes : 56;
                 const unsigned tchunk = (unsigned)(tc.chunk >= 1 && tc.chunk <= 4096 ? tc.chunk : 64);
                 auto ktrace = d_stats ? k_wf_trace<true> : k_wf_trace<false>;
-                auto kd = d_stats ? k_wf_shade<true, false> : k_wf_shade<false, false>;
-                auto ka = d_stats ? k_wf_shade<true, true> : k_wf_shade<false, true>;
+                auto kd = d_stats ? k_wf_shade<true, 0> : k_wf_shade<false, 0>;
+                auto ka = d_stats ? (bset == kBsdfNoLayer ? k_wf_shade<true, 2> : k_wf_shade<true, 1>)
+                                  : (bset == kBsdfNoLayer ? k_wf_shade<false, 2> : k_wf_shade<false, 1>);
                 int perCU = 0;
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, ktrace, kBlock, 0) != hipSuccess || perCU < 1)
                         perCU = 1;
@@ -1062,8 +1072,8 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                 const unsigned chunk = (unsigned)c.chunk;
                 const unsigned gd = queue_grid(s, kd, c.block, count, chunk);
                 const unsigned ga = queue_grid(s, ka, c.block, count, chunk);
-                auto kdrain = d_stats ? (all ? k_mis_drain<true, true> : k_mis_drain<true, false>)
-                                      : (all ? k_mis_drain<false, true> : k_mis_drain<false, false>);
+                auto kdrain = d_stats ? pick_set(bset, k_mis_drain<true, 0>, k_mis_drain<true, 1>, k_mis_drain<true, 2>)
+                                      : pick_set(bset, k_mis_drain<false, 0>, k_mis_drain<false, 1>, k_mis_drain<false, 2>);
                 const unsigned gdrain = queue_grid(s, kdrain, c.block, count, chunk);
                 hipLaunchKernelGGL(k_wf_clear, dim3(1), dim3(256), 0, st, q->wctl, levels);
                 for (int level = 0; level <= params->maxDepth; ++level) {
@@ -1093,10 +1103,10 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                         set_error("%s: %lld paths exceed one queue (2^31 - 1)", who, (long long)count);
                         return GZ_E_INVALID;
                 }
-                auto kern = d_stats ? (all ? k_mis_queue<true, true> : k_mis_queue<true, false>)
-                                    : (all ? k_mis_queue<false, true> : k_mis_queue<false, false>);
-                auto kdrain = d_stats ? (all ? k_mis_drain<true, true> : k_mis_drain<true, false>)
-                                      : (all ? k_mis_drain<false, true> : k_mis_drain<false, false>);
+                auto kern = d_stats ? pick_set(bset, k_mis_queue<true, 0>, k_mis_queue<true, 1>, k_mis_queue<true, 2>)
+                                    : pick_set(bset, k_mis_queue<false, 0>, k_mis_queue<false, 1>, k_mis_queue<false, 2>);
+                auto kdrain = d_stats ? pick_set(bset, k_mis_drain<true, 0>, k_mis_drain<true, 1>, k_mis_drain<true, 2>)
+                                      : pick_set(bset, k_mis_drain<false, 0>, k_mis_drain<false, 1>, k_mis_drain<false, 2>);
                 const int block = s->misLaunch.block;
                 std::function<void(unsigned, const int *, unsigned *, unsigned *, unsigned)> drain;
                 if (s->misLaunch.drainLevel > 0)
@@ -1115,8 +1125,8 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
         // A path makes at most maxDepth + 1 bounces (bounce 0 .. maxDepth); a launch after the
         // last finds every path inactive.
         const int launches = toEnd ? params->maxDepth + 1 : 1;
-        auto kernel = d_stats ? (all ? k_mis_bounce<true, true> : k_mis_bounce<true, false>)
-                              : (all ? k_mis_bounce<false, true> : k_mis_bounce<false, false>);
+        auto kernel = d_stats ? pick_set(bset, k_mis_bounce<true, 0>, k_mis_bounce<true, 1>, k_mis_bounce<true, 2>)
+                              : pick_set(bset, k_mis_bounce<false, 0>, k_mis_bounce<false, 1>, k_mis_bounce<false, 2>);
         for (int b = 0; b < launches; ++b) {
                 hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(kMisBlock), 0, st, a, d_paths, count, stats);
                 if (!hip_ok(hipGetLastError(), "MIS bounce launch")) return GZ_E_HIP;
@@ -1205,8 +1215,9 @@ extern "C" int gz_gpu_mis_adjoint_device(void *handle, GzPath *d_paths, const Gz
                         set_error("gz_gpu_mis_adjoint_device: %lld paths exceed one queue (2^31 - 1)", (long long)count);
                         return GZ_E_INVALID;
                 }
-                auto kern = s->mis->allDiffuse ? k_mis_adjoint_queue<false> : k_mis_adjoint_queue<true>;
-                auto kdrain = s->mis->allDiffuse ? k_mis_adjoint_drain<false> : k_mis_adjoint_drain<true>;
+                const int bset = mis_bsdf_set(s);
+                auto kern = pick_set(bset, k_mis_adjoint_queue<0>, k_mis_adjoint_queue<1>, k_mis_adjoint_queue<2>);
+                auto kdrain = pick_set(bset, k_mis_adjoint_drain<0>, k_mis_adjoint_drain<1>, k_mis_adjoint_drain<2>);
                 const int block = s->misLaunch.block;
                 std::function<void(unsigned, const int *, unsigned *, unsigned *, unsigned)> drain;
                 if (s->misLaunch.drainLevel > 0)
@@ -1222,7 +1233,7 @@ extern "C" int gz_gpu_mis_adjoint_device(void *handle, GzPath *d_paths, const Gz
                         },
                         drain);
         }
-        auto kernel = s->mis->allDiffuse ? k_mis_adjoint<false> : k_mis_adjoint<true>;
+        auto kernel = pick_set(mis_bsdf_set(s), k_mis_adjoint<0>, k_mis_adjoint<1>, k_mis_adjoint<2>);
         for (int b = 0; b <= params->maxDepth; ++b) {
                 hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(kMisBlock), 0, st, a, d_paths, count, adj);
                 if (!hip_ok(hipGetLastError(), "MIS adjoint launch")) return GZ_E_HIP;

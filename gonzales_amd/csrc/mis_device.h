@@ -894,5 +894,12 @@ bool mis_all_diffuse(const Scene *s);
 // ones (at most kDefaultMegakernelPrims primitives), which the queued megakernel serves better.
 constexpr int64_t kDefaultMegakernelPrims = 65536;
 bool default_wavefront(const Scene *s);
+// The BSDF set (kBsdfDiffuse, kBsdfAll or kBsdfNoLayer) a scene's megakernel-form kernels use.
+int mis_bsdf_set(const Scene *s);
+// One of three kernel instantiations by BSDF set.
+template <typename K>
+inline K pick_set(int set, K diffuse, K all, K noLayer) {
+        return set == kBsdfDiffuse ? diffuse : set == kBsdfAll ? all : noLayer;
+}
 
 }  // namespace gz
