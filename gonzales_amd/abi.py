@@ -107,7 +107,7 @@ class GzPlyInfo(ctypes.Structure):
 
 
 class GzBuildOptions(ctypes.Structure):
-    _fields_ = [("maxLeaf", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+    _fields_ = [("maxLeaf", ctypes.c_int32), ("sah", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
 
 
 class GzMisParams(ctypes.Structure):

@@ -320,10 +320,12 @@ def build_bounding_hierarchy(scene: Scene, prims: Primitives) -> BoundingHierarc
     return bh
 
 
-def build_bounding_hierarchy_gpu(scene: Scene, prims: Primitives, max_leaf: int = 4) -> BoundingHierarchy:
+def build_bounding_hierarchy_gpu(scene: Scene, prims: Primitives, max_leaf: int = 4,
+                                 sah: bool = False) -> BoundingHierarchy:
     """The same hierarchy object built on the device (gz_gpu_build_scene: a Morton-order linear
-    BVH). The handle is ready for every device entry point; the tree is also read back in the
-    BVH2 layout (gz_gpu_export_bvh2) with the PrimIds in leaf order, for host consumers."""
+    BVH, or with sah the host builder's binned-SAH decisions made top-down on the device). The
+    handle is ready for every device entry point; the tree is also read back in the BVH2 layout
+    (gz_gpu_export_bvh2) with the PrimIds in leaf order, for host consumers."""
     lib = abi.load()
     bh = BoundingHierarchy(scene, pack_prim_ids(prims), np.zeros(0, abi.BVH2_NODE))
     bh.prepare_materials()
@@ -332,7 +334,7 @@ def build_bounding_hierarchy_gpu(scene: Scene, prims: Primitives, max_leaf: int 
     pts = np.array([m.points.size for m in meshes] or [0], np.int64)
     faces = np.array([m.face_indices.size for m in meshes] or [0], np.int64)
     vtx = np.array([m.vertex_indices.size for m in meshes] or [0], np.int64)
-    opt = abi.GzBuildOptions(max_leaf)
+    opt = abi.GzBuildOptions(max_leaf, 1 if sah else 0)
     h = lib.gz_gpu_build_scene(abi.ptr(bh.prim_ids), len(bh.prim_ids), ctypes.addressof(meshes_c), len(meshes),
                                abi.ptr(pts), abi.ptr(faces), abi.ptr(vtx), abi.ptr(bh.materials_c),
                                len(bh.materials_c), ctypes.byref(opt))

@@ -427,13 +427,14 @@ __global__ __launch_bounds__(kBuildBlock) void k_sorted(const uint32_t *__restri
 __global__ __launch_bounds__(kBuildBlock) void k_emit(const int4 *__restrict__ node, const int *__restrict__ parent,
                                                       const Box *__restrict__ nodeBox, const Box *__restrict__ primBox,
                                                       const uint32_t *__restrict__ order, int n, int maxLeaf,
-                                                      NodePair *__restrict__ pairs, TriRec *__restrict__ tris,
-                                                      int *__restrict__ flags) {
+                                                      const int *__restrict__ leafMark, NodePair *__restrict__ pairs,
+                                                      TriRec *__restrict__ tris, int *__restrict__ flags) {
         const int i = (int)(blockIdx.x * kBuildBlock + threadIdx.x);
         const int m = n - 1;
         if (i >= m) return;
         const int4 nd = node[i];
-        if (nd.y - nd.x + 1 <= maxLeaf) return;
+        // a leaf: a range of at most maxLeaf primitives, or (SAH build) one the host builder keeps whole
+        if (leafMark ? leafMark[i] != 0 : nd.y - nd.x + 1 <= maxLeaf) return;
         int desc[2];
         Box cb[2];
         const int g = nd.z >= 0 ? nd.z : ~nd.z;  // split: left [first, g], right [g + 1, last]
@@ -450,7 +451,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_emit(const int4 *__restrict__ n
                         return;
                 }
                 const int size = cLast - cFirst + 1;
-                if (size <= maxLeaf) {
+                if (leafMark ? (c < 0 || leafMark[c] != 0) : size <= maxLeaf) {
                         desc[side] = ~cFirst;
                         tris[cFirst].p0.w = __int_as_float(size);
                         atomicAdd(&flags[3], 1);
@@ -476,8 +477,502 @@ __global__ __launch_bounds__(kBuildBlock) void k_emit(const int4 *__restrict__ n
         atomicMax(&flags[2], depth + 1);
 }
 
+// ---- Binned SAH hierarchy (GzBuildOptions.sah = 1) ----------------------------------------------
+//
+// The host builder's decisions (bvh_build.cpp after BoundingHierarchyBuilder.swift:199-339) made
+// on the device, top-down: a node's bounds and centroid bounds, its longest centroid axis, 12
+// buckets, the SAH costs in the same float arithmetic, and the split "bucket > best goes right".
+// Every decision depends only on the set of primitives in a node, so the tree's nodes hold the
+// same sets as the host build's (the order inside a child differs: a stable partition here, a
+// Hoare partition there). Nodes of more than kSahSmall primitives are split level by level, one
+// workgroup per node; smaller ones are finished by one lane each. The result is written in the
+// hierarchy layout k_hierarchy writes (a node over [first, last] split at g has children g and
+// g + 1, a leaf position ~p), which any binary tree over the final primitive order fits, so the
+// validation, refit and emit kernels serve both builds. Where the host makes a leaf, this build
+// splits the range at its middle: emit turns every node of at most maxLeaf primitives into one
+// leaf. A node the SAH cannot split (flat centroids, an empty box) is split at its middle too.
+constexpr int kSahSmall = 32;
+constexpr int kSahBuckets = 12;
+
+struct SahTask {
+        int lo, hi, me;  // primitives [lo, hi) of the order array; the node's index
+        int leaf;        // 1: inside a range the host builder makes one leaf
+};
+
+struct B3 {
+        float mn[3], mx[3];
+};
+__device__ __forceinline__ B3 b3_empty() { return B3{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}}; }
+// Swift's min/max on Comparable: min(x, y) = y < x ? y : x; max(x, y) = y >= x ? y : x.
+__device__ __forceinline__ float sw_min(float x, float y) { return y < x ? y : x; }
+__device__ __forceinline__ float sw_max(float x, float y) { return y >= x ? y : x; }
+// Bounds3.swift:104-114 with Bounds3f(first:second:) re-sorting the corners.
+__device__ __forceinline__ B3 b3_unite(const B3 &a, const B3 &b) {
+        B3 r, s;
+        for (int d = 0; d < 3; ++d) {
+                r.mn[d] = sw_min(a.mn[d], b.mn[d]);
+                r.mx[d] = sw_max(a.mx[d], b.mx[d]);
+        }
+        for (int d = 0; d < 3; ++d) {
+                s.mn[d] = sw_min(r.mn[d], r.mx[d]);
+                s.mx[d] = sw_max(r.mn[d], r.mx[d]);
+        }
+        return s;
+}
+__device__ __forceinline__ float b3_area(const B3 &b) {  // Bounds3.swift:98-101
+        const float dx = b.mx[0] - b.mn[0], dy = b.mx[1] - b.mn[1], dz = b.mx[2] - b.mn[2];
+        float s = dx * dy + dx * dz;
+        s = s + dy * dz;
+        return 2.0f * s;
+}
+__device__ __forceinline__ int b3_max_extent(const B3 &b) {  // Bounds3.swift:75-84
+        const float dx = b.mx[0] - b.mn[0], dy = b.mx[1] - b.mn[1], dz = b.mx[2] - b.mn[2];
+        if (dx > dy && dx > dz) return 0;
+        if (dy > dz) return 1;
+        return 2;
+}
+// The host builder's centre: 0.5 pMin + 0.5 pMax (Bounds3.swift:168-171).
+__device__ __forceinline__ float sah_center(const Box &b, int d) {
+        const float h0 = 0.5f * b.lo[d];
+        const float h1 = 0.5f * b.hi[d];
+        return h0 + h1;
+}
+// bucket_of: Bounds3.offset on one axis, 12 buckets, Int() truncation, clamped to 0..11.
+__device__ __forceinline__ int sah_bucket(const B3 &cb, float c, int d) {
+        float r = c - cb.mn[d];
+        if (cb.mx[d] > cb.mn[d]) r /= cb.mx[d] - cb.mn[d];
+        const float f = 12.0f * r;
+        if (f >= 12.0f) return kSahBuckets - 1;
+        if (!(f > 0.0f)) return 0;
+        const int b = (int)f;
+        return b > kSahBuckets - 1 ? kSahBuckets - 1 : b;
+}
+// The SAH costs of the 11 bucket splits and the decision (bvh_build.cpp's build, in its order of
+// float operations): true to split at bucket `best`, false for a leaf.
+__device__ __forceinline__ bool sah_decide(const int *cnt, const B3 *bb, int count, const B3 &bounds, int &best) {
+        float costs[kSahBuckets - 1];
+        for (int k = 0; k < kSahBuckets - 1; ++k) costs[k] = 0.0f;
+        int below = 0;
+        B3 bBelow = b3_empty();
+        for (int k = 0; k < kSahBuckets - 1; ++k) {
+                bBelow = b3_unite(bBelow, bb[k]);
+                below += cnt[k];
+                costs[k] = costs[k] + (float)below * b3_area(bBelow);
+        }
+        int above = 0;
+        B3 bAbove = b3_empty();
+        for (int k = kSahBuckets - 1; k >= 1; --k) {
+                bAbove = b3_unite(bAbove, bb[k]);
+                above += cnt[k];
+                costs[k - 1] = costs[k - 1] + (float)above * b3_area(bAbove);
+        }
+        best = -1;
+        float minCost = INFINITY;
+        for (int k = 0; k < kSahBuckets - 1; ++k)
+                if (costs[k] < minCost) {
+                        minCost = costs[k];
+                        best = k;
+                }
+        const float leafCost = (float)count;
+        minCost = 0.5f + minCost / b3_area(bounds);
+        return count > 4 || minCost < leafCost;
+}
+
+// Records node `me` over [lo, hi) split before `mid` and its children: a child of one primitive
+// is a leaf position, a larger one an internal node (index mid - 1 on the left, mid on the
+// right) queued as a task of the small or the next big list.
+__device__ __forceinline__ void sah_record(int lo, int mid, int hi, int me, int leaf, int4 *__restrict__ node,
+                                           int *__restrict__ leafMark, int *__restrict__ parent,
+                                           int *__restrict__ leafParent, int *__restrict__ parents,
+                                           int *__restrict__ leafParents, SahTask *__restrict__ big, unsigned *nBig,
+                                           SahTask *__restrict__ small, unsigned *nSmall, int cap,
+                                           int *__restrict__ flags) {
+        const int g = mid - 1;
+        const int left = lo == g ? ~g : g;
+        const int right = hi - 1 == mid ? ~mid : mid;
+        node[me] = make_int4(lo, hi - 1, left, right);
+        leafMark[me] = leaf;
+        const int cl[2] = {lo, mid}, ch[2] = {mid, hi}, ci[2] = {left, right};
+        for (int side = 0; side < 2; ++side) {
+                const int c = ci[side];
+                if (c < 0) {
+                        leafParent[~c] = me;
+                        atomicAdd(&leafParents[~c], 1);
+                        continue;
+                }
+                parent[c] = me;
+                atomicAdd(&parents[c], 1);
+                const SahTask t{cl[side], ch[side], c, leaf};
+                if (ch[side] - cl[side] <= kSahSmall) {
+                        if (!small) continue;  // the lane's own stack takes it
+                        const unsigned k = atomicAdd(nSmall, 1u);
+                        if (k < (unsigned)cap)
+                                small[k] = t;
+                        else
+                                atomicOr(&flags[0], kFlagBadTree);
+                } else {
+                        const unsigned k = atomicAdd(nBig, 1u);
+                        if (k < (unsigned)cap)
+                                big[k] = t;
+                        else
+                                atomicOr(&flags[0], kFlagBadTree);
+                }
+        }
+}
+
+// One level of big nodes, one workgroup per node (persistent over the level's list).
+__global__ __launch_bounds__(kBuildBlock) void k_sah_level(const Box *__restrict__ primBox, uint32_t *__restrict__ order,
+                                                           uint32_t *__restrict__ tmp, const SahTask *__restrict__ tasks,
+                                                           const unsigned *nTasks, unsigned *head,
+                                                           SahTask *__restrict__ nextBig, unsigned *nNextBig,
+                                                           SahTask *__restrict__ small, unsigned *nSmall, int cap,
+                                                           int4 *__restrict__ node, int *__restrict__ leafMark,
+                                                           int *__restrict__ parent, int *__restrict__ leafParent,
+                                                           int *__restrict__ parents, int *__restrict__ leafParents,
+                                                           int n, int forceMid, int *__restrict__ flags) {
+        constexpr int kWaves = kBuildBlock / 64;
+        __shared__ unsigned sTask;
+        __shared__ float sRed[kWaves][12];
+        __shared__ float sB[12];  // bounds (mn, mx) then centroid bounds (mn, mx)
+        __shared__ int sCnt[kSahBuckets];
+        __shared__ unsigned sBB[kSahBuckets][6];
+        __shared__ int sPlan[4];  // dim, split bucket (-2: middle, -1: buckets next), mid, host leaf
+        __shared__ int sScan[kWaves][2];
+        const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        const unsigned total = *nTasks;
+        for (;;) {
+                if (tid == 0) sTask = atomicAdd(head, 1u);
+                __syncthreads();
+                const unsigned t = sTask;
+                __syncthreads();
+                if (t >= total || t >= (unsigned)cap) break;
+                const SahTask task = tasks[t];
+                const int lo = task.lo, hi = task.hi;
+                if (lo < 0 || hi > n || hi - lo < 2 || task.me < 0 || task.me >= n - 1) {
+                        if (tid == 0) atomicOr(&flags[0], kFlagBadTree);
+                        continue;
+                }
+                // 1. bounds and centroid bounds
+                float r[12];
+                for (int q = 0; q < 3; ++q) {
+                        r[q] = INFINITY;
+                        r[3 + q] = -INFINITY;
+                        r[6 + q] = INFINITY;
+                        r[9 + q] = -INFINITY;
+                }
+                for (int pidx = lo + tid; pidx < hi; pidx += kBuildBlock) {
+                        const uint32_t k = order[pidx];
+                        if (k >= (uint32_t)n) continue;
+                        const Box b = primBox[k];
+                        for (int q = 0; q < 3; ++q) {
+                                r[q] = fminf(r[q], b.lo[q]);
+                                r[3 + q] = fmaxf(r[3 + q], b.hi[q]);
+                                const float c = sah_center(b, q);
+                                r[6 + q] = fminf(r[6 + q], c);
+                                r[9 + q] = fmaxf(r[9 + q], c);
+                        }
+                }
+                for (int q = 0; q < 12; ++q) {
+                        const bool isMin = (q % 6) < 3;
+                        for (int off = 32; off >= 1; off >>= 1) {
+                                const float o = __shfl_xor(r[q], off);
+                                r[q] = isMin ? fminf(r[q], o) : fmaxf(r[q], o);
+                        }
+                }
+                if (lane == 0)
+                        for (int q = 0; q < 12; ++q) sRed[wave][q] = r[q];
+                if (tid < kSahBuckets) {
+                        sCnt[tid] = 0;
+                        for (int q = 0; q < 3; ++q) {
+                                sBB[tid][q] = fkey(INFINITY);
+                                sBB[tid][3 + q] = fkey(-INFINITY);
+                        }
+                }
+                __syncthreads();
+                if (tid == 0) {
+                        for (int q = 0; q < 12; ++q) {
+                                const bool isMin = (q % 6) < 3;
+                                float v = sRed[0][q];
+                                for (int w = 1; w < kWaves; ++w) v = isMin ? fminf(v, sRed[w][q]) : fmaxf(v, sRed[w][q]);
+                                sB[q] = v;
+                        }
+                        B3 bounds, cb;
+                        for (int q = 0; q < 3; ++q) {
+                                bounds.mn[q] = sB[q];
+                                bounds.mx[q] = sB[3 + q];
+                                cb.mn[q] = sB[6 + q];
+                                cb.mx[q] = sB[9 + q];
+                        }
+                        const int dim = b3_max_extent(cb);
+                        sPlan[0] = dim;
+                        // the host makes a leaf of an empty-area node or one with flat centroids
+                        const bool hostLeaf = task.leaf || b3_area(bounds) == 0.0f || cb.mx[dim] == cb.mn[dim] ||
+                                              !(cb.mx[dim] > cb.mn[dim]);
+                        sPlan[1] = (forceMid || hostLeaf) ? -2 : -1;  // -2: split at the middle, -1: buckets next
+                        sPlan[3] = hostLeaf ? 1 : 0;
+                }
+                __syncthreads();
+                const int dim = sPlan[0];
+                B3 cb;
+                for (int q = 0; q < 3; ++q) {
+                        cb.mn[q] = sB[6 + q];
+                        cb.mx[q] = sB[9 + q];
+                }
+                const int leafHere = sPlan[3];
+                int best = -2;
+                if (sPlan[1] == -1) {
+                        // 2. buckets
+                        for (int pidx = lo + tid; pidx < hi; pidx += kBuildBlock) {
+                                const uint32_t k = order[pidx];
+                                if (k >= (uint32_t)n) continue;
+                                const Box b = primBox[k];
+                                const int bk = sah_bucket(cb, sah_center(b, dim), dim);
+                                atomicAdd(&sCnt[bk], 1);
+                                for (int q = 0; q < 3; ++q) {
+                                        atomicMin(&sBB[bk][q], fkey(b.lo[q]));
+                                        atomicMax(&sBB[bk][3 + q], fkey(b.hi[q]));
+                                }
+                        }
+                        __syncthreads();
+                        if (tid == 0) {
+                                int cnt[kSahBuckets];
+                                B3 bb[kSahBuckets];
+                                for (int k = 0; k < kSahBuckets; ++k) {
+                                        cnt[k] = sCnt[k];
+                                        for (int q = 0; q < 3; ++q) {
+                                                bb[k].mn[q] = fval(sBB[k][q]);
+                                                bb[k].mx[q] = fval(sBB[k][3 + q]);
+                                        }
+                                }
+                                B3 bounds;
+                                for (int q = 0; q < 3; ++q) {
+                                        bounds.mn[q] = sB[q];
+                                        bounds.mx[q] = sB[3 + q];
+                                }
+                                int bst = -1;
+                                sah_decide(cnt, bb, hi - lo, bounds, bst);  // more than kSahSmall (> 4): a split
+                                int left = 0;
+                                for (int k = 0; k <= bst && k < kSahBuckets; ++k) left += cnt[k];
+                                sPlan[1] = (bst < 0 || left == 0 || left == hi - lo) ? -2 : bst;
+                                sPlan[2] = lo + left;
+                        }
+                        __syncthreads();
+                        best = sPlan[1];
+                }
+                int mid = lo + (hi - lo) / 2;
+                if (best >= 0) {
+                        mid = sPlan[2];
+                        // 3. stable partition into tmp, then back into order
+                        int offL = lo, offR = mid;
+                        for (int base = lo; base < hi; base += kBuildBlock) {
+                                const int pidx = base + tid;
+                                uint32_t k = 0;
+                                bool valid = pidx < hi, goLeft = false;
+                                if (valid) {
+                                        k = order[pidx];
+                                        goLeft = k < (uint32_t)n && sah_bucket(cb, sah_center(primBox[k], dim), dim) <= best;
+                                }
+                                const uint64_t mL = __ballot(valid && goLeft), mR = __ballot(valid && !goLeft);
+                                if (lane == 0) {
+                                        sScan[wave][0] = __popcll(mL);
+                                        sScan[wave][1] = __popcll(mR);
+                                }
+                                __syncthreads();
+                                int preL = 0, preR = 0, totL = 0, totR = 0;
+                                for (int w = 0; w < kWaves; ++w) {
+                                        if (w < wave) {
+                                                preL += sScan[w][0];
+                                                preR += sScan[w][1];
+                                        }
+                                        totL += sScan[w][0];
+                                        totR += sScan[w][1];
+                                }
+                                if (valid) {
+                                        const int dst = goLeft ? offL + preL + __popcll(mL & below)
+                                                               : offR + preR + __popcll(mR & below);
+                                        if (dst >= lo && dst < hi)
+                                                tmp[dst] = k;
+                                        else
+                                                atomicOr(&flags[0], kFlagBadTree);
+                                }
+                                offL += totL;
+                                offR += totR;
+                                __syncthreads();
+                        }
+                        if (tid == 0 && (offL != mid || offR != hi)) atomicOr(&flags[0], kFlagBadTree);
+                        __threadfence_block();
+                        __syncthreads();
+                        for (int pidx = lo + tid; pidx < hi; pidx += kBuildBlock) order[pidx] = tmp[pidx];
+                        __syncthreads();
+                }
+                // 4. the node and its children
+                if (tid == 0)
+                        sah_record(lo, mid, hi, task.me, leafHere, node, leafMark, parent, leafParent, parents, leafParents,
+                                   nextBig, nNextBig, small, nSmall, cap, flags);
+                __syncthreads();
+        }
+}
+
+// Nodes of at most kSahSmall primitives, one lane each, depth first on a private stack; the
+// host builder's steps, with its Hoare partition and its two-primitive sort.
+__global__ __launch_bounds__(kBuildBlock) void k_sah_small(const Box *__restrict__ primBox, uint32_t *__restrict__ order,
+                                                           const SahTask *__restrict__ tasks, const unsigned *nTasks,
+                                                           int cap, int4 *__restrict__ node, int *__restrict__ leafMark,
+                                                           int *__restrict__ parent,
+                                                           int *__restrict__ leafParent, int *__restrict__ parents,
+                                                           int *__restrict__ leafParents, int n, int *__restrict__ flags) {
+        const unsigned t = blockIdx.x * kBuildBlock + threadIdx.x;
+        if (t >= *nTasks || t >= (unsigned)cap) return;
+        const SahTask root = tasks[t];
+        if (root.lo < 0 || root.hi > n || root.hi - root.lo < 2 || root.hi - root.lo > kSahSmall || root.me < 0 ||
+            root.me >= n - 1) {
+                atomicOr(&flags[0], kFlagBadTree);
+                return;
+        }
+        int sLo[kSahSmall], sHi[kSahSmall], sMe[kSahSmall], sLeaf[kSahSmall];
+        int sp = 0;
+        sLo[0] = root.lo;
+        sHi[0] = root.hi;
+        sMe[0] = root.me;
+        sLeaf[0] = root.leaf;
+        sp = 1;
+        for (int guard = 0; sp > 0; ++guard) {
+                if (guard > 2 * kSahSmall) {
+                        atomicOr(&flags[0], kFlagBadTree);
+                        return;
+                }
+                --sp;
+                const int lo = sLo[sp], hi = sHi[sp], me = sMe[sp];
+                int leaf = sLeaf[sp];
+                const int count = hi - lo;
+                B3 bounds = b3_empty(), cb = b3_empty();
+                for (int p = lo; p < hi; ++p) {
+                        const Box b = primBox[order[p] < (uint32_t)n ? order[p] : 0];
+                        for (int q = 0; q < 3; ++q) {
+                                bounds.mn[q] = fminf(bounds.mn[q], b.lo[q]);
+                                bounds.mx[q] = fmaxf(bounds.mx[q], b.hi[q]);
+                                const float c = sah_center(b, q);
+                                cb.mn[q] = fminf(cb.mn[q], c);
+                                cb.mx[q] = fmaxf(cb.mx[q], c);
+                        }
+                }
+                const int dim = b3_max_extent(cb);
+                int mid = lo + count / 2;
+                if (!leaf && (b3_area(bounds) == 0.0f || cb.mx[dim] == cb.mn[dim] || !(cb.mx[dim] > cb.mn[dim])))
+                        leaf = 1;  // the host's leaf of an empty-area node or flat centroids
+                if (!leaf) {
+                        if (count <= 2) {
+                                const uint32_t a = order[lo], b = order[lo + 1];
+                                if (a < (uint32_t)n && b < (uint32_t)n &&
+                                    sah_center(primBox[b], dim) < sah_center(primBox[a], dim)) {
+                                        order[lo] = b;
+                                        order[lo + 1] = a;
+                                }
+                                mid = lo + 1;
+                        } else {
+                                int cnt[kSahBuckets];
+                                B3 bb[kSahBuckets];
+                                for (int k = 0; k < kSahBuckets; ++k) {
+                                        cnt[k] = 0;
+                                        bb[k] = b3_empty();
+                                }
+                                for (int p = lo; p < hi; ++p) {
+                                        const Box b = primBox[order[p] < (uint32_t)n ? order[p] : 0];
+                                        const int bk = sah_bucket(cb, sah_center(b, dim), dim);
+                                        cnt[bk] += 1;
+                                        for (int q = 0; q < 3; ++q) {
+                                                bb[bk].mn[q] = fminf(bb[bk].mn[q], b.lo[q]);
+                                                bb[bk].mx[q] = fmaxf(bb[bk].mx[q], b.hi[q]);
+                                        }
+                                }
+                                int best = -1;
+                                if (!sah_decide(cnt, bb, count, bounds, best)) {
+                                        leaf = 1;  // the host's leaf of at most four primitives
+                                } else if (best >= 0) {
+                                        // Hoare partition: bucket > best to the second half
+                                        int a = lo, z = hi;
+                                        for (int step = 0; step <= count; ++step) {
+                                                while (a < z) {
+                                                        const uint32_t k = order[a];
+                                                        if (sah_bucket(cb, sah_center(primBox[k < (uint32_t)n ? k : 0], dim), dim) > best)
+                                                                break;
+                                                        ++a;
+                                                }
+                                                if (a == z) break;
+                                                --z;
+                                                while (a < z) {
+                                                        const uint32_t k = order[z];
+                                                        if (!(sah_bucket(cb, sah_center(primBox[k < (uint32_t)n ? k : 0], dim), dim) > best))
+                                                                break;
+                                                        --z;
+                                                }
+                                                if (a == z) break;
+                                                const uint32_t tswap = order[a];
+                                                order[a] = order[z];
+                                                order[z] = tswap;
+                                                ++a;
+                                        }
+                                        if (a > lo && a < hi) mid = a;
+                                }
+                        }
+                }
+                const int g = mid - 1;
+                const int left = lo == g ? ~g : g;
+                const int right = hi - 1 == mid ? ~mid : mid;
+                node[me] = make_int4(lo, hi - 1, left, right);
+                leafMark[me] = leaf;
+                const int cl[2] = {lo, mid}, ch[2] = {mid, hi}, ci[2] = {left, right};
+                for (int side = 0; side < 2; ++side) {
+                        const int c = ci[side];
+                        if (c < 0) {
+                                leafParent[~c] = me;
+                                atomicAdd(&leafParents[~c], 1);
+                                continue;
+                        }
+                        parent[c] = me;
+                        atomicAdd(&parents[c], 1);
+                        if (sp >= kSahSmall) {
+                                atomicOr(&flags[0], kFlagBadTree);
+                                return;
+                        }
+                        sLo[sp] = cl[side];
+                        sHi[sp] = ch[side];
+                        sMe[sp] = c;
+                        sLeaf[sp] = leaf;
+                        ++sp;
+                }
+        }
+}
+
+__global__ void k_sah_init(SahTask *big, unsigned *counters, int n) {
+        if (threadIdx.x != 0 || blockIdx.x != 0) return;
+        big[0] = SahTask{0, n, 0, 0};  // (the root of a small scene goes to the small list)
+        counters[0] = 1;  // big tasks of the level
+        counters[1] = 0;  // head
+        counters[2] = 0;  // next level's big tasks
+        counters[3] = 0;  // small tasks
+}
+
+__global__ void k_sah_roll(unsigned *counters) {
+        if (threadIdx.x != 0 || blockIdx.x != 0) return;
+        counters[0] = counters[2];
+        counters[1] = 0;
+        counters[2] = 0;
+}
+
+__global__ void k_sah_identity(uint32_t *order, int n) {
+        const int i = (int)(blockIdx.x * kBuildBlock + threadIdx.x);
+        if (i < n) order[i] = (uint32_t)i;
+}
+
 __global__ void k_root_leaf(TriRec *tris, int n) {
         if (threadIdx.x == 0 && blockIdx.x == 0) tris[0].p0.w = __int_as_float(n);
+}
+
+// SAH build: the whole scene is one leaf when the host builder keeps the root whole.
+__global__ void k_root_leaf_marked(const int *leafMark, TriRec *tris, int n) {
+        if (threadIdx.x == 0 && blockIdx.x == 0 && (n == 1 || leafMark[0] != 0)) tris[0].p0.w = __int_as_float(n);
 }
 
 template <class T>
@@ -495,10 +990,18 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
                                     const GzBuildOptions *options) {
         clear_error();
         const int maxLeaf = options ? options->maxLeaf : 4;
-        if (nPrims < 0 || nMeshes < 0 || nMaterials < 0 || maxLeaf < 1 || maxLeaf > 64) {
-                set_error("gz_gpu_build_scene: negative count or maxLeaf %d outside 1..64", maxLeaf);
+        const int sah = options ? options->sah : 0;
+        if (nPrims < 0 || nMeshes < 0 || nMaterials < 0 || maxLeaf < 1 || maxLeaf > 64 || (sah != 0 && sah != 1)) {
+                set_error("gz_gpu_build_scene: negative count, maxLeaf %d outside 1..64 or sah %d not 0 or 1", maxLeaf,
+                          sah);
                 return nullptr;
         }
+        if (options)
+                for (int k = 0; k < 6; ++k)
+                        if (options->reserved[k] != 0) {
+                                set_error("gz_gpu_build_scene: reserved option fields must be zero");
+                                return nullptr;
+                        }
         if ((nPrims > 0 && !prims) || (nMeshes > 0 && (!meshes || !ptsCounts || !vtxCounts)) ||
             (nMaterials > 0 && !materials)) {
                 set_error("gz_gpu_build_scene: null array with a non-zero count");
@@ -538,6 +1041,9 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
         uint32_t *vals = nullptr, *vals2 = nullptr;
         int4 *node = nullptr;
         PrimId_C *sortedPrims = nullptr;
+        SahTask *bigA = nullptr, *bigB = nullptr, *smallTasks = nullptr;
+        unsigned *sahCounters = nullptr;
+        int *leafMark = nullptr;
         const int m = std::max(n - 1, 1);
         const int sortBlocks = (n + kSortBlock - 1) / kSortBlock;
         bool ok = dalloc(&verts, 3 * (size_t)n, "build vertices") && dalloc(&cent, n, "build centroids") &&
@@ -549,18 +1055,24 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
                   dalloc(&leafParents, n, "build parent counts") && dalloc(&keys, n, "build keys") &&
                   dalloc(&keys2, n, "build keys") && dalloc(&vals, n, "build order") && dalloc(&vals2, n, "build order") &&
                   dalloc(&node, m, "build nodes") && dalloc(&sortedPrims, n, "build primIds") &&
-                  dalloc(&s->dTris, n, "build triangle records") && dalloc(&s->dNodes, m, "build node records");
+                  dalloc(&s->dTris, n, "build triangle records") && dalloc(&s->dNodes, m, "build node records") &&
+                  (!sah || (dalloc(&bigA, n, "build SAH tasks") && dalloc(&bigB, n, "build SAH tasks") &&
+                            dalloc(&smallTasks, n, "build SAH tasks") && dalloc(&sahCounters, 4, "build SAH counters") &&
+                            dalloc(&leafMark, m, "build SAH leaves")));
         int h[4] = {0, 0, 0, 0};
         if (ok) {
                 const unsigned gb = (unsigned)((n + kBuildBlock - 1) / kBuildBlock);
                 hipLaunchKernelGGL(k_build_init, dim3(1), dim3(64), 0, st, cb, flags);
                 hipLaunchKernelGGL(k_gather, dim3(gb), dim3(kBuildBlock), 0, st, s->dPrims, n, s->dMeshes, nMeshes, verts,
                                    primBox, cent, cb, flags);
-                hipLaunchKernelGGL(k_morton, dim3(gb), dim3(kBuildBlock), 0, st, cent, n, cb, keys, vals);
+                if (sah)  // the SAH build reorders the identity in place
+                        hipLaunchKernelGGL(k_sah_identity, dim3(gb), dim3(kBuildBlock), 0, st, vals, n);
+                else
+                        hipLaunchKernelGGL(k_morton, dim3(gb), dim3(kBuildBlock), 0, st, cent, n, cb, keys, vals);
                 // 8 stable passes of 8 bits; an even number of passes leaves the result in keys, vals
                 uint64_t *kin = keys, *kout = keys2;
                 uint32_t *vin = vals, *vout = vals2;
-                for (int shift = 0; shift < 64; shift += 8) {
+                for (int shift = 0; !sah && shift < 64; shift += 8) {
                         hipLaunchKernelGGL(k_digit_counts, dim3((unsigned)sortBlocks), dim3(kSortBlock), 0, st, kin, n,
                                            shift, counts, sortBlocks);
                         hipLaunchKernelGGL(k_digit_scan, dim3(1), dim3(1024), 0, st, counts, 256 * sortBlocks);
@@ -572,26 +1084,73 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
                 ok = hip_ok(hipGetLastError(), "build launch");
         }
         uint32_t *order = vals;  // sorted primitive indices (leaf order)
+        if (ok && n > 1) {
+                const unsigned gb = (unsigned)((n + kBuildBlock - 1) / kBuildBlock);
+                ok = hip_ok(hipMemsetAsync(arrivals, 0, sizeof(unsigned) * (size_t)m, st), "build counters") &&
+                     hip_ok(hipMemsetAsync(parents, 0, sizeof(int) * (size_t)m, st), "build counters") &&
+                     hip_ok(hipMemsetAsync(leafParents, 0, sizeof(int) * (size_t)n, st), "build counters");
+                if (ok && sah) {
+                        // top-down binned SAH: big nodes level by level, then the small ones
+                        hipLaunchKernelGGL(k_sah_init, dim3(1), dim3(64), 0, st, n > kSahSmall ? bigA : smallTasks,
+                                           sahCounters, n);
+                        unsigned hc[4] = {n > kSahSmall ? 1u : 0u, 0u, 0u, n > kSahSmall ? 0u : 1u};
+                        if (n <= kSahSmall)
+                                ok = hip_ok(hipMemcpyAsync(sahCounters, hc, sizeof hc, hipMemcpyHostToDevice, st),
+                                            "build SAH counters");
+                        int perCU = 0;
+                        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_sah_level, kBuildBlock, 0) !=
+                                    hipSuccess ||
+                            perCU < 1)
+                                perCU = 1;
+                        // a level halves no node in the worst case but always splits it: the level count is
+                        // bounded by n; past 4096 levels every node is split at its middle
+                        for (int level = 0; ok && hc[0] > 0 && level < n; ++level) {
+                                const unsigned grid = (unsigned)std::min<int64_t>((int64_t)hc[0],
+                                                                                  (int64_t)s->cuCount * perCU);
+                                hipLaunchKernelGGL(k_sah_level, dim3(grid), dim3(kBuildBlock), 0, st, primBox, order, vals2,
+                                                   bigA, sahCounters, sahCounters + 1, bigB, sahCounters + 2, smallTasks,
+                                                   sahCounters + 3, n, node, leafMark, parent, leafParent, parents,
+                                                   leafParents, n, level >= 4096 ? 1 : 0, flags);
+                                hipLaunchKernelGGL(k_sah_roll, dim3(1), dim3(64), 0, st, sahCounters);
+                                ok = hip_ok(hipGetLastError(), "build SAH level") &&
+                                     hip_ok(hipMemcpyAsync(hc, sahCounters, sizeof hc, hipMemcpyDeviceToHost, st),
+                                            "build SAH counters") &&
+                                     hip_ok(hipStreamSynchronize(st), "build SAH level");
+                                std::swap(bigA, bigB);
+                        }
+                        if (ok && hc[0] > 0) {
+                                set_error("gz_gpu_build_scene: the SAH build did not finish");
+                                ok = false;
+                        }
+                        if (ok && hc[3] > 0)
+                                hipLaunchKernelGGL(k_sah_small, dim3((hc[3] + kBuildBlock - 1) / kBuildBlock),
+                                                   dim3(kBuildBlock), 0, st, primBox, order, smallTasks, sahCounters + 3, n,
+                                                   node, leafMark, parent, leafParent, parents, leafParents, n, flags);
+                } else if (ok) {
+                        const unsigned gi = (unsigned)((n - 1 + kBuildBlock - 1) / kBuildBlock);
+                        hipLaunchKernelGGL(k_hierarchy, dim3(gi), dim3(kBuildBlock), 0, st, keys, n, node, parent,
+                                           leafParent, parents, leafParents);
+                }
+        }
         if (ok) {
                 const unsigned gb = (unsigned)((n + kBuildBlock - 1) / kBuildBlock);
                 hipLaunchKernelGGL(k_sorted, dim3(gb), dim3(kBuildBlock), 0, st, order, verts, s->dPrims, n, s->dTris,
                                    sortedPrims, flags);
                 if (n > 1) {
                         const unsigned gi = (unsigned)((n - 1 + kBuildBlock - 1) / kBuildBlock);
-                        ok = hip_ok(hipMemsetAsync(arrivals, 0, sizeof(unsigned) * (size_t)m, st), "build counters") &&
-                             hip_ok(hipMemsetAsync(parents, 0, sizeof(int) * (size_t)m, st), "build counters") &&
-                             hip_ok(hipMemsetAsync(leafParents, 0, sizeof(int) * (size_t)n, st), "build counters");
-                        hipLaunchKernelGGL(k_hierarchy, dim3(gi), dim3(kBuildBlock), 0, st, keys, n, node, parent,
-                                           leafParent, parents, leafParents);
                         hipLaunchKernelGGL(k_validate, dim3(gb), dim3(kBuildBlock), 0, st, node, n, parents, leafParents,
                                            flags);
                         hipLaunchKernelGGL(k_refit, dim3(gb), dim3(kBuildBlock), 0, st, node, parent, leafParent, order,
                                            primBox, nodeBox, arrivals, n, flags);
-                        if (n > maxLeaf)
+                        if (sah || n > maxLeaf)
                                 hipLaunchKernelGGL(k_emit, dim3(gi), dim3(kBuildBlock), 0, st, node, parent, nodeBox,
-                                                   primBox, order, n, maxLeaf, s->dNodes, s->dTris, flags);
+                                                   primBox, order, n, maxLeaf, sah ? leafMark : nullptr, s->dNodes,
+                                                   s->dTris, flags);
                 }
-                if (n <= maxLeaf) hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(64), 0, st, s->dTris, n);
+                if (sah)
+                        hipLaunchKernelGGL(k_root_leaf_marked, dim3(1), dim3(64), 0, st, leafMark, s->dTris, n);
+                else if (n <= maxLeaf)
+                        hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(64), 0, st, s->dTris, n);
                 ok = ok && hip_ok(hipGetLastError(), "build launch") &&
                      hip_ok(hipMemcpyAsync(h, flags, sizeof h, hipMemcpyDeviceToHost, st), "build flags") &&
                      hip_ok(hipStreamSynchronize(st), "build");
@@ -611,10 +1170,12 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
                 sortedPrims = nullptr;
                 s->dBuildOrder = order;
                 vals = nullptr;
-                s->nodeRecords = n > maxLeaf ? m : 0;
+                // a root leaf: no pair record was emitted
+                const bool rootLeaf = sah ? h[1] == 0 : n <= maxLeaf;
+                s->nodeRecords = rootLeaf ? 0 : m;
                 s->triCount = n;
                 s->boundsSafe = !(h[0] & (kFlagUnsafe | kFlagSkip));  // finite boxes within 2^40, none empty
-                if (n <= maxLeaf) {
+                if (rootLeaf) {
                         s->rootDesc = ~0;
                         s->interiorCount = 0;
                         s->maxDepth = 0;
@@ -627,8 +1188,9 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
                 }
                 s->deviceBytes += (int64_t)(sizeof(TriRec) * (size_t)n + sizeof(NodePair) * (size_t)m);
         }
-        void *tmp[] = {verts, cent, primBox, nodeBox, cb, arrivals, counts, flags, parent, leafParent, parents,
-                       leafParents, keys, keys2, vals, vals2, node, sortedPrims};
+        void *tmp[] = {verts,       cent,    primBox, nodeBox, cb,   arrivals, counts,      flags,
+                       parent,      leafParent, parents, leafParents, keys, keys2, vals, vals2,
+                       node,        sortedPrims, bigA, bigB, smallTasks, sahCounters, leafMark};
         for (void *p : tmp)
                 if (p) (void)hipFree(p);
         if (!ok) {

@@ -598,10 +598,16 @@ int gz_ply_parse(const uint8_t *data, int64_t size, struct GzPlyInfo *info, floa
 /* Accelerators/BoundingHierarchyBuilder.swift).                                              */
 /* ---------------------------------------------------------------------------------------- */
 
-/* maxLeaf: subtrees of at most this many primitives become one leaf (1..64, default 4). */
+/* maxLeaf: subtrees of at most this many primitives become one leaf (1..64, default 4).
+ * sah: 0 = linear BVH (Morton order, Karras hierarchy); 1 = top-down binned SAH with the host
+ * builder's decisions (12 buckets on the longest centroid axis, BoundingHierarchyBuilder.swift:
+ * 199-339): the nodes hold the same primitive sets as gz_bvh2_build's, except that a node the host
+ * keeps as a leaf of several primitives is one leaf only if it has at most maxLeaf.
+ * reserved: zero. */
 struct GzBuildOptions {
         int32_t maxLeaf;
-        int32_t reserved[7];
+        int32_t sah;
+        int32_t reserved[6];
 };
 
 /* gz_gpu_upload_scene without the BVH: uploads the primIds (any order), meshes and materials and

@@ -109,11 +109,12 @@ def _same_records(got, ref, what):
     return len(bad) == 0
 
 
+@pytest.mark.parametrize("sah", [False, True])
 @pytest.mark.parametrize("max_leaf", [1, 2, 4, 8])
-def test_device_build_traversal_matches_oracle_on_the_exported_tree(gpu, max_leaf):
+def test_device_build_traversal_matches_oracle_on_the_exported_tree(gpu, max_leaf, sah):
     sc, prims = _soup_scene(60000)
-    _step(f"device build, 60000 triangles, max_leaf {max_leaf}")
-    bh = build_bounding_hierarchy_gpu(sc, prims, max_leaf=max_leaf)
+    _step(f"device build, 60000 triangles, max_leaf {max_leaf}, sah {sah}")
+    bh = build_bounding_hierarchy_gpu(sc, prims, max_leaf=max_leaf, sah=sah)
     _step(f"built: {len(bh.nodes)} nodes; device traversal of 8192 rays")
     rays, tmax = _rays(8192, 0.0, 1.0, 5)
     got = bh.intersect_batch_gpu(rays, tmax)
@@ -132,15 +133,63 @@ def test_device_build_traversal_matches_oracle_on_the_exported_tree(gpu, max_lea
     assert ok_upload and ok_dev
     assert np.array_equal(bh.occluded(rays, tmax), ref["hit"].astype(bool))
     info = bh.info()
-    depth = _check_tree(bh, max_leaf)
+    # the SAH build puts leaves where the host builder does (at most four primitives here)
+    depth = _check_tree(bh, 4 if sah else max_leaf)
     assert info.maxDepth == depth
     assert ref["hit"].mean() > 0.02  # tiny soup triangles: a few percent of the rays hit
     bh.free()
 
 
-def test_device_build_equals_host_build_distances(gpu):
+def _node_sets(bh, min_size):
+    """Frozensets of (id1, id2) of the primitives under every node of a BVH2 array with more than
+    min_size primitives."""
+    nodes, ids = bh.nodes, bh.prim_ids
+    keys = list(zip(ids["id1"].tolist(), ids["id2"].tolist()))
+    out, memo = [], {}
+
+    def under(i):
+        if i in memo:
+            return memo[i]
+        nd = nodes[i]
+        if nd["count"] > 0:
+            r = keys[nd["offset"]:nd["offset"] + nd["count"]]
+        else:
+            r = under(i + 1) + under(int(nd["offset"]))
+        memo[i] = r
+        return r
+
+    import sys
+    sys.setrecursionlimit(100000)
+    for i in range(len(nodes)):
+        u = under(i)
+        if len(u) > min_size:
+            out.append(frozenset(u))
+    return sorted(out, key=lambda f: (len(f), min(f)))
+
+
+def test_device_sah_build_has_the_host_builds_nodes(gpu):
+    """The device SAH build makes the host builder's decisions: every node, leaves included,
+    holds the same primitive set in both trees, so the two trees have the same node boxes and SAH
+    cost (only the order of the primitives inside a leaf and of the nodes in the array differ)."""
+    sc, prims = _soup_scene(30000, seed=12)
+    _step("device SAH build and host SAH build, 30000 triangles")
+    dev = build_bounding_hierarchy_gpu(sc, prims, sah=True)
+    host = build_bounding_hierarchy(sc, prims)
+    _step("node sets")
+    a, b = _node_sets(dev, 0), _node_sets(host, 0)
+    print(f"nodes: device {len(a)}, host {len(b)}", flush=True)
+    assert len(dev.nodes) == len(host.nodes)
+    assert len(a) == len(b) and a == b
+    c_dev, c_host = sah_cost(dev.nodes), sah_cost(host.nodes)
+    print(f"SAH cost: device {c_dev:.4f}, host {c_host:.4f}", flush=True)
+    assert abs(c_dev - c_host) <= 1e-4 * c_host
+    dev.free()
+
+
+@pytest.mark.parametrize("sah", [False, True])
+def test_device_build_equals_host_build_distances(gpu, sah):
     sc, prims = _soup_scene(50000, seed=4)
-    dev = build_bounding_hierarchy_gpu(sc, prims)
+    dev = build_bounding_hierarchy_gpu(sc, prims, sah=sah)
     host = build_bounding_hierarchy(sc, prims)
     host.upload_to_gpu()
     rays, tmax = _rays(16384, 0.0, 1.0, 6)
@@ -153,7 +202,7 @@ def test_device_build_equals_host_build_distances(gpu):
     assert same.mean() > 0.999
     c_dev, c_host = sah_cost(dev.nodes), sah_cost(host.nodes)
     print(f"SAH cost: device build {c_dev:.2f}, host SAH build {c_host:.2f}", flush=True)
-    assert c_dev < 2.0 * c_host
+    assert c_dev < (1.0001 if sah else 2.0) * c_host
     dev.free()
     host.free()
 
@@ -202,7 +251,7 @@ def _sphere_scene():
     return Scene([mesh, light], mats, light_emission=np.full((2, 3), 5.0, np.float32)), prims
 
 
-def _raw_build(prim_ids, meshes, max_leaf=4):
+def _raw_build(prim_ids, meshes, max_leaf=4, sah=0, reserved=0):
     lib = abi.load()
     meshes_c = (abi.TriangleMeshC * len(meshes))()
     for i, m in enumerate(meshes):
@@ -211,20 +260,23 @@ def _raw_build(prim_ids, meshes, max_leaf=4):
     faces = np.array([m.face_indices.size for m in meshes], np.int64)
     vtx = np.array([m.vertex_indices.size for m in meshes], np.int64)
     mats = np.zeros(1, abi.MATERIAL)
-    opt = abi.GzBuildOptions(max_leaf)
+    opt = abi.GzBuildOptions(max_leaf, sah)
+    opt.reserved[2] = reserved
     return lib.gz_gpu_build_scene(abi.ptr(prim_ids) if len(prim_ids) else None, len(prim_ids),
                                   ctypes.addressof(meshes_c), len(meshes), abi.ptr(pts), abi.ptr(faces), abi.ptr(vtx),
                                   abi.ptr(mats), 1, ctypes.byref(opt))
 
 
-def test_device_build_edge_cases(gpu):
+@pytest.mark.parametrize("sah", [False, True])
+def test_device_build_edge_cases(gpu, sah):
     lib = abi.load()
-    # identical triangles (equal Morton codes), a single triangle, a root leaf, skipped primitives
+    # identical triangles (equal Morton codes, flat centroids), a single triangle, a root leaf,
+    # skipped primitives
     tri = np.array([[0.2, 0.2, 0.5], [0.8, 0.2, 0.5], [0.5, 0.8, 0.5]], np.float32)
-    for count, max_leaf in ((1, 4), (3, 4), (4, 4), (5, 4), (300, 2)):
+    for count, max_leaf in ((1, 4), (3, 4), (4, 4), (5, 4), (33, 4), (300, 2)):
         mesh = TriangleMesh.from_xyz(np.tile(tri, (count, 1)), np.arange(3 * count, dtype=np.int64))
         sc = Scene([mesh], np.zeros(1, abi.MATERIAL))
-        bh = build_bounding_hierarchy_gpu(sc, Primitives.geometric(0, count, 0), max_leaf=max_leaf)
+        bh = build_bounding_hierarchy_gpu(sc, Primitives.geometric(0, count, 0), max_leaf=max_leaf, sah=sah)
         rays, tmax = _rays(512, 0.3, 0.7, 8)
         # slightly tilted: a zero direction component gives 0 x inf = NaN in that axis' slab, which
         # the reference's select-form slab test turns into a miss of every box
@@ -236,14 +288,15 @@ def test_device_build_edge_cases(gpu):
         if not got["hit"].any():
             print(f"{count} identical triangles, max_leaf {max_leaf}: no hit; nodes {bh.nodes[:8].tolist()}", flush=True)
         assert got["hit"].any(), count
-        _check_tree(bh, max_leaf)
+        # the SAH build keeps a group of flat centroids as one leaf, as the host builder does
+        _check_tree(bh, count if sah else max_leaf)
         bh.free()
     # a skipped primitive (non-triangle shape: id2 == -1) among triangles is never hit
     sc, prims = _soup_scene(1000, seed=9)
     ids = pack_prim_ids(prims)
     ids["type"][::7] = 1
     ids["id2"][::7] = -1
-    h = _raw_build(ids, sc.meshes)
+    h = _raw_build(ids, sc.meshes, sah=int(sah))
     assert h, abi.last_error()
     nodes = np.zeros(2 * len(ids), abi.BVH2_NODE)
     order = np.zeros(len(ids), np.int32)
@@ -251,7 +304,7 @@ def test_device_build_edge_cases(gpu):
     assert cnt > 0 and sorted(order.tolist()) == list(range(len(ids)))
     lib.gz_gpu_free_scene(h)
     # an empty scene misses everything
-    h = _raw_build(np.zeros(0, abi.PRIM_ID), sc.meshes)
+    h = _raw_build(np.zeros(0, abi.PRIM_ID), sc.meshes, sah=int(sah))
     assert h
     assert lib.gz_gpu_export_bvh2(h, abi.ptr(nodes), len(nodes), None) == 0
     lib.gz_gpu_free_scene(h)
@@ -269,3 +322,5 @@ def test_device_build_rejects_bad_indices(gpu):
     assert not _raw_build(bad, sc.meshes)
     assert not _raw_build(ids, sc.meshes, max_leaf=0)
     assert not _raw_build(ids, sc.meshes, max_leaf=65)
+    assert not _raw_build(ids, sc.meshes, sah=2)
+    assert not _raw_build(ids, sc.meshes, reserved=1)
