@@ -322,11 +322,12 @@ def full_path_leg(args, dev, stream, cores: int, run_cpu: bool) -> dict:
 
 
 def device_build_leg(args, host_bh, d_rays, d_tmax, n, stream, host_mrays) -> dict:
-    """The same 1M-triangle soup built on the device (gz_gpu_build_scene: Morton codes, radix
-    sort, Karras hierarchy, bottom-up boxes, leaves of <= 4) against the host SAH build: build
-    time (upload of primIds and meshes included), SAH cost of the exported tree in the host
-    builder's model, and incoherent closest-hit Mrays/s over the same rays with its own autotuned
-    launch configuration; the hit distances of both trees are compared bit for bit."""
+    """The same 1M-triangle soup built on the device against the host SAH build, two ways: the
+    linear BVH (Morton codes, radix sort, Karras hierarchy, bottom-up boxes, leaves of <= 4 or
+    <= 2) and the binned SAH with the host builder's decisions (the host's tree, built on the
+    device). Build time (upload of primIds and meshes included), SAH cost of the exported tree in
+    the host builder's model, and incoherent closest-hit Mrays/s over the same rays with its own
+    autotuned launch configuration; the hit distances of both trees are compared bit for bit."""
     import torch
 
     from gonzales_amd import scenes
@@ -335,11 +336,11 @@ def device_build_leg(args, host_bh, d_rays, d_tmax, n, stream, host_mrays) -> di
 
     scene, prims = scenes.soup_scene(args.tris, 1)
     out = {}
-    for max_leaf in (4, 2):
-        build_bounding_hierarchy_gpu(scene, prims, max_leaf).free()  # warm-up (module load, allocator)
+    for max_leaf, sah in ((4, False), (2, False), (4, True)):
+        build_bounding_hierarchy_gpu(scene, prims, max_leaf, sah).free()  # warm-up (module load, allocator)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        dbh = build_bounding_hierarchy_gpu(scene, prims, max_leaf)
+        dbh = build_bounding_hierarchy_gpu(scene, prims, max_leaf, sah)
         t_dev = time.perf_counter() - t0
         best, _ = autotune(dbh, d_rays, d_tmax, any_hit=False, sample=n, reps=2)
         d_hits = torch.empty(n * 16, dtype=torch.uint8, device=d_rays.device)
@@ -360,11 +361,13 @@ def device_build_leg(args, host_bh, d_rays, d_tmax, n, stream, host_mrays) -> di
         torch.cuda.synchronize()
         t_same = bool(torch.equal(d_hits.view(torch.int32).view(n, 4)[:, 1], h_hits.view(torch.int32).view(n, 4)[:, 1]))
         info = dbh.info()
-        out[f"max_leaf_{max_leaf}"] = {
+        key = "sah" if sah else f"max_leaf_{max_leaf}"
+        out[key] = {
             "build_s_including_upload": round(t_dev, 3), "sah_cost": round(sah_cost(dbh.nodes), 3),
             "closest_mrays_per_s": round(n / (ms * 1e-3) / 1e6, 2), "depth": info.maxDepth,
             "hit_distances_equal_host_build": t_same}
-        log(f"device build (leaves <= {max_leaf}): {t_dev:.3f}s, SAH {out[f'max_leaf_{max_leaf}']['sah_cost']}, "
+        what = "binned SAH" if sah else f"linear BVH, leaves <= {max_leaf}"
+        log(f"device build ({what}): {t_dev:.3f}s, SAH {out[key]['sah_cost']}, "
             f"{n / (ms * 1e-3) / 1e6:.1f} Mrays/s, distances equal: {t_same}")
         dbh.free()
     out["host_build"] = {"sah_cost": round(sah_cost(host_bh.nodes), 3), "closest_mrays_per_s": round(host_mrays, 2)}

@@ -369,12 +369,22 @@ __device__ __forceinline__ bool child_box(int c, const Box *nodeBox, const Box *
         return true;
 }
 
+// Whether internal node i becomes (part of) a leaf: a range of at most maxLeaf primitives, or in
+// an SAH build one the host builder keeps whole.
+__device__ __forceinline__ bool in_leaf(const int4 &nd, int i, int maxLeaf, const int *__restrict__ leafMark) {
+        return leafMark ? leafMark[i] != 0 : nd.y - nd.x + 1 <= maxLeaf;
+}
+
 // Bottom-up boxes: the lane of leaf position p climbs while it is the second child to arrive.
+// With them the number of child-pair records in each subtree (rec), which numbers the records
+// depth first in k_emit.
 __global__ __launch_bounds__(kBuildBlock) void k_refit(const int4 *__restrict__ node, const int *__restrict__ parent,
                                                        const int *__restrict__ leafParent,
                                                        const uint32_t *__restrict__ order,
                                                        const Box *__restrict__ primBox, Box *nodeBox,
-                                                       unsigned *arrivals, int n, int *__restrict__ flags) {
+                                                       unsigned *arrivals, int *rec, int maxLeaf,
+                                                       const int *__restrict__ leafMark, int n,
+                                                       int *__restrict__ flags) {
         const int p = (int)(blockIdx.x * kBuildBlock + threadIdx.x);
         if (p >= n) return;
         const int m = n - 1;
@@ -394,6 +404,8 @@ __global__ __launch_bounds__(kBuildBlock) void k_refit(const int4 *__restrict__ 
                         return;
                 }
                 nodeBox[cur] = box_union(a, b);
+                const int rl = nd.z >= 0 && nd.z < m ? rec[nd.z] : 0, rr = nd.w >= 0 && nd.w < m ? rec[nd.w] : 0;
+                rec[cur] = in_leaf(nd, cur, maxLeaf, leafMark) ? 0 : 1 + rl + rr;
                 cur = cur == 0 ? -1 : parent[cur];
         }
 }
@@ -427,14 +439,33 @@ __global__ __launch_bounds__(kBuildBlock) void k_sorted(const uint32_t *__restri
 __global__ __launch_bounds__(kBuildBlock) void k_emit(const int4 *__restrict__ node, const int *__restrict__ parent,
                                                       const Box *__restrict__ nodeBox, const Box *__restrict__ primBox,
                                                       const uint32_t *__restrict__ order, int n, int maxLeaf,
-                                                      const int *__restrict__ leafMark, NodePair *__restrict__ pairs,
-                                                      TriRec *__restrict__ tris, int *__restrict__ flags) {
+                                                      const int *__restrict__ leafMark, const int *__restrict__ rec,
+                                                      NodePair *__restrict__ pairs, TriRec *__restrict__ tris,
+                                                      int *__restrict__ flags) {
         const int i = (int)(blockIdx.x * kBuildBlock + threadIdx.x);
         const int m = n - 1;
         if (i >= m) return;
         const int4 nd = node[i];
         // a leaf: a range of at most maxLeaf primitives, or (SAH build) one the host builder keeps whole
-        if (leafMark ? leafMark[i] != 0 : nd.y - nd.x + 1 <= maxLeaf) return;
+        if (in_leaf(nd, i, maxLeaf, leafMark)) return;
+        // the record's depth-first number: one per edge from the root, plus the records of the
+        // left sibling's subtree on every edge that enters a right child
+        int pre = 0;
+        for (int c = i, steps = 0; c != 0; ++steps) {
+                const int p = c >= 0 && c < m ? parent[c] : -1;
+                if (p < 0 || p >= m || steps >= m) {
+                        atomicOr(&flags[0], kFlagBadTree);
+                        return;
+                }
+                const int4 pn = node[p];
+                pre += 1;
+                if (pn.w == c && pn.z >= 0 && pn.z < m) pre += rec[pn.z];
+                c = p;
+        }
+        if (pre >= m) {
+                atomicOr(&flags[0], kFlagBadTree);
+                return;
+        }
         int desc[2];
         Box cb[2];
         const int g = nd.z >= 0 ? nd.z : ~nd.z;  // split: left [first, g], right [g + 1, last]
@@ -456,7 +487,9 @@ __global__ __launch_bounds__(kBuildBlock) void k_emit(const int4 *__restrict__ n
                         tris[cFirst].p0.w = __int_as_float(size);
                         atomicAdd(&flags[3], 1);
                 } else {
-                        desc[side] = c;
+                        // depth-first numbers: the left child follows its parent, the right child
+                        // the left child's subtree
+                        desc[side] = side == 0 ? pre + 1 : pre + 1 + (nd.z >= 0 && nd.z < m ? rec[nd.z] : 0);
                 }
         }
         NodePair p;
@@ -464,7 +497,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_emit(const int4 *__restrict__ n
         p.b = make_float4(cb[0].hi[1], cb[0].hi[2], cb[1].lo[0], cb[1].lo[1]);
         p.c = make_float4(cb[1].lo[2], cb[1].hi[0], cb[1].hi[1], cb[1].hi[2]);
         p.d = make_int4(desc[0], desc[1], 0, 0);
-        pairs[i] = p;
+        pairs[pre] = p;
         atomicAdd(&flags[1], 1);
         int depth = 0;
         for (int c = i; c != 0; c = parent[c]) {
@@ -1043,7 +1076,7 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
         PrimId_C *sortedPrims = nullptr;
         SahTask *bigA = nullptr, *bigB = nullptr, *smallTasks = nullptr;
         unsigned *sahCounters = nullptr;
-        int *leafMark = nullptr;
+        int *leafMark = nullptr, *rec = nullptr;
         const int m = std::max(n - 1, 1);
         const int sortBlocks = (n + kSortBlock - 1) / kSortBlock;
         bool ok = dalloc(&verts, 3 * (size_t)n, "build vertices") && dalloc(&cent, n, "build centroids") &&
@@ -1054,7 +1087,8 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
                   dalloc(&leafParent, n, "build parents") && dalloc(&parents, m, "build parent counts") &&
                   dalloc(&leafParents, n, "build parent counts") && dalloc(&keys, n, "build keys") &&
                   dalloc(&keys2, n, "build keys") && dalloc(&vals, n, "build order") && dalloc(&vals2, n, "build order") &&
-                  dalloc(&node, m, "build nodes") && dalloc(&sortedPrims, n, "build primIds") &&
+                  dalloc(&node, m, "build nodes") && dalloc(&rec, m, "build subtree records") &&
+                  dalloc(&sortedPrims, n, "build primIds") &&
                   dalloc(&s->dTris, n, "build triangle records") && dalloc(&s->dNodes, m, "build node records") &&
                   (!sah || (dalloc(&bigA, n, "build SAH tasks") && dalloc(&bigB, n, "build SAH tasks") &&
                             dalloc(&smallTasks, n, "build SAH tasks") && dalloc(&sahCounters, 4, "build SAH counters") &&
@@ -1141,10 +1175,10 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
                         hipLaunchKernelGGL(k_validate, dim3(gb), dim3(kBuildBlock), 0, st, node, n, parents, leafParents,
                                            flags);
                         hipLaunchKernelGGL(k_refit, dim3(gb), dim3(kBuildBlock), 0, st, node, parent, leafParent, order,
-                                           primBox, nodeBox, arrivals, n, flags);
+                                           primBox, nodeBox, arrivals, rec, maxLeaf, sah ? leafMark : nullptr, n, flags);
                         if (sah || n > maxLeaf)
                                 hipLaunchKernelGGL(k_emit, dim3(gi), dim3(kBuildBlock), 0, st, node, parent, nodeBox,
-                                                   primBox, order, n, maxLeaf, sah ? leafMark : nullptr, s->dNodes,
+                                                   primBox, order, n, maxLeaf, sah ? leafMark : nullptr, rec, s->dNodes,
                                                    s->dTris, flags);
                 }
                 if (sah)
@@ -1172,7 +1206,7 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
                 vals = nullptr;
                 // a root leaf: no pair record was emitted
                 const bool rootLeaf = sah ? h[1] == 0 : n <= maxLeaf;
-                s->nodeRecords = rootLeaf ? 0 : m;
+                s->nodeRecords = rootLeaf ? 0 : h[1];  // records numbered depth first, densely
                 s->triCount = n;
                 s->boundsSafe = !(h[0] & (kFlagUnsafe | kFlagSkip));  // finite boxes within 2^40, none empty
                 if (rootLeaf) {
@@ -1190,7 +1224,7 @@ extern "C" void *gz_gpu_build_scene(const PrimId_C *prims, int64_t nPrims, const
         }
         void *tmp[] = {verts,       cent,    primBox, nodeBox, cb,   arrivals, counts,      flags,
                        parent,      leafParent, parents, leafParents, keys, keys2, vals, vals2,
-                       node,        sortedPrims, bigA, bigB, smallTasks, sahCounters, leafMark};
+                       node,        sortedPrims, bigA, bigB, smallTasks, sahCounters, leafMark, rec};
         for (void *p : tmp)
                 if (p) (void)hipFree(p);
         if (!ok) {
