@@ -321,11 +321,13 @@ def build_bounding_hierarchy(scene: Scene, prims: Primitives) -> BoundingHierarc
 
 
 def build_bounding_hierarchy_gpu(scene: Scene, prims: Primitives, max_leaf: int = 4,
-                                 sah: bool = False) -> BoundingHierarchy:
-    """The same hierarchy object built on the device (gz_gpu_build_scene: a Morton-order linear
-    BVH, or with sah the host builder's binned-SAH decisions made top-down on the device). The
-    handle is ready for every device entry point; the tree is also read back in the BVH2 layout
-    (gz_gpu_export_bvh2) with the PrimIds in leaf order, for host consumers."""
+                                 sah: bool = True) -> BoundingHierarchy:
+    """The same hierarchy object built on the device (gz_gpu_build_scene): with sah (the default)
+    the host builder's binned-SAH tree, its decisions made top-down on the device (max_leaf does
+    not apply: leaves are where the host builder puts them); otherwise a Morton-order linear BVH
+    with leaves of at most max_leaf primitives. The handle is ready for every device entry point;
+    the tree is also read back in the BVH2 layout (gz_gpu_export_bvh2) with the PrimIds in leaf
+    order, for host consumers."""
     lib = abi.load()
     bh = BoundingHierarchy(scene, pack_prim_ids(prims), np.zeros(0, abi.BVH2_NODE))
     bh.prepare_materials()

@@ -270,19 +270,29 @@ __global__ void k_bw_plan(const int *__restrict__ cursor, const BatchRound *__re
 // queue; false once the kind's chunks are done. Wave w takes the chunks w, w + waves, ... (no
 // atomic: a counter that every wave hits serialises). Wave-uniform.
 struct Chunk {
-        int k;
+        int k;  // the item of the current chunk (-1 before the first)
         unsigned e0, e1;
         unsigned g;  // the wave's next chunk
 };
 __device__ __forceinline__ void first_chunk(Chunk &c) {
         c.g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        c.k = -1;
+}
+// The item holding chunk g: the wave's previous item or the next one when g lies in it (a
+// round's items are few and long, and a wave's chunks come in order), else a binary search.
+__device__ __forceinline__ int chunk_item(const BatchRound &R, const unsigned *base, unsigned g, int prev) {
+        if (prev >= 0 && prev < R.nItems && g >= base[prev]) {
+                if (g < base[prev + 1]) return prev;
+                if (prev + 1 < R.nItems && g < base[prev + 2]) return prev + 1;
+        }
+        return find_item(R.nItems, (int)g, [&](int i) { return (int)base[i]; });
 }
 __device__ __forceinline__ bool next_chunk(const BatchRound &R, const unsigned *base, const unsigned *n, unsigned chunk,
                                            Chunk &c) {
         const unsigned g = (unsigned)__builtin_amdgcn_readfirstlane((int)c.g);
         c.g = g + gridDim.x * (blockDim.x >> 6);
         if (g >= base[R.nItems]) return false;
-        c.k = __builtin_amdgcn_readfirstlane(find_item(R.nItems, (int)g, [&](int i) { return (int)base[i]; }));
+        c.k = __builtin_amdgcn_readfirstlane(chunk_item(R, base, g, c.k));
         const unsigned cnt = n[c.k];
         c.e0 = (g - base[c.k]) * chunk;
         c.e1 = cnt - c.e0 < chunk ? cnt : c.e0 + chunk;
@@ -315,7 +325,8 @@ __global__ __launch_bounds__(kMisBlock) void k_bw_rays(const int *__restrict__ c
                 const unsigned g = (unsigned)__builtin_amdgcn_readfirstlane((int)ch.g);
                 ch.g = g + gridDim.x * (blockDim.x >> 6);
                 if (g >= base[R.nItems]) break;
-                const int k = __builtin_amdgcn_readfirstlane(find_item(R.nItems, (int)g, [&](int i) { return (int)base[i]; }));
+                const int k = __builtin_amdgcn_readfirstlane(chunk_item(R, base, g, ch.k));
+                ch.k = k;
                 const BatchItem it = its[k];
                 const unsigned n = (kMode == 0 && level == 0) ? (unsigned)it.count : nIn[k];
                 const unsigned e0 = (g - base[k]) * chunk;
@@ -382,8 +393,10 @@ __global__ __launch_bounds__(kMisBlock) void k_bw_split(const int *__restrict__ 
         const int wave = (int)(threadIdx.x >> 6);
         const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
         const unsigned tiles = base[R.nItems];
+        int kPrev = -1;
         for (unsigned g = blockIdx.x; g < tiles; g += gridDim.x) {
-                const int k = __builtin_amdgcn_readfirstlane(find_item(R.nItems, (int)g, [&](int i) { return (int)base[i]; }));
+                const int k = __builtin_amdgcn_readfirstlane(chunk_item(R, base, g, kPrev));
+                kPrev = k;
                 const BatchItem it = its[k];
                 const unsigned n = level == 0 ? (unsigned)it.count : nIn[k];
                 const unsigned t0 = (g - base[k]) * kSplitTile;
