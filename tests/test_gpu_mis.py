@@ -45,9 +45,11 @@ def _agree(got, ref):
     return close & same
 
 
-def _run_both(bh, paths, opt: MisOptions, sky=None, materials=None, lights=None):
+def _run_both(bh, paths, opt: MisOptions, sky=None, materials=None, lights=None, launch=None):
     bh.upload_to_gpu()
     integ = VolumePathIntegrator(bh, opt, sky=sky, materials=materials, lights=lights)
+    if launch is not None:
+        integ.set_launch(*launch)
     d = _to_device(paths)
     integ.trace(d)
     got = _to_host(d)
@@ -202,6 +204,29 @@ def test_mixed_materials_match_oracle(gpu, sky):
     paths = abi.concat([new_paths(rays, px, py, s) for s in range(spp)], abi.GZ_PATH)
     opt = MisOptions.for_image(w, h, spp, max_depth=5)
     got, ref = _run_both(bh, paths, opt, sky=sky, materials=gz)
+    assert np.any(ref["flags"] == 0)
+    _check(got, ref, min_frac=0.98)
+
+
+@pytest.mark.parametrize("queued", [1, 3])
+def test_materials_without_the_layered_bsdf_match_oracle(gpu, queued):
+    """The mixed scene with its coated conductors replaced by conductors: the bounce kernels are
+    then compiled without the layered BSDF (the set a scene without coated conductors gets), in
+    the queued megakernel and the full wavefront."""
+    from gonzales_amd.integrator import conductor_material
+    from mis_helpers import mixed_material_scene
+    bh, gz = mixed_material_scene()
+    coated = gz["type"] == abi.GZ_MAT_COATED_CONDUCTOR
+    assert coated.any()
+    gz = gz.copy()
+    gz[coated] = conductor_material(roughness=0.35)
+    cam = {"eye": (0.0, 1.4, 3.5), "look": (0.1, 0.5, -0.2), "up": (0.0, 1.0, 0.0), "fov": 35.0}
+    w = h = 32
+    spp = 8
+    rays, px, py = pinhole_rays(cam, w, h)
+    paths = abi.concat([new_paths(rays, px, py, s) for s in range(spp)], abi.GZ_PATH)
+    opt = MisOptions.for_image(w, h, spp, max_depth=5)
+    got, ref = _run_both(bh, paths, opt, sky=(0.6, 0.7, 0.8), materials=gz, launch=(queued, 256, 1, 64, 3))
     assert np.any(ref["flags"] == 0)
     _check(got, ref, min_frac=0.98)
 
