@@ -76,9 +76,13 @@ def parse():
     ap.add_argument("--full-spp", type=int, default=64)
     ap.add_argument("--full-subdiv", type=int, default=9)
     ap.add_argument("--full-frames", type=int, default=1)
+    ap.add_argument("--full-batch-paths", type=int, default=1 << 28,
+                    help="paths per launch sequence of the frame (default: the whole frame)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch leg")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=1)
+    ap.add_argument("--batch-round-paths", type=int, default=1 << 27,
+                    help="slots per round of the batch, and paths per launch sequence of the one-at-a-time loop")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the inverse-rendering adjoint leg")
     ap.add_argument("--adj-res", type=int, default=512)
     ap.add_argument("--adj-spp", type=int, default=32)
@@ -244,7 +248,7 @@ def full_path_leg(args, dev, stream, cores: int, run_cpu: bool) -> dict:
     sky = (0.4, 0.45, 0.5)
     integ = VolumePathIntegrator(bh, opt, sky=sky, materials=gz)
     film = DeviceFilm(cam, dev)
-    rnd = FrameRenderer(cam, integ, film, batch_paths=1 << 23)
+    rnd = FrameRenderer(cam, integ, film, batch_paths=args.full_batch_paths)
     n = cam.path_count
     # counting pass over a whole frame (untimed; it also warms the kernels up)
     m = rnd.batch
@@ -422,7 +426,7 @@ def batch_leg(args, dev, stream) -> dict:
     tune = SceneBatch(tune_entries, round_paths=1 << 23)
     best, timings = autotune_batch(tune, BATCH_CANDIDATES, reps=1)
     tune.close()
-    batch = SceneBatch(entries, round_paths=1 << 23)
+    batch = SceneBatch(entries, round_paths=args.batch_round_paths)
     batch.set_launch(best.block, best.grid_scale, best.chunk, best.drain_level, best.queued)
     info = batch.info()
     n = info.totalPaths
@@ -442,7 +446,8 @@ def batch_leg(args, dev, stream) -> dict:
     # the same frames, one scene at a time through the single-scene loop (queued trace)
     seq_ms, per_scene = 0.0, []
     for e in entries:
-        rnd = FrameRenderer(e.camera, e.integrator, e.film, batch_paths=1 << 23)
+        rnd = FrameRenderer(e.camera, e.integrator, e.film, batch_paths=args.batch_round_paths)
+        rnd.render(stream)  # untimed: allocates the scene's queue storage
         a.record(stream)
         rnd.render(stream)
         b.record(stream)
@@ -451,6 +456,7 @@ def batch_leg(args, dev, stream) -> dict:
         seq_ms += t
         per_scene.append(e.camera.path_count / (t * 1e-3))
         del rnd
+        e.integrator.release_queues()  # up to 170 B per path: 64 scenes' worth would not fit
     rates = np.array(per_scene)
     res = {"workload": f"{len(specs)} scenes (seed 7): {sum(s['tris'] for s in specs)} triangles in total, "
                        "log-uniform 1e4..2e6 per scene, displaced spheres and soups, 256^2/512^2/1024^2, "

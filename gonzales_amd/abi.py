@@ -181,6 +181,7 @@ SIGNATURES = {
     "gz_gpu_mis_adjoint_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P]),
     "gz_gpu_set_mis_launch": (ctypes.c_int, [_P, _P]),
     "gz_gpu_get_mis_launch": (ctypes.c_int, [_P, _P]),
+    "gz_gpu_release_mis_queues": (ctypes.c_int, [_P]),
     "gz_gpu_camera_paths_device": (ctypes.c_int, [_P, _I64, _I64, _P, _P]),
     "gz_gpu_film_add_device": (ctypes.c_int, [_P, _P, _I64, ctypes.c_float, ctypes.c_float, _P, _P, _P, _P, _P]),
     "gz_gpu_film_resolve_device": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),

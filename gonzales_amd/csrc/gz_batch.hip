@@ -468,16 +468,32 @@ __global__ __launch_bounds__(kMisBlock, mis_min_waves(kAll)) void k_bw_phase(con
         const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
         const int *qin = ws.q[kind];
         unsigned *nNext = c.n + (size_t)(level + 1) * c.maxItems;
+        // Held runs carry over from chunk to chunk while the item stays the same, so that small
+        // chunks cost no more queue atomics than large ones.
+        Keep4 keep;
+        unsigned bits[3] = {0u, 0u, 0u};
+        int runs = 0;
+        int heldK = -1, heldOff = 0;
+        auto flush = [&]() {
+                if (runs > 0) {
+                        int *const qo[3] = {qout + heldOff, ws.q[kWShadow] + heldOff, ws.q[kWBsdf] + heldOff};
+                        unsigned *const on[3] = {nNext + heldK, w.count(level, kWShadow) + heldK,
+                                                 w.count(level, kWBsdf) + heldK};
+                        flush_appends<3>(keep, bits, runs, qo, on, lane, below);
+                }
+                runs = 0;
+                bits[0] = bits[1] = bits[2] = 0u;
+        };
         Chunk ch;
         first_chunk(ch);
         while (next_chunk(R, w.bases(level, kind), w.count(level, kind), chunk, ch)) {
                 const BatchItem it = its[ch.k];
                 const MisArgs &a = args[__builtin_amdgcn_readfirstlane(it.scene)];
-                int *const qo[3] = {qout + it.offset, ws.q[kWShadow] + it.offset, ws.q[kWBsdf] + it.offset};
-                unsigned *const on[3] = {nNext + ch.k, w.count(level, kWShadow) + ch.k, w.count(level, kWBsdf) + ch.k};
-                Keep4 keep;
-                unsigned bits[3] = {0u, 0u, 0u};
-                int runs = 0;
+                if (ch.k != heldK) {
+                        flush();
+                        heldK = ch.k;
+                        heldOff = it.offset;
+                }
                 for (unsigned j0 = ch.e0; j0 < ch.e1; j0 += 64) {
                         const unsigned j = j0 + (unsigned)lane;
                         int idx = -1;
@@ -505,14 +521,10 @@ __global__ __launch_bounds__(kMisBlock, mis_min_waves(kAll)) void k_bw_phase(con
                         keep.set(runs, idx);
 #pragma unroll
                         for (int q = 0; q < 3; ++q) bits[q] |= ((mask >> q) & 1u) << runs;
-                        if (++runs == 4) {
-                                flush_appends<3>(keep, bits, runs, qo, on, lane, below);
-                                runs = 0;
-                                bits[0] = bits[1] = bits[2] = 0u;
-                        }
+                        if (++runs == 4) flush();
                 }
-                if (runs > 0) flush_appends<3>(keep, bits, runs, qo, on, lane, below);
         }
+        flush();
 }
 
 // finish_phase over one class queue of the items.

@@ -1170,6 +1170,25 @@ extern "C" int gz_gpu_get_mis_launch(void *handle, GzMisLaunch *launch) {
         return GZ_OK;
 }
 
+extern "C" int gz_gpu_release_mis_queues(void *handle) {
+        clear_error();
+        if (!handle) {
+                set_error("gz_gpu_release_mis_queues: NULL handle");
+                return GZ_E_INVALID;
+        }
+        Scene *s = static_cast<Scene *>(handle);
+        std::lock_guard<std::mutex> lock(s->mu);
+        if (!s->misQueues) return GZ_OK;
+        if (!enter_device(s)) return GZ_E_HIP;
+        // the queues may still be in use by work queued on their streams
+        for (auto &kv : s->misQueues->perStream)
+                if (!hip_ok(hipStreamSynchronize(kv.first), "gz_gpu_release_mis_queues: stream synchronize"))
+                        return GZ_E_HIP;
+        release_mis_queues(s->misQueues);
+        s->misQueues = nullptr;
+        return GZ_OK;
+}
+
 extern "C" int gz_gpu_mis_bounce_device(void *handle, GzPath *d_paths, int64_t count, const GzMisParams *params,
                                         void *stream) {
         return mis_launch(handle, d_paths, count, params, nullptr, stream, false, "gz_gpu_mis_bounce_device");

@@ -188,6 +188,11 @@ class VolumePathIntegrator:
         abi.check(abi.load().gz_gpu_get_mis_launch(self.bh.handle, ctypes.byref(c)), "gz_gpu_get_mis_launch")
         return c.queued, c.block, c.gridScale, c.chunk, c.drainLevel
 
+    def release_queues(self) -> None:
+        """Frees the path-queue storage kept on this scene handle for its largest launch; the
+        next trace allocates it again."""
+        abi.check(abi.load().gz_gpu_release_mis_queues(self.bh.handle), "gz_gpu_release_mis_queues")
+
     def trace(self, paths_t, count: int | None = None, stream=None) -> None:
         """All bounces of every active path (maxDepth + 1 levels), in place."""
         n = int(paths_t.numel() * paths_t.element_size() // abi.GZ_PATH.itemsize) if count is None else count

@@ -389,6 +389,13 @@ struct GzMisLaunch {
 int gz_gpu_set_mis_launch(void *handle, const struct GzMisLaunch *launch);
 int gz_gpu_get_mis_launch(void *handle, struct GzMisLaunch *launch);
 
+/* Frees the path-queue storage the MIS entry points keep per handle and stream (about 170 bytes
+ * per path of the largest launch, kept so that later launches of that size allocate nothing);
+ * the next MIS launch allocates it again. Waits for the work queued on those streams first. No
+ * counterpart in the reference, which renders one scene per process: for callers that render
+ * many large scenes in turn. */
+int gz_gpu_release_mis_queues(void *handle);
+
 
 /* Traversal work of one path, summed over its bounces and all rays of each bounce (closest hit,
  * shadow ray, BSDF ray): interior-node visits, triangle tests, bounces run. */

@@ -279,8 +279,9 @@ def test_queued_launches_equal_the_full_relaunch(gpu, mixed):
 
 
 def test_queue_grows_and_stays_valid_across_sizes(gpu):
-    """The per-stream queue storage grows with the path count; a smaller trace after a larger one
-    and a trace on another stream give the same records as fresh runs."""
+    """The per-stream queue storage grows with the path count; a smaller trace after a larger one,
+    a trace on another stream and a trace after the storage is released give the same records as
+    fresh runs."""
     import torch
     bh, cam = scenes.cornell_hierarchy()
     bh.upload_to_gpu()
@@ -309,6 +310,15 @@ def test_queue_grows_and_stays_valid_across_sizes(gpu):
     got = _to_host(d_part)
     assert got[:777].tobytes() == np.frombuffer(ref_big, abi.GZ_PATH)[:777].tobytes()
     assert got[777:].tobytes() == big[777:].tobytes()
+    # released storage is allocated again by the next trace, in either queued mode
+    for launch in [(1, 256, 1, 256, 3), (3, 256, 1, 64, 3)]:
+        integ.set_launch(*launch)
+        integ.release_queues()
+        integ.release_queues()  # nothing left to free
+        d_again = _to_device(big)
+        integ.trace(d_again)
+        assert _to_host(d_again).tobytes() == ref_big, launch
+    assert abi.load().gz_gpu_release_mis_queues(None) == abi.GZ_E_INVALID
 
 
 def test_bad_launch_configurations_are_rejected(gpu):
