@@ -25,7 +25,7 @@ class BatchEntry:
 
 
 class SceneBatch:
-    def __init__(self, entries: list[BatchEntry], round_paths: int = 1 << 22):
+    def __init__(self, entries: list[BatchEntry], round_paths: int = 1 << 25):
         if not entries:
             raise ValueError("a batch needs at least one scene")
         self.entries = list(entries)

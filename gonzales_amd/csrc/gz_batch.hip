@@ -885,6 +885,10 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
                 }
         }
         close_round();
+        // slots for the largest round only: a batch smaller than roundPaths allocates what it uses
+        int maxRound = 1;
+        for (const BatchRound &r : b->hRounds) maxRound = std::max(maxRound, (int)r.total);
+        b->roundPaths = std::min(b->roundPaths, maxRound);
         b->ctr.maxItems = std::max(b->maxItems, 1);
         b->ctr.levels = b->levels;
         const size_t slots = (size_t)b->roundPaths;

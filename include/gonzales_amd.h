@@ -500,14 +500,15 @@ struct GzBatchInfo {
         int64_t totalPaths;      /* path samples of all scenes */
         int32_t rounds;          /* launch sequences one render runs */
         int32_t items;           /* (scene, whole pixels) pieces over all rounds */
-        int32_t roundPaths;      /* path slots of a round */
+        int32_t roundPaths;      /* path slots of a round: the largest round's paths (<= the roundPaths asked) */
         int32_t maxItemsPerRound;
         int32_t levels;          /* bounce levels of a round: 1 + the largest maxDepth */
         int32_t allBsdfs;        /* 1 if some scene has a non-diffuse material */
 };
 
 /* Plans the batch: the paths of all scenes (in scene order, whole pixels) are cut into rounds of
- * at most roundPaths path slots, and device tables are built for them. The batch snapshots each
+ * at most roundPaths path slots, and device tables are built for them; device state (about 280 B
+ * per slot) is allocated for the largest round only. The batch snapshots each
  * scene's MIS tables and parameters: call gz_gpu_set_mis_scene on a scene only before creating a
  * batch over it, and free the batch before its scenes. All scenes must be on one device. NULL on
  * failure, with gz_last_error() set. */

@@ -102,4 +102,11 @@ def test_batch_rejects_bad_input(gpu):
                 (1, 256, 1, 256, 65)]:
         c = abi.GzMisLaunch(*cfg)
         assert abi.load().gz_batch_set_launch(batch.handle, ctypes.byref(c)) == abi.GZ_E_INVALID
+    info = batch.info()
+    assert info.roundPaths <= 1 << 14
     batch.close()
+    # slots for the largest round only: one round holds every path of a small batch
+    big = SceneBatch(entries, round_paths=1 << 30)
+    info = big.info()
+    assert info.rounds == 1 and info.roundPaths == info.totalPaths
+    big.close()
