@@ -9,8 +9,8 @@
 * :func:`cornell_box` -- the Cornell-box geometry of the reference's Scenes/cornell-box.pbrt
   (5 wall quads, two boxes rotated about y, one emissive ceiling quad, radiance (17, 12, 4)).
 
-All generators use numpy's PCG64 bit generator with fixed seeds, so every run (here and on the
-GPU box) sees the same inputs.
+All generators use numpy's PCG64 bit generator with fixed seeds, so every run, on any machine,
+sees the same inputs.
 """
 from __future__ import annotations
 
