@@ -57,6 +57,52 @@ def measured_traffic() -> float | None:
     return None
 
 
+def cpu_host() -> dict:
+    """The host CPU the baseline runs on: logical CPUs of the machine, those this process may use,
+    the share per GPU the box grants (OMP_NUM_THREADS, which the box sets to its CPU share), and
+    the CPU model from /proc/cpuinfo. The baseline runs at share - 1 threads, the reference's rule
+    of one worker less than the cores it sees (TileRenderer.swift:77), and at share threads."""
+    nproc = os.cpu_count() or 1
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    try:
+        share = int(os.environ.get("OMP_NUM_THREADS", "") or aff)
+    except ValueError:
+        share = aff
+    share = max(1, min(share, aff))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": nproc, "affinity": aff, "share": share, "model": model,
+            "threads": [max(1, share - 1), share]}
+
+
+CPU_BUILD = {"flags": None}
+
+
+def cpu_build() -> str:
+    """Builds the oracle with -march=native on this machine once (the CPU baseline's build)."""
+    if CPU_BUILD["flags"] is None:
+        import oracle
+        CPU_BUILD["flags"] = oracle.use_native()
+        log(f"cpu baseline build: {CPU_BUILD['flags']}")
+    return CPU_BUILD["flags"]
+
+
+def cpu_baseline_record(unit: str, rates: dict, host: dict, kind: str, sample: str) -> dict:
+    """cpu_baseline object: value at share - 1 threads (the reference's worker rule), the rate at
+    all share threads beside it, the host, the build flags and the sample."""
+    n1, n = host["threads"]
+    return {"value": rates[n1], "unit": unit, "cores": n1, "kind": kind, "value_at_share_threads": rates[n],
+            "share_threads": n, "host_nproc": host["nproc"], "host_affinity": host["affinity"],
+            "cpu_model": host["model"], "build": CPU_BUILD["flags"], "sample": sample}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +136,8 @@ def parse():
     ap.add_argument("--adj-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device copy (reference dispatch for HBM byte counters)")
+    ap.add_argument("--only", choices=["path", "full", "batch", "adjoint"], default=None,
+                    help="run one of the extra legs alone and print its JSON (for profiling)")
     return ap.parse_args()
 
 
@@ -107,11 +155,86 @@ def spread_sample(cam, n_paths: int, table, block_pixels: int = 16):
     return abi.concat(parts, abi.GZ_PATH), [(j * stride * spp, block_pixels * spp) for j in range(blocks)]
 
 
+def gpu_paths_of_ranges(cam, integ, ranges, dev):
+    """Device records of the path samples in `ranges` ((first, count) of the frame's path
+    numbering): camera paths of each range into one buffer, then one trace of them all. The
+    records do not depend on the launch that traces them (tests/test_gpu_mis.py), so they equal
+    the timed frame's records of the same path samples."""
+    import torch
+
+    from gonzales_amd import abi
+    k = sum(c for _, c in ranges)
+    rec = abi.GZ_PATH.itemsize
+    d = torch.empty(k * rec, dtype=torch.uint8, device=dev)
+    off = 0
+    for first, count in ranges:
+        cam.paths_device(d[off * rec:(off + count) * rec], first, count)
+        off += count
+    integ.trace(d, k)
+    torch.cuda.synchronize()
+    integ.bh.check_traversal()
+    return d.cpu().numpy().view(abi.GZ_PATH)
+
+
+def paths_agreeing(got, ref) -> float:
+    """Fraction of paths with the same bounce count and flags and estimate within 1e-4 relative."""
+    e_g, e_r = got["estimate"].astype(np.float64), ref["estimate"].astype(np.float64)
+    ok = ((np.abs(e_g - e_r) <= 1e-4 * np.abs(e_r) + 1e-6).all(1) & (got["bounce"] == ref["bounce"])
+          & (got["flags"] == ref["flags"]))
+    return float(ok.mean())
+
+
+def path_cpu_leg(args, bh, integ, cam, sky, host: dict, dev, label: str, seconds: float) -> dict:
+    """The oracle's whole-frame chain (camera paths -> every MIS bounce, each ray through the
+    oracle's traversal -> film sums) on blocks of 16 pixels spread over the frame, timed at
+    share - 1 and share threads (about `seconds` in all), and the fraction of those path samples
+    whose device records agree with the oracle's."""
+    import ctypes
+
+    import oracle
+    from gonzales_amd import abi
+    cpu_build()
+    opt = integ.options
+    desc, keep = bh.descriptor()
+    table = oracle.sobol_matrices()
+    skyv = (0.0, 0.0, 0.0) if sky is None else tuple(sky)
+    prm = oracle.MisParams(opt.max_depth, 0, opt.zsobol_seed, opt.log2_spp, opt.base4_digits, opt.rr_seed,
+                           (ctypes.c_float * 3)(*skyv), 0 if sky is None else 1)
+    n1, nn = host["threads"]
+    calib, _ = spread_sample(cam, 1 << 13, table)
+    t1 = time.perf_counter()
+    oracle.mis_trace(ctypes.addressof(desc), len(bh.nodes), integ.materials, integ.lights, prm, calib, nn, table)
+    rate = len(calib) / (time.perf_counter() - t1)
+    start, ranges = spread_sample(cam, int(max(1 << 13, rate * seconds / 2)), table)
+    c = cam.c
+    px = c.resX * c.resY
+    rates, sample = {}, None
+    for th in sorted({n1, nn}):
+        sample = start.copy()
+        light, albedo, normal = (np.zeros(3 * px, np.float32) for _ in range(3))
+        weight = np.zeros(px, np.float32)
+        t1 = time.perf_counter()
+        oracle.mis_trace(ctypes.addressof(desc), len(bh.nodes), integ.materials, integ.lights, prm, sample, th,
+                         table)
+        oracle.film_add(c, sample, cam.film.iso, cam.film.max_component_value, light, weight, albedo, normal)
+        rates[th] = round(len(sample) / (time.perf_counter() - t1), 1)
+    rates.setdefault(n1, rates[nn])
+    rates.setdefault(nn, rates[n1])
+    agree = paths_agreeing(gpu_paths_of_ranges(cam, integ, ranges, dev), sample)
+    m = len(sample)
+    rec = cpu_baseline_record("path samples/s", rates, host, "port",
+                              f"{m} of the {cam.path_count} path samples ({len(ranges)} blocks of 16 pixels spread "
+                              f"over the {label}): camera + every bounce (the oracle's traversal per ray) + film")
+    log(f"{label} cpu oracle {rates[n1]:.0f} / {rates[nn]:.0f} path samples/s on {n1} / {nn} threads "
+        f"({m} paths), agreeing {agree:.5f}")
+    return {"cpu_baseline": rec, "cpu_sample_paths_agreeing": round(agree, 5)}
+
+
 PATH_RECORD_BYTES = 224  # GzPath read + write per bounce
 SHADING_FETCH_BYTES = 160  # PrimId 32 + mesh view 40 + 3 vertex indices 24 + 3 points 48 + material 16
 
 
-def path_leg(args, dev, stream, cores: int, run_cpu: bool) -> dict:
+def path_leg(args, dev, stream, host: dict, run_cpu: bool) -> dict:
     """Cornell box, 512 x 512 x 16 spp, max depth 5: a whole frame on the device.
 
     One step = one frame: camera paths for all 4,194,304 path samples (Z-Sobol camera sample,
@@ -140,7 +263,6 @@ def path_leg(args, dev, stream, cores: int, run_cpu: bool) -> dict:
     d_stats = torch.zeros(n * 3, dtype=torch.int32, device=dev)
     film = DeviceFilm(cam, dev)
     cam.paths_device(d_paths)
-    start = d_paths.cpu().numpy().view(abi.GZ_PATH).copy() if run_cpu else None
     integ.trace_stats(d_paths, d_stats)
     bh.check_traversal()
     st = d_stats.view(n, 3).to(torch.int64).sum(0).tolist()
@@ -185,43 +307,12 @@ def path_leg(args, dev, stream, cores: int, run_cpu: bool) -> dict:
     log(f"path leg: {res_d['path_samples_per_s'] / 1e6:.2f} M path samples/s, {ms:.2f} ms/frame, "
         f"{res_d['bounces_per_path']} bounces/path, {res_d['bytes_per_path']} B/path")
     if run_cpu:
-        import oracle
-
-        desc, keep = bh.descriptor()
-        table = oracle.sobol_matrices()
-        prm = oracle.MisParams(opt.max_depth, 0, opt.zsobol_seed, opt.log2_spp, opt.base4_digits, opt.rr_seed,
-                               (ctypes.c_float * 3)(0, 0, 0), 0)
-        calib = start[:1 << 14].copy()
-        t1 = time.perf_counter()
-        oracle.mis_trace(ctypes.addressof(desc), len(bh.nodes), integ.materials, integ.lights, prm, calib, cores, table)
-        rate = len(calib) / (time.perf_counter() - t1)
-        want = int(min(n, max(1 << 14, rate * args.cpu_seconds / 2)))
-        t1 = time.perf_counter()
-        sample, ranges = spread_sample(cam, want, table)
-        oracle.mis_trace(ctypes.addressof(desc), len(bh.nodes), integ.materials, integ.lights, prm, sample, cores,
-                         table)
-        px = res * res
-        light, albedo, normal = (np.zeros(3 * px, np.float32) for _ in range(3))
-        weight = np.zeros(px, np.float32)
-        oracle.film_add(cam.c, sample, 100.0, 0.0, light, weight, albedo, normal)
-        dt = time.perf_counter() - t1
-        m = len(sample)
-        allp = d_paths.cpu().numpy().view(abi.GZ_PATH)
-        got = abi.concat([allp[a:a + c] for a, c in ranges], abi.GZ_PATH)
-        e_g, e_r = got["estimate"].astype(np.float64), sample["estimate"].astype(np.float64)
-        agree = ((np.abs(e_g - e_r) <= 1e-4 * np.abs(e_r) + 1e-6).all(1) & (got["bounce"] == sample["bounce"]))
-        res_d["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "path samples/s", "cores": cores,
-                                 "kind": "port",
-                                 "sample": f"{m} of the {n} path samples ({len(ranges)} blocks of 16 pixels spread "
-                                           f"over the image), camera + bounces (the oracle's traversal per ray, "
-                                           f"{cores} threads) + film, {dt:.1f} s"}
-        res_d["cpu_sample_paths_agreeing"] = round(float(agree.mean()), 5)
-        log(f"path leg cpu oracle {m / dt:.0f} path samples/s on {cores} threads, agreeing {agree.mean():.5f}")
+        res_d.update(path_cpu_leg(args, bh, integ, cam, None, host, dev, "Cornell image", args.cpu_seconds / 2))
     bh.free()
     return res_d
 
 
-def full_path_leg(args, dev, stream, cores: int, run_cpu: bool) -> dict:
+def full_path_leg(args, dev, stream, host: dict, run_cpu: bool) -> dict:
     """Displaced icosphere (20 x 4^9 = 5,242,880 triangles, fBm amplitude 0.05) on a ground plane,
     diffuse / dielectric / coated-conductor patches of 64 faces, one quad area light and a
     constant sky; 1920 x 1080, 64 spp, max depth 8, RR after bounce 1. One step = one frame of
@@ -300,27 +391,7 @@ def full_path_leg(args, dev, stream, cores: int, run_cpu: bool) -> dict:
     log(f"full path leg: {res['path_samples_per_s'] / 1e6:.2f} M path samples/s, {ms:.1f} ms/frame, "
         f"{res['bounces_per_path']} bounces/path")
     if run_cpu:
-        import oracle
-
-        desc, keep = bh.descriptor()
-        table = oracle.sobol_matrices()
-        prm = oracle.MisParams(opt.max_depth, 0, opt.zsobol_seed, opt.log2_spp, opt.base4_digits, opt.rr_seed,
-                               (ctypes.c_float * 3)(*sky), 1)
-        calib, _ = spread_sample(cam, 1 << 13, table)
-        t1 = time.perf_counter()
-        oracle.mis_trace(ctypes.addressof(desc), len(bh.nodes), integ.materials, integ.lights, prm, calib, cores,
-                         table)
-        rate = len(calib) / (time.perf_counter() - t1)
-        t1 = time.perf_counter()
-        sample, ranges = spread_sample(cam, int(max(1 << 13, rate * args.cpu_seconds / 2)), table)
-        oracle.mis_trace(ctypes.addressof(desc), len(bh.nodes), integ.materials, integ.lights, prm, sample, cores,
-                         table)
-        dt = time.perf_counter() - t1
-        k = len(sample)
-        res["cpu_baseline"] = {"value": round(k / dt, 1), "unit": "path samples/s", "cores": cores, "kind": "port",
-                               "sample": f"{k} path samples ({len(ranges)} blocks of 16 pixels spread over the "
-                                         f"frame): camera + bounces, {cores} threads, {dt:.1f} s"}
-        log(f"full path leg cpu oracle {k / dt:.0f} path samples/s on {cores} threads")
+        res.update(path_cpu_leg(args, bh, integ, cam, sky, host, dev, "1080p frame", args.cpu_seconds / 2))
     bh.free()
     return res
 
@@ -378,7 +449,7 @@ def device_build_leg(args, host_bh, d_rays, d_tmax, n, stream, host_mrays) -> di
     return out
 
 
-def batch_leg(args, dev, stream) -> dict:
+def batch_leg(args, dev, stream, host: dict, run_cpu: bool) -> dict:
     """64 scenes of uneven size in one launch sequence: triangle counts log-uniform in [1e4, 2e6]
     (displaced spheres and triangle soups), resolutions 256^2 / 512^2 / 1024^2, 4 / 16 / 64 spp,
     patches of diffuse / dielectric / GGX conductor materials, an area light and a sky, max depth 6
@@ -443,8 +514,11 @@ def batch_leg(args, dev, stream) -> dict:
     ms = a.elapsed_time(b) / args.batch_steps
     for bh in hs:
         bh.check_traversal()
-    # the same frames, one scene at a time through the single-scene loop (queued trace)
-    seq_ms, per_scene = 0.0, []
+    # the same frames, one scene at a time through the single-scene loop (queued trace), and
+    # (untimed) each scene's work counts for the algorithmic bytes
+    from gonzales_amd import abi
+    seq_t, sums = [], torch.zeros(3, dtype=torch.int64, device=dev)
+    d_st = None
     for e in entries:
         rnd = FrameRenderer(e.camera, e.integrator, e.film, batch_paths=args.batch_round_paths)
         rnd.render(stream)  # untimed: allocates the scene's queue storage
@@ -452,30 +526,129 @@ def batch_leg(args, dev, stream) -> dict:
         rnd.render(stream)
         b.record(stream)
         torch.cuda.synchronize()
-        t = a.elapsed_time(b)
-        seq_ms += t
-        per_scene.append(e.camera.path_count / (t * 1e-3))
+        seq_t.append(a.elapsed_time(b))
+        if d_st is None or d_st.numel() < 3 * rnd.batch:
+            d_st = torch.empty(3 * rnd.batch, dtype=torch.int32, device=dev)
+        first, cn = 0, e.camera.path_count
+        while first < cn:
+            c = min(rnd.batch, cn - first)
+            e.camera.paths_device(rnd.d_paths, first, c, stream)
+            d_st[:3 * c].zero_()
+            e.integrator.trace_stats(rnd.d_paths, d_st, c, stream)
+            sums += d_st[:3 * c].view(c, 3).to(torch.int64).sum(0)
+            first += c
         del rnd
         e.integrator.release_queues()  # up to 170 B per path: 64 scenes' worth would not fit
-    rates = np.array(per_scene)
+    del d_st
+    torch.cuda.synchronize()
+    for bh in hs:
+        bh.check_traversal()
+    seq_ms = float(sum(seq_t))
+    st = sums.tolist()
+    algo = 2 * abi.GZ_PATH.itemsize * n + (PATH_RECORD_BYTES + SHADING_FETCH_BYTES) * st[2] + 64 * st[0] + 40 * st[1]
+    achieved = algo / (ms * 1e-3) / 1e9
+    rates = np.array([e.camera.path_count / (t * 1e-3) for e, t in zip(entries, seq_t)])
+    # each size class alone in one batch (same launch shape) against the loop over its scenes
+    counts = np.array([e.camera.path_count for e in entries])
+    by_class = {}
+    for name, sel in (("small: <= 1M paths (256^2 x 4-16 spp, 512^2 x 4 spp)", counts <= (1 << 20)),
+                      ("medium: 1M-4M paths", (counts > (1 << 20)) & (counts <= (1 << 22))),
+                      ("large: > 4M paths", counts > (1 << 22))):
+        idx = np.nonzero(sel)[0]
+        if len(idx) == 0:
+            continue
+        sub = SceneBatch([entries[k] for k in idx], round_paths=args.batch_round_paths)
+        sub.set_launch(best.block, best.grid_scale, best.chunk, best.drain_level, best.queued)
+        sub.render(True)
+        torch.cuda.synchronize()
+        a.record(stream)
+        sub.render(True, stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        t_b = a.elapsed_time(b)
+        sub.close()
+        t_l = float(sum(seq_t[k] for k in idx))
+        by_class[name] = {"scenes": int(len(idx)), "path_samples": int(counts[idx].sum()),
+                          "batch_ms": round(t_b, 2), "one_scene_at_a_time_ms": round(t_l, 2),
+                          "batch_speedup": round(t_l / t_b, 3)}
+        log(f"batch class {name}: {len(idx)} scenes, batch {t_b:.1f} ms vs loop {t_l:.1f} ms")
     res = {"workload": f"{len(specs)} scenes (seed 7): {sum(s['tris'] for s in specs)} triangles in total, "
                        "log-uniform 1e4..2e6 per scene, displaced spheres and soups, 256^2/512^2/1024^2, "
                        "4/16/64 spp, diffuse/dielectric/GGX conductor patches, area light + sky, max depth 6",
            "path_samples": n, "rounds": info.rounds, "round_paths": info.roundPaths,
            "path_samples_per_s": round(n / (ms * 1e-3), 1), "ms_per_batch": round(ms, 2),
            "one_scene_at_a_time_ms": round(seq_ms, 2), "batch_speedup": round(seq_ms / ms, 3),
+           "by_size_class": by_class,
            "per_scene_path_samples_per_s_alone": {"min": round(float(rates.min()), 1),
                                                    "median": round(float(np.median(rates)), 1),
                                                    "max": round(float(rates.max()), 1)},
+           "bounces_per_path": round(st[2] / n, 3), "interior_per_path": round(st[0] / n, 2),
+           "tris_per_path": round(st[1] / n, 2), "bytes_per_path": round(algo / n, 1),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4)},
            "host_scene_build_s": round(t_build, 2), "launch": describe(timings)[0], "launch_ms": describe(timings)}
     log(f"batch leg: {n / (ms * 1e-3) / 1e6:.1f} M path samples/s over {n} samples, {ms:.0f} ms "
-        f"(one scene at a time {seq_ms:.0f} ms), {info.rounds} rounds")
+        f"(one scene at a time {seq_ms:.0f} ms), {info.rounds} rounds, {algo / n:.0f} B/path")
     batch.close()
+    if run_cpu:
+        res.update(batch_cpu_leg(args, entries, host, dev))
     for e in entries:
         e.film = None
     for bh in hs:
         bh.free()
     return res
+
+
+def batch_cpu_leg(args, entries, host: dict, dev) -> dict:
+    """The oracle's chain on blocks of 16 pixels spread over every scene of the batch, timed at
+    share - 1 and share threads (about --cpu-seconds / 2 in all), and the fraction of those path
+    samples whose device records agree with the oracle's."""
+    import ctypes
+
+    import oracle
+    cpu_build()
+    table = oracle.sobol_matrices()
+    n1, nn = host["threads"]
+    views = []
+    for e in entries:
+        integ, opt = e.integrator, e.integrator.options
+        sky = (0.0, 0.0, 0.0) if integ.sky is None else tuple(float(x) for x in integ.sky)
+        prm = oracle.MisParams(opt.max_depth, 0, opt.zsobol_seed, opt.log2_spp, opt.base4_digits, opt.rr_seed,
+                               (ctypes.c_float * 3)(*sky), 0 if integ.sky is None else 1)
+        desc, keep = integ.bh.descriptor()
+        views.append((e, prm, desc, keep))
+
+    def run(samples, th):
+        t1 = time.perf_counter()
+        for (e, prm, desc, _), smp in zip(views, samples):
+            oracle.mis_trace(ctypes.addressof(desc), len(e.integrator.bh.nodes), e.integrator.materials,
+                             e.integrator.lights, prm, smp, th, table)
+        return time.perf_counter() - t1
+
+    calib = [spread_sample(e.camera, 256, table)[0] for e in entries]
+    rate = sum(len(c) for c in calib) / run(calib, nn)
+    per_scene = int(max(256, rate * args.cpu_seconds / 4 / len(entries)))
+    picks = [spread_sample(e.camera, per_scene, table) for e in entries]
+    m = sum(len(p[0]) for p in picks)
+    rates, last = {}, None
+    for th in sorted({n1, nn}):
+        last = [p[0].copy() for p in picks]
+        rates[th] = round(m / run(last, th), 1)
+    rates.setdefault(n1, rates[nn])
+    rates.setdefault(nn, rates[n1])
+    agree = [paths_agreeing(gpu_paths_of_ranges(e.camera, e.integrator, p[1], dev), smp)
+             for e, p, smp in zip(entries, picks, last)]
+    w = np.array([len(s) for s in last], np.float64)
+    frac = float((np.array(agree) * w).sum() / w.sum())
+    for e in entries:
+        e.integrator.release_queues()
+    rec = cpu_baseline_record("path samples/s", rates, host, "port",
+                              f"{m} path samples of the {len(entries)} scenes (blocks of 16 pixels spread over "
+                              "each frame): every MIS bounce, the oracle's traversal per ray")
+    log(f"batch cpu oracle {rates[n1]:.0f} / {rates[nn]:.0f} path samples/s on {n1} / {nn} threads ({m} paths), "
+        f"agreeing {frac:.5f} (worst scene {min(agree):.5f})")
+    return {"cpu_baseline": rec, "cpu_sample_paths_agreeing": round(frac, 5),
+            "cpu_sample_paths_agreeing_worst_scene": round(float(min(agree)), 5)}
 
 
 def adjoint_leg(args, dev, stream) -> dict:
@@ -525,11 +698,33 @@ def adjoint_leg(args, dev, stream) -> dict:
         bwd.append(ev[1].elapsed_time(ev[2]))
     loss = float(((img.double() - target) ** 2).sum())
     f, b = float(np.median(fwd)), float(np.median(bwd))
+    # work counts of the frame's paths (untimed): the replay traces the same rays and reads and
+    # writes the same records as the forward trace
+    from gonzales_amd import abi
+    n = r.n
+    d_st = torch.zeros(3 * n, dtype=torch.int32, device=dev)
+    r.d_paths.copy_(r.d_start)
+    r.integrator.trace_stats(r.d_paths, d_st)
+    st = d_st.view(n, 3).to(torch.int64).sum(0).tolist()
+    bh.check_traversal()
+    trace_bytes = (PATH_RECORD_BYTES + SHADING_FETCH_BYTES) * st[2] + 64 * st[0] + 40 * st[1]
+    # forward: camera records written + trace + film read; backward: the start records copied,
+    # the per-path adjoint input (radiance + pixel adjoint, 24 B written and read) + the replay
+    fwd_bytes = 2 * abi.GZ_PATH.itemsize * n + trace_bytes
+    bwd_bytes = 2 * abi.GZ_PATH.itemsize * n + 48 * n + trace_bytes
     res = {"workload": f"inverse rendering: Cornell box + 2 displaced icospheres ({len(bh.prim_ids)} triangles), "
                        f"32 diffuse + 32 conductor parameter materials, {args.adj_res}x{args.adj_res} x "
                        f"{args.adj_spp} spp, max depth 5, L2 loss vs a {args.adj_target_spp}-spp target",
            "forward_ms": round(f, 3), "backward_ms": round(b, 3), "adjoint_over_forward": round(b / f, 3),
            "loss": loss, "grad_albedo_norm": float(ga.norm()), "grad_roughness_norm": float(gr.norm()),
+           "bounces_per_path": round(st[2] / n, 3), "forward_bytes_per_path": round(fwd_bytes / n, 1),
+           "backward_bytes_per_path": round(bwd_bytes / n, 1),
+           "roofline": {"bound": "hbm", "kernel": "backward (path replay)",
+                        "achieved": round(bwd_bytes / (b * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(bwd_bytes / (b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+           "roofline_forward": {"bound": "hbm", "achieved": round(fwd_bytes / (f * 1e-3) / 1e9, 1),
+                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(fwd_bytes / (f * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
            "gradient_check": "tests/test_gpu_adjoint.py: reflectance within 1% of central differences per "
                              "parameter; conductor roughness within 2% per parameter without visibility edges "
                              "(4M paths), gradient direction cosine > 0.8 with occluders"}
@@ -568,6 +763,17 @@ def main() -> int:
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+
+    if args.only:
+        stream = torch.cuda.current_stream()
+        host = cpu_host()
+        run_cpu = not args.no_cpu
+        leg = {"path": lambda: path_leg(args, dev, stream, host, run_cpu),
+               "full": lambda: full_path_leg(args, dev, stream, host, run_cpu),
+               "batch": lambda: batch_leg(args, dev, stream, host, run_cpu),
+               "adjoint": lambda: adjoint_leg(args, dev, stream)}[args.only]()
+        print(json.dumps({"leg": args.only, **leg}), flush=True)
+        return 0
 
     t0 = time.perf_counter()
     bh = scenes.soup_hierarchy(args.tris, 1)
@@ -660,56 +866,78 @@ def main() -> int:
         bh.set_trace_config(best.persistent, best.lds_stack, best.refill_lanes, best.grid_scale, best.chunk)
         extra["device_build"] = device_build_leg(args, bh, d_rays, d_tmax, n, stream, mrays)
         # End to end through host buffers (H2D + trace + Intersection_C expansion + D2H), 1M rays.
+        # (steady state: the first call grows the staging buffers and is not timed)
         m = 1 << 20
-        t1 = time.perf_counter()
         bh.intersect_batch_gpu(rays[:m], tmax[:m])
+        t1 = time.perf_counter()
+        for _ in range(3):
+            bh.intersect_batch_gpu(rays[:m], tmax[:m])
         t2 = time.perf_counter()
-        extra["host_buffer_path_mrays_per_s"] = round(m / (t2 - t1) / 1e6, 2)
+        extra["host_buffer_path_mrays_per_s"] = round(3 * m / (t2 - t1) / 1e6, 2)
         bh.check_traversal()
         log(f"any-hit {extra['incoherent_anyhit_mrays_per_s']} Mrays/s, coherent "
             f"{extra['coherent_closest_mrays_per_s']} / {extra['coherent_anyhit_mrays_per_s']}")
 
     cpu = None
+    host = cpu_host()
     if not args.no_cpu:
         import ctypes
 
         import oracle
 
+        cpu_build()
         desc, meshes_c = bh.descriptor()
-        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        cores = max(1, min(cores, 16))
-        # calibrate, then size the sample to about --cpu-seconds of work
-        calib = 1 << 14
+        n1, nn = host["threads"]
+        # calibrate at all share threads, then size the sample so that both timed runs together
+        # take about --cpu-seconds
+        calib = 1 << 15
         t1 = time.perf_counter()
-        oracle.traverse(ctypes.addressof(desc), len(bh.nodes), rays[:calib], tmax[:calib], abi.INTERSECTION, cores)
+        oracle.traverse(ctypes.addressof(desc), len(bh.nodes), rays[:calib], tmax[:calib], abi.INTERSECTION, nn)
         rate = calib / (time.perf_counter() - t1)
-        m = int(min(n, max(calib, rate * args.cpu_seconds)))
+        m = int(min(n, max(calib, rate * args.cpu_seconds / 2)))
         m -= m % 4096 if m > 4096 else 0
-        t1 = time.perf_counter()
-        ref, ref_st = oracle.traverse(ctypes.addressof(desc), len(bh.nodes), rays[:m], tmax[:m], abi.INTERSECTION,
-                                      cores, with_stats=True)
-        dt = time.perf_counter() - t1
-        cpu_rate = m / dt / 1e6
-        # parity of the timed GPU kernel on the same sample (bit-exact t, u, v and work counts)
+        rates, outs = {}, {}
+        for th in sorted({n1, nn}):
+            t1 = time.perf_counter()
+            outs[th] = oracle.traverse(ctypes.addressof(desc), len(bh.nodes), rays[:m], tmax[:m], abi.INTERSECTION,
+                                       th, with_stats=True)
+            rates[th] = round(m / (time.perf_counter() - t1) / 1e6, 4)
+        rates.setdefault(n1, rates[nn])
+        rates.setdefault(nn, rates[n1])
+        ref, ref_st = outs[n1]
+        # parity of the timed GPU kernel on the same sample: bit-exact t, u, v, the whole PrimId
+        # of the hit primitive, the hit flag and the per-ray work counts
         hits = d_hits.view(torch.int32).view(n, 4)[:m].cpu().numpy().view(abi.GZ_HIT).reshape(-1)
         gst = d_stats.view(torch.int32).view(n, 2)[:m].cpu().numpy().view(abi.TRACE_STATS).reshape(-1)
+        hit = hits["prim"] >= 0
+        pid = bh.prim_ids[hits["prim"][hit]]
         ok = bool(np.array_equal(hits["t"].view(np.uint32), ref["tHit"].view(np.uint32))
-                  and np.array_equal(hits["prim"] >= 0, ref["hit"].astype(bool))
-                  and gst.tobytes() == ref_st.tobytes())
+                  and np.array_equal(hits["u"][hit].view(np.uint32), ref["u"][hit].view(np.uint32))
+                  and np.array_equal(hits["v"][hit].view(np.uint32), ref["v"][hit].view(np.uint32))
+                  and np.array_equal(hit, ref["hit"].astype(bool))
+                  and all(np.array_equal(pid[f], ref["primId"][f][hit]) for f in ("id1", "id2", "materialIndex", "type"))
+                  and gst.tobytes() == ref_st.tobytes()
+                  and all(o[0].tobytes() == ref.tobytes() for o in outs.values()))
         extra["cpu_sample_parity_bit_exact"] = ok
-        cpu = {"value": round(cpu_rate, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
-               "sample": f"first {m} of the {n} incoherent rays, same scene and BVH2, "
-                         f"{cores} threads over 4096-ray chunks, {dt:.1f} s"}
-        log(f"cpu oracle {cpu_rate:.3f} Mrays/s on {cores} threads ({m} rays, {dt:.1f}s), parity {ok}")
+        cpu = cpu_baseline_record("Mrays/s", rates, host, "port",
+                                  f"first {m} of the {n} incoherent rays, same scene and BVH2, the oracle's "
+                                  "mojo_cpu_traverse_batch loop over 4096-ray chunks")
+        per_thread = rates[n1] / n1
+        extra["gpu_over_cpu_baseline"] = round(mrays / rates[n1], 1)
+        extra["cpu_extrapolated_all_host_threads"] = {
+            "value": round(per_thread * host["nproc"], 3), "unit": "Mrays/s",
+            "note": f"not measured: the per-thread rate at {n1} threads times the host's {host['nproc']} logical "
+                    "CPUs (perfect scaling assumed); the box grants this process a share of "
+                    f"{host['share']} threads", "gpu_over_it": round(mrays / (per_thread * host["nproc"]), 1)}
+        log(f"cpu oracle {rates[n1]:.3f} / {rates[nn]:.3f} Mrays/s on {n1} / {nn} threads ({m} rays, "
+            f"{host['model']}, nproc {host['nproc']}), parity {ok}")
 
     if not args.no_path:
-        pcores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        extra["path_trace"] = path_leg(args, dev, stream, max(1, min(pcores, 16)), not args.no_cpu)
+        extra["path_trace"] = path_leg(args, dev, stream, host, not args.no_cpu)
     if not args.no_full_path:
-        pcores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        extra["path_trace_full"] = full_path_leg(args, dev, stream, max(1, min(pcores, 16)), not args.no_cpu)
+        extra["path_trace_full"] = full_path_leg(args, dev, stream, host, not args.no_cpu)
     if not args.no_batch:
-        extra["batch"] = batch_leg(args, dev, stream)
+        extra["batch"] = batch_leg(args, dev, stream, host, not args.no_cpu)
     if not args.no_adjoint:
         extra["adjoint"] = adjoint_leg(args, dev, stream)
 

@@ -192,6 +192,7 @@ SIGNATURES = {
     "gz_batch_info": (ctypes.c_int, [_P, _P]),
     "gz_batch_set_launch": (ctypes.c_int, [_P, _P]),
     "gz_batch_render_device": (ctypes.c_int, [_P, ctypes.c_int32, _P]),
+    "gz_batch_check": (ctypes.c_int, [_P, _P]),
     "gz_batch_free": (None, [_P]),
 }
 

@@ -70,6 +70,11 @@ class SceneBatch:
         abi.check(abi.load().gz_batch_render_device(self.handle, 1 if use_graph else 0, abi.stream_ptr(stream)),
                   "gz_batch_render_device")
 
+    def check(self, stream=None) -> None:
+        """Waits for `stream`; raises if a traversal of the batch's paths overflowed its stack since
+        the last check (gz_batch_check)."""
+        abi.check(abi.load().gz_batch_check(self.handle, abi.stream_ptr(stream)), "gz_batch_check")
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             abi.load().gz_batch_free(self.handle)

@@ -590,6 +590,12 @@ struct Batch {
         hipStream_t captureStream = nullptr;
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
+
+        // The batch's own error flag (the MisArgs copies raise it; gz_batch_check reads it) and
+        // each scene's MIS-table generation at create: dArgs holds pointers into those tables,
+        // which gz_gpu_set_mis_scene frees and replaces.
+        int *dErr = nullptr;
+        std::vector<uint64_t> misGeneration;
 };
 
 static void release_graph(Batch *b) {
@@ -605,7 +611,7 @@ static void free_batch(Batch *b) {
         if (hipGetDevice(&cur) == hipSuccess && cur != b->device) (void)hipSetDevice(b->device);
         release_graph(b);
         if (b->captureStream) (void)hipStreamDestroy(b->captureStream);
-        void *ptrs[] = {b->dCursor,   b->dRounds,  b->dItems,   b->dArgs,     b->dCams,     b->dFilms,
+        void *ptrs[] = {b->dErr,      b->dCursor,   b->dRounds,  b->dItems,   b->dArgs,     b->dCams,     b->dFilms,
                         b->dPaths,    b->dQ[0],    b->dQ[1],    b->ctr.n,     b->ctr.head,  b->ctr.base,
                         b->wctr.n,    b->wctr.base, b->ws.hits,  b->ws.pend,   b->ws.srays,
                         b->ws.mrays,  b->ws.occ,   b->ws.mhits, b->ws.q[0],  b->ws.q[1],   b->ws.q[2],
@@ -802,6 +808,7 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
         std::vector<BatchFilm> films((size_t)sceneCount);
         std::vector<char> waveScene((size_t)sceneCount, 0);
         int maxDepth = 0;
+        b->misGeneration.assign((size_t)sceneCount, 0);
         for (int32_t s = 0; s < sceneCount; ++s) {
                 const GzBatchScene &bs = scenes[s];
                 Scene *sc = static_cast<Scene *>(bs.handle);
@@ -810,6 +817,7 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
                         free_batch(b);
                         return nullptr;
                 }
+                b->scenes.push_back(sc);
                 if (!camera_ok(&bs.camera, "gz_batch_create")) {
                         free_batch(b);
                         return nullptr;
@@ -829,12 +837,18 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
                 if (s == 0) {
                         b->device = sc->device;
                         b->cuCount = sc->cuCount;
+                        if (!enter_device(sc) || !hip_ok(hipMalloc((void **)&b->dErr, sizeof(int)), "batch error flag") ||
+                            !hip_ok(hipMemset(b->dErr, 0, sizeof(int)), "batch error flag")) {
+                                free_batch(b);
+                                return nullptr;
+                        }
                 } else if (sc->device != b->device) {
                         set_error("gz_batch_create: scene %d is on device %d, scene 0 on %d", s, sc->device, b->device);
                         free_batch(b);
                         return nullptr;
                 }
-                if (!enter_device(sc) || !mis_args(sc, &bs.params, args[(size_t)s])) {
+                b->misGeneration[(size_t)s] = sc->misGeneration;
+                if (!enter_device(sc) || !mis_args(sc, &bs.params, b->dErr, args[(size_t)s])) {
                         free_batch(b);
                         return nullptr;
                 }
@@ -973,6 +987,15 @@ extern "C" int gz_batch_render_device(void *batch, int32_t useGraph, void *strea
                 return GZ_E_INVALID;
         }
         Batch *b = static_cast<Batch *>(batch);
+        for (int k = 0; k < b->nScenes; ++k) {
+                Scene *sc = b->scenes[(size_t)k];
+                std::lock_guard<std::mutex> lock(sc->mu);
+                if (sc->misGeneration != b->misGeneration[(size_t)k]) {
+                        set_error("gz_batch_render_device: scene %d's MIS tables were replaced after gz_batch_create "
+                                  "(gz_gpu_set_mis_scene); create the batch again", k);
+                        return GZ_E_INVALID;
+                }
+        }
         int cur = -1;
         if (!hip_ok(hipGetDevice(&cur), "get device") || (cur != b->device && !hip_ok(hipSetDevice(b->device), "set device")))
                 return GZ_E_HIP;
@@ -988,6 +1011,33 @@ extern "C" int gz_batch_render_device(void *batch, int32_t useGraph, void *strea
                 }
         }
         return GZ_OK;
+}
+
+extern "C" int gz_batch_check(void *batch, void *stream) {
+        clear_error();
+        if (!batch) {
+                set_error("gz_batch_check: null batch");
+                return GZ_E_INVALID;
+        }
+        Batch *b = static_cast<Batch *>(batch);
+        int cur = -1;
+        if (!hip_ok(hipGetDevice(&cur), "get device") || (cur != b->device && !hip_ok(hipSetDevice(b->device), "set device")))
+                return GZ_E_HIP;
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        int flag = 0;
+        if (!hip_ok(hipMemcpyAsync(&flag, b->dErr, sizeof(int), hipMemcpyDeviceToHost, st), "read batch error flag") ||
+            !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+                return GZ_E_HIP;
+        if (!flag) return GZ_OK;
+        if (!hip_ok(hipMemsetAsync(b->dErr, 0, sizeof(int), st), "clear batch error flag") ||
+            !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+                return GZ_E_HIP;
+        if (flag & 1) {
+                set_error("batch traversal stack overflow: a ray needed more than %d entries", GZ_STACK_CAPACITY);
+                return GZ_E_OVERFLOW;
+        }
+        set_error("batch: a shading kernel met a primitive, mesh or vertex index out of range");
+        return GZ_E_INVALID;
 }
 
 extern "C" void gz_batch_free(void *batch) { free_batch(static_cast<Batch *>(batch)); }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <string>
 
@@ -56,8 +57,21 @@ void release_mis(MisScene *m);
 struct MisQueues;
 void release_mis_queues(MisQueues *m);
 
+// Per-stream device state of a scene handle (allocated zeroed at the stream's first use):
+//   dErr     : bit 0 = a traversal stack overflowed, bit 1 = a shading kernel met a bad index;
+//              raised by the launches on the stream, read and cleared by
+//              gz_gpu_check_traversal(stream), so concurrent streams never see each other's flag;
+//   dCounter : the ray counter of the persistent traversal kernel, reset by a kernel at the start
+//              of each launch; launches on one stream run in order, so they can share it.
+struct StreamSlot {
+        int *dErr = nullptr;
+        unsigned long long *dCounter = nullptr;
+};
+
 struct Scene {
         std::mutex mu;
+        std::map<hipStream_t, StreamSlot> slots;
+        uint64_t misGeneration = 0;  // bumped by every gz_gpu_set_mis_scene (batches check it)
         MisScene *mis = nullptr;
         MisQueues *misQueues = nullptr;
         GzMisLaunch misLaunch{1, 256, 1, 256, 3, {0, 0, 0}};
@@ -66,6 +80,7 @@ struct Scene {
         hipStream_t stream = nullptr;  // used by the synchronous host-buffer entry points
 
         int64_t nodeCount = 0, interiorCount = 0, primCount = 0, triCount = 0;
+        int64_t interfacePrims = 0;  // surface primitives with materialIndex < 0 (interfaces)
         int64_t meshCount = 0, materialCount = 0;
         int32_t maxDepth = 0;
         int32_t rootDesc = 0;
@@ -78,7 +93,6 @@ struct Scene {
         PrimId_C *dPrims = nullptr;
         DevMesh *dMeshes = nullptr;
         Material_C *dMaterials = nullptr;
-        int *dErr = nullptr;  // bit 0: stack overflow seen
         void **meshBuffers = nullptr;  // per-mesh device arrays to free
         int64_t meshBufferCount = 0;
         int64_t deviceBytes = 0;
@@ -87,16 +101,11 @@ struct Scene {
         void *dStage = nullptr;
         size_t stageBytes = 0;
 
-        // Traversal launch configuration and the persistent kernel's ray counters (a ring, so
-        // launches queued on different streams never share a counter).
+        // Traversal launch configuration.
         GzTraceConfig traceConfig{1, 16, 56, 1, 64, {0, 0, 0}};
         bool boundsSafe = false;  // every node box coordinate is finite with magnitude <= 2^40
         int cuCount = 256;
-        unsigned long long *dCounters = nullptr;
-        uint64_t launchSeq = 0;
 };
-
-constexpr int kCounterRing = 1024;
 
 void set_error(const char *fmt, ...);
 void clear_error();
@@ -104,6 +113,11 @@ bool hip_ok(hipError_t e, const char *what);
 
 // Makes the scene's device current; returns false (error set) on failure.
 bool enter_device(Scene *s);
+// The slot of `stream` on this handle (the caller holds s->mu); nullptr (error set) on failure.
+StreamSlot *stream_slot(Scene *s, hipStream_t stream);
+// Reads and clears the error flag of `stream` (synchronizes the stream): GZ_OK, GZ_E_OVERFLOW or
+// GZ_E_HIP.
+int check_stream(Scene *s, hipStream_t stream);
 // Ensures the staging buffer holds at least `bytes`.
 bool ensure_stage(Scene *s, size_t bytes);
 

@@ -857,11 +857,23 @@ extern "C" int gz_gpu_set_mis_scene(void *handle, const GzMaterial *materials, i
         }
         Scene *s = static_cast<Scene *>(handle);
         std::lock_guard<std::mutex> lock(s->mu);
+        // Interface materials (materialIndex < 0 on a surface): the reference's integrator asserts
+        // materialIndex >= 0 at every surface hit and then indexes its materials with it
+        // (VolumePathIntegrator.swift:341-342), so a path that meets one cannot be rendered; only
+        // its shadow rays continue through them (Visibility.swift:8-20). Such scenes are refused.
+        if (s->interfacePrims > 0) {
+                set_error("gz_gpu_set_mis_scene: %lld surface primitives carry an interface material "
+                          "(materialIndex < 0), which the MIS integrator does not render",
+                          (long long)s->interfacePrims);
+                return GZ_E_INVALID;
+        }
         if (!enter_device(s)) return GZ_E_HIP;
-        return set_mis_scene(s, materials, materialCount, lights, lightCount, skyRadiance);
+        const int rc = set_mis_scene(s, materials, materialCount, lights, lightCount, skyRadiance);
+        if (rc == GZ_OK) ++s->misGeneration;
+        return rc;
 }
 
-bool gz::mis_args(Scene *s, const GzMisParams *params, MisArgs &a) {
+bool gz::mis_args(Scene *s, const GzMisParams *params, int *err, MisArgs &a) {
         const MisScene *ms = s->mis;
         if (!ms) {
                 set_error("no MIS scene on this handle: call gz_gpu_set_mis_scene first");
@@ -898,7 +910,7 @@ bool gz::mis_args(Scene *s, const GzMisParams *params, MisArgs &a) {
         a.maxDepth = params->maxDepth;
         a.flags = params->flags;
         a.rrSeed = params->rrSeed;
-        a.err = s->dErr;
+        a.err = err;
         return true;
 }
 
@@ -956,8 +968,10 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
         Scene *s = static_cast<Scene *>(handle);
         std::lock_guard<std::mutex> lock(s->mu);
         if (!enter_device(s)) return GZ_E_HIP;
+        const StreamSlot *slot = stream_slot(s, static_cast<hipStream_t>(stream));
+        if (!slot) return GZ_E_ALLOC;
         MisArgs a;
-        if (!mis_args(s, params, a)) return GZ_E_INVALID;
+        if (!mis_args(s, params, slot->dErr, a)) return GZ_E_INVALID;
         if (count == 0) return GZ_OK;
         const int64_t blocks = (count + kMisBlock - 1) / kMisBlock;
         if (blocks > 0x7fffffffLL) {
@@ -1218,8 +1232,10 @@ extern "C" int gz_gpu_mis_adjoint_device(void *handle, GzPath *d_paths, const Gz
         Scene *s = static_cast<Scene *>(handle);
         std::lock_guard<std::mutex> lock(s->mu);
         if (!enter_device(s)) return GZ_E_HIP;
+        const StreamSlot *slot = stream_slot(s, static_cast<hipStream_t>(stream));
+        if (!slot) return GZ_E_ALLOC;
         MisArgs a;
-        if (!mis_args(s, params, a)) return GZ_E_INVALID;
+        if (!mis_args(s, params, slot->dErr, a)) return GZ_E_INVALID;
         if (count == 0) return GZ_OK;
         const int64_t blocks = (count + kMisBlock - 1) / kMisBlock;
         if (blocks > 0x7fffffffLL) {

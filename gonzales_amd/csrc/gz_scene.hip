@@ -60,6 +60,40 @@ bool ensure_stage(Scene *s, size_t bytes) {
         return true;
 }
 
+StreamSlot *stream_slot(Scene *s, hipStream_t stream) {
+        auto it = s->slots.find(stream);
+        if (it != s->slots.end()) return &it->second;
+        // 64 bytes: the flag at 0, the counter at 8 (zeroed before any launch can use them)
+        void *p = nullptr;
+        if (!hip_ok(hipMalloc(&p, 64), "hipMalloc(stream slot)")) return nullptr;
+        if (!hip_ok(hipMemset(p, 0, 64), "hipMemset(stream slot)")) {
+                (void)hipFree(p);
+                return nullptr;
+        }
+        s->deviceBytes += 64;
+        StreamSlot slot{static_cast<int *>(p), reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 8)};
+        return &s->slots.emplace(stream, slot).first->second;
+}
+
+int check_stream(Scene *s, hipStream_t st) {
+        auto it = s->slots.find(st);
+        if (it == s->slots.end()) return GZ_OK;  // nothing ran on this stream
+        int flag = 0;
+        if (!hip_ok(hipMemcpyAsync(&flag, it->second.dErr, sizeof(int), hipMemcpyDeviceToHost, st), "read error flag") ||
+            !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+                return GZ_E_HIP;
+        if (!flag) return GZ_OK;
+        if (!hip_ok(hipMemsetAsync(it->second.dErr, 0, sizeof(int), st), "clear error flag") ||
+            !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+                return GZ_E_HIP;
+        if (flag & 1) {
+                set_error("traversal stack overflow: a ray needed more than %d entries", GZ_STACK_CAPACITY);
+                return GZ_E_OVERFLOW;
+        }
+        set_error("a shading kernel met a primitive, mesh or vertex index out of range");
+        return GZ_E_INVALID;
+}
+
 static void free_scene(Scene *s) {
         if (!s) return;
         int cur = -1;
@@ -67,8 +101,9 @@ static void free_scene(Scene *s) {
         if (s->stream) (void)hipStreamSynchronize(s->stream);
         release_mis(s->mis);
         release_mis_queues(s->misQueues);
-        void *ptrs[] = {s->dNodes, s->dTris, s->dPrims, s->dMeshes, s->dMaterials, s->dErr, s->dStage, s->dCounters,
-                        s->dBuildOrder};
+        for (auto &kv : s->slots)
+                if (kv.second.dErr) (void)hipFree(kv.second.dErr);
+        void *ptrs[] = {s->dNodes, s->dTris, s->dPrims, s->dMeshes, s->dMaterials, s->dStage, s->dBuildOrder};
         for (void *p : ptrs)
                 if (p) (void)hipFree(p);
         for (int64_t i = 0; i < s->meshBufferCount; ++i)
@@ -94,15 +129,20 @@ bool scene_upload_common(Scene *s, const PrimId_C *prims, int64_t nPrims, const 
                          const int64_t *vtxCounts, const Material_C *materials, int64_t nMaterials) {
         if (!s->stream && !hip_ok(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate"))
                 return false;
+        // surfaces whose material is an interface: triangles (type 0) and geometric or transformed
+        // primitives (types 1, 2) the traversal does not skip (id2 != -1); area lights (type 3)
+        // shade with material 0
+        s->interfacePrims = 0;
+        for (int64_t i = 0; i < nPrims; ++i) {
+                const PrimId_C &p = prims[i];
+                const bool surface = p.type == 0 || ((p.type == 1 || p.type == 2) && p.id2 != -1);
+                if (surface && p.materialIndex < 0) ++s->interfacePrims;
+        }
         bool ok = upload(&s->dPrims, prims, nPrims, s, "upload primIds") &&
                   upload(&s->dMaterials, materials, nMaterials, s, "upload materials") &&
-                  hip_ok(hipMalloc((void **)&s->dErr, sizeof(int)), "hipMalloc(err)") &&
-                  hip_ok(hipMemset(s->dErr, 0, sizeof(int)), "hipMemset(err)") &&
-                  hip_ok(hipMalloc((void **)&s->dCounters, sizeof(unsigned long long) * kCounterRing),
-                         "hipMalloc(counters)") &&
                   hip_ok(hipDeviceGetAttribute(&s->cuCount, hipDeviceAttributeMultiprocessorCount, s->device),
-                         "hipDeviceGetAttribute");
-        s->deviceBytes += (int64_t)(sizeof(unsigned long long) * kCounterRing + sizeof(int));
+                         "hipDeviceGetAttribute") &&
+                  stream_slot(s, s->stream) != nullptr;
         if (ok && nMeshes > 0) {
                 // Per-mesh arrays for the shading kernels (kernel.mojo:488-531 uploads the same).
                 std::vector<DevMesh> dm((size_t)nMeshes);
@@ -382,19 +422,7 @@ extern "C" int gz_gpu_check_traversal(void *handle, void *stream) {
         Scene *s = static_cast<Scene *>(handle);
         std::lock_guard<std::mutex> lock(s->mu);
         if (!enter_device(s)) return GZ_E_HIP;
-        hipStream_t st = static_cast<hipStream_t>(stream);
-        int flag = 0;
-        if (!hip_ok(hipMemcpyAsync(&flag, s->dErr, sizeof(int), hipMemcpyDeviceToHost, st), "read overflow flag") ||
-            !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
-                return GZ_E_HIP;
-        if (flag) {
-                if (!hip_ok(hipMemsetAsync(s->dErr, 0, sizeof(int), st), "clear overflow flag") ||
-                    !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
-                        return GZ_E_HIP;
-                set_error("traversal stack overflow: a ray needed more than %d entries", GZ_STACK_CAPACITY);
-                return GZ_E_OVERFLOW;
-        }
-        return GZ_OK;
+        return check_stream(s, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int gz_gpu_set_trace_config(void *handle, const GzTraceConfig *config) {

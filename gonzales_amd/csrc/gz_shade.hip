@@ -161,8 +161,10 @@ using namespace gz;
 
 static int shade_launch(Scene *s, PathState_C *d_paths, int64_t n, const Intersection_C *d_isects, hipStream_t st) {
         if (n <= 0) return GZ_OK;
+        const StreamSlot *slot = stream_slot(s, st);
+        if (!slot) return GZ_E_ALLOC;
         hipLaunchKernelGGL(k_shade, dim3((unsigned)((n + kShadeBlock - 1) / kShadeBlock)), dim3(kShadeBlock), 0, st,
-                           d_paths, d_isects, s->dMeshes, s->meshCount, s->dMaterials, s->materialCount, n, s->dErr);
+                           d_paths, d_isects, s->dMeshes, s->meshCount, s->dMaterials, s->materialCount, n, slot->dErr);
         return hip_ok(hipGetLastError(), "shade launch") ? GZ_OK : GZ_E_HIP;
 }
 

@@ -181,8 +181,8 @@ __global__ __launch_bounds__(kBlock) void k_expand(const GzHit *__restrict__ hit
 static inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 template <bool kAny, bool kStats, int kLds>
-static void launch_variant(Scene *s, const Ray_C *d_rays, const float *d_tMax, int64_t n, GzHit *d_hits,
-                           uint8_t *d_occ, GzTraceStats *d_stats, hipStream_t stream) {
+static void launch_variant(Scene *s, const StreamSlot &slot, const Ray_C *d_rays, const float *d_tMax, int64_t n,
+                           GzHit *d_hits, uint8_t *d_occ, GzTraceStats *d_stats, hipStream_t stream) {
         const GzTraceConfig &c = s->traceConfig;
         if (c.persistent == 2) {
                 auto kern = k_trace_strided<kAny, kStats, kLds>;
@@ -192,7 +192,7 @@ static void launch_variant(Scene *s, const Ray_C *d_rays, const float *d_tMax, i
                 const int64_t want = (int64_t)s->cuCount * perCU * (c.gridScale > 0 ? c.gridScale : 1);
                 const unsigned g = (unsigned)std::min<int64_t>(want, grid_for(n));
                 hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, stream, s->dNodes, s->dTris, s->rootDesc,
-                                   s->boundsSafe, d_rays, d_tMax, n, d_hits, d_occ, d_stats, s->dErr);
+                                   s->boundsSafe, d_rays, d_tMax, n, d_hits, d_occ, d_stats, slot.dErr);
         } else if (c.persistent == 1) {
                 auto kern = k_trace_persistent<kAny, kStats, kLds>;
                 int perCU = 0;
@@ -200,28 +200,28 @@ static void launch_variant(Scene *s, const Ray_C *d_rays, const float *d_tMax, i
                         perCU = 1;
                 const int64_t want = (int64_t)s->cuCount * perCU * (c.gridScale > 0 ? c.gridScale : 1);
                 const unsigned g = (unsigned)std::min<int64_t>(want, grid_for(n));
-                unsigned long long *ctr = s->dCounters + (s->launchSeq++ % kCounterRing);
+                unsigned long long *ctr = slot.dCounter;
                 hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, stream, ctr);
                 hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, stream, s->dNodes, s->dTris, s->rootDesc,
-                                   s->boundsSafe, d_rays, d_tMax, n, d_hits, d_occ, d_stats, s->dErr, ctr,
+                                   s->boundsSafe, d_rays, d_tMax, n, d_hits, d_occ, d_stats, slot.dErr, ctr,
                                    c.refillLanes > 0 && c.refillLanes <= 64 ? c.refillLanes : 16,
                                    c.chunk >= 1 && c.chunk <= 4096 ? c.chunk : 64);
         } else {
                 hipLaunchKernelGGL((k_trace<kAny, kStats, kLds>), dim3(grid_for(n)), dim3(kBlock), 0, stream, s->dNodes,
                                    s->dTris, s->rootDesc, s->boundsSafe, d_rays, d_tMax, n, d_hits, d_occ, d_stats,
-                                   s->dErr);
+                                   slot.dErr);
         }
 }
 
 template <bool kAny, bool kStats>
-static void launch_lds(Scene *s, const Ray_C *d_rays, const float *d_tMax, int64_t n, GzHit *d_hits, uint8_t *d_occ,
-                       GzTraceStats *d_stats, hipStream_t stream) {
+static void launch_lds(Scene *s, const StreamSlot &slot, const Ray_C *d_rays, const float *d_tMax, int64_t n,
+                       GzHit *d_hits, uint8_t *d_occ, GzTraceStats *d_stats, hipStream_t stream) {
         if (s->traceConfig.ldsStack >= 16)
-                launch_variant<kAny, kStats, 16>(s, d_rays, d_tMax, n, d_hits, d_occ, d_stats, stream);
+                launch_variant<kAny, kStats, 16>(s, slot, d_rays, d_tMax, n, d_hits, d_occ, d_stats, stream);
         else if (s->traceConfig.ldsStack >= 8)
-                launch_variant<kAny, kStats, 8>(s, d_rays, d_tMax, n, d_hits, d_occ, d_stats, stream);
+                launch_variant<kAny, kStats, 8>(s, slot, d_rays, d_tMax, n, d_hits, d_occ, d_stats, stream);
         else
-                launch_variant<kAny, kStats, 0>(s, d_rays, d_tMax, n, d_hits, d_occ, d_stats, stream);
+                launch_variant<kAny, kStats, 0>(s, slot, d_rays, d_tMax, n, d_hits, d_occ, d_stats, stream);
 }
 
 int launch_trace(Scene *s, const Ray_C *d_rays, const float *d_tMax, int64_t n, GzHit *d_hits, uint8_t *d_occ,
@@ -231,19 +231,21 @@ int launch_trace(Scene *s, const Ray_C *d_rays, const float *d_tMax, int64_t n, 
                 set_error("traversal batch of %lld rays is too large", (long long)n);
                 return GZ_E_INVALID;
         }
+        const StreamSlot *slot = stream_slot(s, stream);
+        if (!slot) return GZ_E_ALLOC;
         if (s->empty) {
                 hipLaunchKernelGGL(k_miss, dim3(grid_for(n)), dim3(kBlock), 0, stream, d_tMax, n,
                                    anyHit ? nullptr : d_hits, anyHit ? d_occ : nullptr, d_stats);
         } else if (anyHit) {
                 if (d_stats)
-                        launch_lds<true, true>(s, d_rays, d_tMax, n, nullptr, d_occ, d_stats, stream);
+                        launch_lds<true, true>(s, *slot, d_rays, d_tMax, n, nullptr, d_occ, d_stats, stream);
                 else
-                        launch_lds<true, false>(s, d_rays, d_tMax, n, nullptr, d_occ, nullptr, stream);
+                        launch_lds<true, false>(s, *slot, d_rays, d_tMax, n, nullptr, d_occ, nullptr, stream);
         } else {
                 if (d_stats)
-                        launch_lds<false, true>(s, d_rays, d_tMax, n, d_hits, nullptr, d_stats, stream);
+                        launch_lds<false, true>(s, *slot, d_rays, d_tMax, n, d_hits, nullptr, d_stats, stream);
                 else
-                        launch_lds<false, false>(s, d_rays, d_tMax, n, d_hits, nullptr, nullptr, stream);
+                        launch_lds<false, false>(s, *slot, d_rays, d_tMax, n, d_hits, nullptr, nullptr, stream);
         }
         return hip_ok(hipGetLastError(), "traversal launch") ? GZ_OK : GZ_E_HIP;
 }
@@ -267,19 +269,7 @@ static Scene *scene_of(void *h, const char *fn) {
         return static_cast<Scene *>(h);
 }
 
-static int check_overflow_sync(Scene *s) {
-        int flag = 0;
-        if (!hip_ok(hipMemcpyAsync(&flag, s->dErr, sizeof(int), hipMemcpyDeviceToHost, s->stream), "read overflow flag") ||
-            !hip_ok(hipStreamSynchronize(s->stream), "hipStreamSynchronize"))
-                return GZ_E_HIP;
-        if (flag) {
-                (void)hipMemsetAsync(s->dErr, 0, sizeof(int), s->stream);
-                (void)hipStreamSynchronize(s->stream);
-                set_error("traversal stack overflow: a ray needed more than %d entries", GZ_STACK_CAPACITY);
-                return GZ_E_OVERFLOW;
-        }
-        return GZ_OK;
-}
+static int check_overflow_sync(Scene *s) { return check_stream(s, s->stream); }
 
 extern "C" int gz_gpu_traverse_closest_device(void *handle, const Ray_C *d_rays, const float *d_tMax, int64_t count,
                                               GzHit *d_hits, void *stream) {

@@ -888,7 +888,7 @@ __device__ __forceinline__ void flush_appends(const Keep4 &keep, const unsigned 
 // Host side (gz_mis.hip): the device arguments of a scene's MIS tables and parameters (false
 // with the error set when the handle has none or the parameters are out of range), and whether
 // every material is diffuse (the kernels built without the other BSDFs then suffice).
-bool mis_args(Scene *s, const GzMisParams *params, MisArgs &a);
+bool mis_args(Scene *s, const GzMisParams *params, int *err, MisArgs &a);
 bool mis_all_diffuse(const Scene *s);
 // Whether a scene's default launch is the full wavefront: every scene but the small all-diffuse
 // ones (at most kDefaultMegakernelPrims primitives), which the queued megakernel serves better.

@@ -1,6 +1,6 @@
 // Sobol generator matrices for the Z-Sobol sampler: 1024 dimensions x 52 columns of uint32.
 //
-// Restates the reference's build-time generator (Sources/SobolGenerator/main.swift:117-185),
+// Restates the reference's build-time generator (Sources/SobolGenerator/main.swift:11-76),
 // which the reference runs as a SwiftPM plugin and embeds as base64 (SobolAccessor.swift:1-20):
 //   dimension 0: column i = 1 << (31 - i) for i < 32, 0 above (van der Corput);
 //   dimension d >= 1: Joe-Kuo row d - 1 (s, a, m_1..m_s): v_i = m_i << (32 - i) for i <= s,

@@ -247,8 +247,12 @@ struct GzTraceConfig {
 int gz_gpu_set_trace_config(void *handle, const struct GzTraceConfig *config);
 int gz_gpu_get_trace_config(void *handle, struct GzTraceConfig *config);
 
-/* Waits for `stream` and returns GZ_E_OVERFLOW if any traversal since the last call stopped on
- * a full stack (the flag is then cleared), else GZ_OK. */
+/* Waits for `stream` and returns GZ_E_OVERFLOW if any traversal launched on `stream` since the
+ * last call stopped on a full stack (GZ_E_INVALID if a shading kernel met an index out of range),
+ * and clears the flag; else GZ_OK. Each stream a handle is used with has its own flag (and its
+ * own ray counter for the persistent traversal), so calls on concurrent streams never read or
+ * clear each other's. No counterpart in the reference (its stack is unchecked,
+ * kernel.mojo:250). */
 int gz_gpu_check_traversal(void *handle, void *stream);
 
 /* Device-resident diffuse shade: the kernel behind gz_gpu_shade_batch without the host round
@@ -509,9 +513,10 @@ struct GzBatchInfo {
 /* Plans the batch: the paths of all scenes (in scene order, whole pixels) are cut into rounds of
  * at most roundPaths path slots, and device tables are built for them; device state (about 280 B
  * per slot) is allocated for the largest round only. The batch snapshots each
- * scene's MIS tables and parameters: call gz_gpu_set_mis_scene on a scene only before creating a
- * batch over it, and free the batch before its scenes. All scenes must be on one device. NULL on
- * failure, with gz_last_error() set. */
+ * scene's MIS tables and parameters: a gz_gpu_set_mis_scene on one of its scenes afterwards makes
+ * gz_batch_render_device refuse to run (GZ_E_INVALID; create the batch again). Free the batch
+ * before its scenes. All scenes must be on one device. NULL on failure, with gz_last_error()
+ * set. */
 void *gz_batch_create(const struct GzBatchScene *scenes, int32_t sceneCount, int64_t roundPaths);
 int gz_batch_info(void *batch, struct GzBatchInfo *info);
 /* Launch shape of the batch's bounce levels (queued 1 or 3, as in GzMisLaunch). queued 3 bounces
@@ -526,6 +531,10 @@ int gz_batch_set_launch(void *batch, const struct GzMisLaunch *launch);
  * replays it once per round; 0 launches it directly. Both give identical films. Calls on one
  * batch must not overlap (they share its path slots and queues). */
 int gz_batch_render_device(void *batch, int32_t useGraph, void *stream);
+/* gz_gpu_check_traversal for a batch: waits for `stream`, returns GZ_E_OVERFLOW if a traversal
+ * of any scene's paths stopped on a full stack since the last call (the batch's own flag; then
+ * cleared), else GZ_OK. */
+int gz_batch_check(void *batch, void *stream);
 void gz_batch_free(void *batch);
 
 /* ---------------------------------------------------------------------------------------- */
@@ -560,7 +569,7 @@ int gz_zsobol_tile_device(const struct GzZSobolParams *params, int32_t x0, int32
                           int32_t nCalls, float *d_out, int32_t layout, void *stream);
 
 /* Host: the 1024 x 52 Sobol generator matrices (column-per-bit uint32) the sampler uses, as the
- * reference's SobolGenerator emits them (SobolGenerator/main.swift:117-185). */
+ * reference's SobolGenerator emits them (SobolGenerator/main.swift:11-76). */
 int gz_sobol_matrices(uint32_t *out);
 
 /* ---------------------------------------------------------------------------------------- */

@@ -71,14 +71,29 @@ _SIGS = {
 }
 
 _lib = None
+_path = LIB
+
+
+def use_native() -> str:
+    """Switches to a build of the same sources with -march=native, compiled now on this machine
+    (make native; the CPU baseline's build). Returns the compiler flags in effect: the native
+    build's, or the portable library's if gcc cannot build it here."""
+    global _lib, _path
+    import subprocess
+    native = HERE / "_native" / "liboracle.so"
+    proc = subprocess.run(["make", "-s", "-C", str(HERE), "native"], capture_output=True, text=True)
+    if proc.returncode != 0 or not native.exists():
+        return "-O3 -ffp-contract=off (portable; the -march=native build failed)"
+    _path, _lib = native, None
+    return "-O3 -march=native -ffp-contract=off"
 
 
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
-        if not LIB.exists():
-            raise RuntimeError(f"{LIB} is not built (make -C oracle)")
-        l = ctypes.CDLL(str(LIB))
+        if not _path.exists():
+            raise RuntimeError(f"{_path} is not built (make -C oracle)")
+        l = ctypes.CDLL(str(_path))
         for name, (res, args) in _SIGS.items():
             f = getattr(l, name)
             f.restype = res

@@ -15,7 +15,7 @@
  *   getSampleHash .......... ZSobolSampler.swift:188-199
  *   get1D / get2D / get3D .. ZSobolSampler.swift:203-280 (hash and index from the pre-wrap dim)
  *   nextPowerOfTwo ......... ZSobolSampler.swift:283-295
- *   generateSobolMatrices .. SobolGenerator/main.swift:117-185 (1024 dims x 52 columns)
+ *   generateSobolMatrices .. SobolGenerator/main.swift:11-76 (1024 dims x 52 columns)
  *
  * Pinning (tests/test_oracle_zsobol.py): the reference's own KATs (SobolSamplerTests.swift:9-75),
  * the matrix table against scipy's independent Joe-Kuo implementation (all 1024 x 52 columns),
@@ -85,7 +85,7 @@ int64_t orc_next_pow2(int64_t v) {
         return v + 1;
 }
 
-/* SobolGenerator/main.swift:117-185. Reads the Joe-Kuo direction-number text ("d s a m_1 ...";
+/* SobolGenerator/main.swift:11-76. Reads the Joe-Kuo direction-number text ("d s a m_1 ...";
  * header and blank lines skipped) and fills out[1024 * 52]. Returns 0, or -1 on a read error. */
 int orc_sobol_matrices(const char *path, uint32_t *out) {
         FILE *f = fopen(path, "r");

@@ -272,3 +272,131 @@ def brute_force_sky(bh, sky, rays: np.ndarray, max_depth: int, seed: int, nthrea
         cur["orgX"], cur["orgY"], cur["orgZ"] = org[:, 0], org[:, 1], org[:, 2]
         cur["dirX"], cur["dirY"], cur["dirZ"] = wi[:, 0], wi[:, 1], wi[:, 2]
     return L
+
+
+# ---- GPU-against-oracle comparison of whole path records and films ---------------------------
+
+def to_device(paths):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(paths).view(np.uint8).copy()).cuda()
+
+
+def to_host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy().view(abi.GZ_PATH)
+
+
+def oracle_params(om: OracleMis, opt) -> oracle.MisParams:
+    return om.params(opt.max_depth, fix=opt.fix_light_pdf, fix_sky=opt.fix_sky_direction, seed=opt.zsobol_seed,
+                     log2spp=opt.log2_spp, digits=opt.base4_digits, rr_seed=opt.rr_seed)
+
+
+def agree_mask(got: np.ndarray, ref: np.ndarray) -> np.ndarray:
+    """Per path: same bounce count, dimensions and flags, estimate and throughput within 1e-4
+    relative (+1e-6 absolute)."""
+    e_g, e_r = got["estimate"].astype(np.float64), ref["estimate"].astype(np.float64)
+    t_g, t_r = got["throughput"].astype(np.float64), ref["throughput"].astype(np.float64)
+    close = (np.abs(e_g - e_r) <= 1e-4 * np.abs(e_r) + 1e-6).all(1) & \
+            (np.abs(t_g - t_r) <= 1e-4 * np.abs(t_r) + 1e-6).all(1)
+    same = (got["bounce"] == ref["bounce"]) & (got["dim"] == ref["dim"]) & \
+           (got["lightDim"] == ref["lightDim"]) & (got["flags"] == ref["flags"])
+    return close & same
+
+
+def check_paths(got: np.ndarray, ref: np.ndarray, min_frac: float = 0.99) -> float:
+    """The bar of the MIS parity tests: at least `min_frac` of the paths agree (agree_mask), the
+    mean estimate within 2e-3 relative, every estimate finite and no path left active."""
+    ok = agree_mask(got, ref)
+    print(f"paths agreeing: {ok.mean():.5f} of {len(ok)}", flush=True)
+    assert ok.mean() >= min_frac
+    mg = got["estimate"].astype(np.float64).mean(0)
+    mr = ref["estimate"].astype(np.float64).mean(0)
+    assert np.allclose(mg, mr, rtol=2e-3, atol=1e-7), (mg, mr)
+    assert np.all(np.isfinite(got["estimate"]))
+    assert not np.any(got["flags"] & abi.GZ_PATH_ACTIVE)
+    return float(ok.mean())
+
+
+def run_gpu_and_oracle(bh, paths, opt, sky=None, materials=None, lights=None, launch=None, integ=None):
+    """Traces `paths` (GzPath records) on the device with the MIS integrator and on the CPU with
+    the oracle, from the same starting records. Returns (device records, oracle records)."""
+    from gonzales_amd.integrator import VolumePathIntegrator
+    if integ is None:
+        bh.upload_to_gpu()
+        integ = VolumePathIntegrator(bh, opt, sky=sky, materials=materials, lights=lights)
+    if launch is not None:
+        integ.set_launch(*launch)
+    d = to_device(paths)
+    integ.trace(d)
+    got = to_host(d)
+    bh.check_traversal()
+    ref = paths.copy()
+    om = OracleMis(bh, sky=sky, materials=integ.materials, lights=integ.lights)
+    om.trace(ref, oracle_params(om, opt))
+    return got, ref
+
+
+def oracle_render(bh, camera, opt, materials, lights, sky=None, start=None, nthreads: int = 8) -> np.ndarray:
+    """The oracle chain of a whole frame or crop: camera paths (oracle.camera_paths, unless the
+    starting records `start` are given) -> every MIS bounce (oracle.mis_trace) -> film sums
+    (oracle.film_add) -> resolve. Returns (res_y, res_x, 3) float32 (pixels outside the crop 0)."""
+    c = camera.c
+    n = camera.path_count
+    table = oracle.sobol_matrices()
+    paths = oracle.camera_paths(c, 0, n, abi.GZ_PATH, table) if start is None else np.array(start, copy=True)
+    om = OracleMis(bh, sky=sky, materials=materials, lights=lights)
+    om.trace(paths, oracle_params(om, opt), nthreads)
+    px = c.resX * c.resY
+    light, albedo, normal = (np.zeros(3 * px, np.float32) for _ in range(3))
+    weight = np.zeros(px, np.float32)
+    f = camera.film
+    oracle.film_add(c, paths, f.iso, f.max_component_value, light, weight, albedo, normal)
+    img, bad = oracle.film_resolve(light, weight)
+    assert bad == 0
+    return img.reshape(c.resY, c.resX, 3)
+
+
+def check_image(img: np.ndarray, ref: np.ndarray, crop=None, rel: float = 1e-3, min_frac: float = 0.97) -> float:
+    """Image bar against the oracle chain: at least `min_frac` of the crop's pixels within `rel`
+    relative (+1e-5 absolute) in every channel, the crop's mean within 2e-3 relative."""
+    img = np.asarray(img, np.float64)
+    ref = np.asarray(ref, np.float64)
+    if crop is not None:
+        x0, x1, y0, y1 = crop
+        img, ref = img[y0:y1, x0:x1], ref[y0:y1, x0:x1]
+    assert np.all(np.isfinite(img))
+    close = (np.abs(img - ref) <= rel * np.abs(ref) + 1e-5).all(-1)
+    print(f"pixels agreeing: {close.mean():.4f} of {close.size}", flush=True)
+    assert close.mean() >= min_frac
+    mi, mr = img.reshape(-1, 3).mean(0), ref.reshape(-1, 3).mean(0)
+    assert np.allclose(mi, mr, rtol=2e-3, atol=1e-7), (mi, mr)
+    return float(close.mean())
+
+
+def conductor_strips_under_light(n: int = 8, roughness=None, radiance=(6.0, 5.0, 4.0)):
+    """n conductor strips (as conductor_strips_scene) lit by one triangle area light above them,
+    facing down, and no sky. Every strip point sees the whole light and nothing else: the light
+    strategy's directions do not depend on the roughness, and the BSDF strategy's rays toward the
+    light carry MIS weight 0 (the reference's triangle light pdf of a BSDF direction is 0), so the
+    image is a smooth function of the roughness with no visibility edge, and its derivative goes
+    through the BSDF value and the power heuristic's BSDF pdf. Returns (hierarchy, GzMaterial)."""
+    from gonzales_amd.integrator import conductor_material
+    rough = np.linspace(0.1, 0.5, n) if roughness is None else np.asarray(roughness)
+    verts, idx = [], []
+    for i in range(n):
+        x0, x1 = -2 + 4 * i / n, -2 + 4 * (i + 1) / n
+        v, q = quad([x0, 0, -2], [x0, 0, 2], [x1, 0, 2], [x1, 0, -2])
+        idx.append(q + 4 * i)
+        verts.append(v)
+    tri = np.array([[-1.5, 2.0, -1.0], [1.5, 2.0, -1.2], [0.0, 2.0, 1.5]], np.float32)
+    nrm = np.cross(tri[1] - tri[0], tri[2] - tri[0])
+    if nrm[1] > 0:
+        tri = tri[[0, 2, 1]]
+    meshes = [TriangleMesh.from_xyz(np.concatenate(verts), np.concatenate(idx)),
+              TriangleMesh.from_xyz(tri, np.arange(3, dtype=np.int64))]
+    prims = Primitives.concat([Primitives.geometric(0, 2 * n, np.repeat(np.arange(n), 2)),
+                               Primitives.area_lights(1, 1, 0)])
+    gz = abi.concat([conductor_material(roughness=float(r)) for r in rough], abi.GZ_MATERIAL)
+    scene = Scene(meshes, _materials([(0.5, 0.5, 0.5)] * n), light_emission=np.array([radiance], np.float32))
+    return build_bounding_hierarchy(scene, prims), gz

@@ -104,7 +104,20 @@ def test_batch_rejects_bad_input(gpu):
         assert abi.load().gz_batch_set_launch(batch.handle, ctypes.byref(c)) == abi.GZ_E_INVALID
     info = batch.info()
     assert info.roundPaths <= 1 << 14
+    batch.render(use_graph=True)
+    batch.check()
+    # replacing a scene's MIS tables after the batch was created: the batch's copies of its table
+    # pointers are stale, so the batch refuses to render until it is created again
+    integ0 = sc[0][0]
+    abi.check(abi.load().gz_gpu_set_mis_scene(integ0.bh.handle, abi.ptr(integ0.materials), len(integ0.materials),
+                                              abi.ptr(integ0.lights), len(integ0.lights), None))
+    with pytest.raises(abi.GzError, match="replaced"):
+        batch.render(use_graph=True)
     batch.close()
+    again = SceneBatch(entries, round_paths=1 << 14)
+    again.render(use_graph=True)
+    again.check()
+    again.close()
     # slots for the largest round only: one round holds every path of a small batch
     big = SceneBatch(entries, round_paths=1 << 30)
     info = big.info()

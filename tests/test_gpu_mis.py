@@ -14,61 +14,13 @@ import pytest
 from gonzales_amd import abi, scenes
 from gonzales_amd.integrator import MisOptions, VolumePathIntegrator, new_paths, pinhole_rays
 
-from mis_helpers import OracleMis, box_on_plane_scene, light_over_plane, point_paths
+from mis_helpers import (OracleMis, agree_mask, box_on_plane_scene, check_paths, light_over_plane, point_paths,
+                         run_gpu_and_oracle, to_device, to_host)
 
 pytestmark = pytest.mark.gpu
 
 
-def _to_device(paths):
-    import torch
-    return torch.from_numpy(np.ascontiguousarray(paths).view(np.uint8).copy()).cuda()
-
-
-def _to_host(t):
-    import torch
-    torch.cuda.synchronize()
-    return t.cpu().numpy().view(abi.GZ_PATH)
-
-
-def _oracle_params(om, opt: MisOptions):
-    return om.params(opt.max_depth, fix=opt.fix_light_pdf, fix_sky=opt.fix_sky_direction, seed=opt.zsobol_seed,
-                     log2spp=opt.log2_spp, digits=opt.base4_digits, rr_seed=opt.rr_seed)
-
-
-def _agree(got, ref):
-    e_g, e_r = got["estimate"].astype(np.float64), ref["estimate"].astype(np.float64)
-    t_g, t_r = got["throughput"].astype(np.float64), ref["throughput"].astype(np.float64)
-    close = (np.abs(e_g - e_r) <= 1e-4 * np.abs(e_r) + 1e-6).all(1) & \
-            (np.abs(t_g - t_r) <= 1e-4 * np.abs(t_r) + 1e-6).all(1)
-    same = (got["bounce"] == ref["bounce"]) & (got["dim"] == ref["dim"]) & \
-           (got["lightDim"] == ref["lightDim"]) & (got["flags"] == ref["flags"])
-    return close & same
-
-
-def _run_both(bh, paths, opt: MisOptions, sky=None, materials=None, lights=None, launch=None):
-    bh.upload_to_gpu()
-    integ = VolumePathIntegrator(bh, opt, sky=sky, materials=materials, lights=lights)
-    if launch is not None:
-        integ.set_launch(*launch)
-    d = _to_device(paths)
-    integ.trace(d)
-    got = _to_host(d)
-    bh.check_traversal()
-    ref = paths.copy()
-    om = OracleMis(bh, sky=sky, materials=integ.materials, lights=integ.lights)
-    om.trace(ref, _oracle_params(om, opt))
-    return got, ref
-
-
-def _check(got, ref, min_frac=0.99):
-    ok = _agree(got, ref)
-    print(f"paths agreeing: {ok.mean():.5f} of {len(ok)}", flush=True)
-    assert ok.mean() >= min_frac
-    mg = got["estimate"].astype(np.float64).mean(0)
-    mr = ref["estimate"].astype(np.float64).mean(0)
-    assert np.allclose(mg, mr, rtol=2e-3, atol=1e-7), (mg, mr)
-    assert np.all(np.isfinite(got["estimate"]))
-    assert not np.any(got["flags"] & abi.GZ_PATH_ACTIVE)
+_to_device, _to_host, _agree, _check, _run_both = to_device, to_host, agree_mask, check_paths, run_gpu_and_oracle
 
 
 def test_cornell_box_matches_oracle(gpu):
@@ -183,6 +135,18 @@ def test_errors_are_reported(gpu):
     # no light at all
     rc = lib.gz_gpu_set_mis_scene(bh.handle, abi.ptr(integ.materials), len(integ.materials), None, 0, None)
     assert rc == abi.GZ_E_INVALID
+    # interface materials (materialIndex < 0 on a surface) are refused: the reference's integrator
+    # asserts a material at every surface hit (VolumePathIntegrator.swift:341-342)
+    from gonzales_amd.accelerator import BoundingHierarchy
+    ids = bh.prim_ids.copy()
+    surf = np.nonzero(ids["type"] != 3)[0]
+    ids["materialIndex"][surf[0]] = -1
+    ib = BoundingHierarchy(bh.scene, ids, bh.nodes)
+    ib.upload_to_gpu()
+    rc = lib.gz_gpu_set_mis_scene(ib.handle, abi.ptr(integ.materials), len(integ.materials), abi.ptr(integ.lights),
+                                  len(integ.lights), None)
+    assert rc == abi.GZ_E_INVALID and "interface" in abi.last_error()
+    ib.free()
     # bad parameters; the scene set before the failed calls stays usable
     prm = MisOptions(max_depth=-1).params()
     assert lib.gz_gpu_mis_trace_device(bh.handle, abi.ptr(d), 16, ctypes.byref(prm), None) == abi.GZ_E_INVALID

@@ -204,6 +204,20 @@ def test_stack_overflow_is_reported_not_faulted(gpu):
     with pytest.raises(abi.GzError) as e:
         bh.intersect_batch_gpu(rays, tmax)
     assert "overflow" in str(e.value)
+    # device launches: the flag belongs to the stream that launched the overflowing traversal,
+    # so a check on another stream neither reports nor clears it
+    import torch
+    lib = abi.load()
+    d_rays = torch.from_numpy(rays.view(np.uint8).copy()).cuda()
+    d_tmax = torch.from_numpy(tmax).cuda()
+    d_hits = torch.empty(64 * 16, dtype=torch.uint8, device="cuda")
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    abi.check(lib.gz_gpu_traverse_closest_device(bh.handle, abi.ptr(d_rays), abi.ptr(d_tmax), 64, abi.ptr(d_hits),
+                                                 abi.stream_ptr(s1)))
+    assert lib.gz_gpu_check_traversal(bh.handle, abi.stream_ptr(s2)) == abi.GZ_OK
+    assert lib.gz_gpu_check_traversal(bh.handle, abi.stream_ptr(s1)) == abi.GZ_E_OVERFLOW
+    assert lib.gz_gpu_check_traversal(bh.handle, abi.stream_ptr(s1)) == abi.GZ_OK  # cleared by the read
     bh.free()
 
 
