@@ -157,6 +157,9 @@ SIGNATURES = {
     "gz_gpu_traverse_batch": (None, [_P, _P, _P, _I64, _P]),
     "gz_gpu_shade_batch": (None, [_P, _P, _I64, _P]),
     "gz_gpu_free_scene": (None, [_P]),
+    "gz_traverse_bvh2": (None, [_P, _P, ctypes.c_float, _P]),
+    "gz_cpu_traverse_batch": (None, [_P, _P, _P, _I64, _P]),
+    "gz_cpu_shade_batch": (None, [_P, _I64, _P, _P, _P]),
     "gz_last_error": (ctypes.c_char_p, []),
     "gz_gpu_scene_info": (ctypes.c_int, [_P, _P]),
     "gz_gpu_occluded_batch": (ctypes.c_int, [_P, _P, _P, _I64, _P]),

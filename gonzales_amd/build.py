@@ -25,7 +25,7 @@ ARCH = os.environ.get("GZ_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["gz_scene.hip", "gz_traverse.hip", "gz_shade.hip", "gz_zsobol.hip", "gz_mis.hip", "gz_film.hip",
                "gz_batch.hip", "gz_build.hip"]
-CXX_SOURCES = ["bvh_build.cpp", "sobol_table.cpp", "gz_ply.cpp"]
+CXX_SOURCES = ["gz_error.cpp", "bvh_build.cpp", "sobol_table.cpp", "gz_ply.cpp", "gz_cpu.cpp"]
 
 # Device code: no fused multiply-add contraction (the traversal is bit-exact against the
 # reference's separately rounded arithmetic), IEEE division/sqrt (hipcc's default), and the
@@ -107,6 +107,24 @@ def build_library(verbose: bool = False) -> Path:
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
+
+
+SANITIZE = ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-g"]
+HOST_ASAN_LIB = BUILD / "asan" / "libgz_host_asan.so"
+
+
+def build_host_sanitized() -> Path:
+    """The host-only sources (CXX_SOURCES: error state, SAH builder, Sobol table, PLY reader, CPU
+    traversal and shade) with AddressSanitizer and UndefinedBehaviorSanitizer, as one shared
+    library for the CPU robustness tests (loaded into a process that preloads libasan)."""
+    out = HOST_ASAN_LIB
+    out.parent.mkdir(parents=True, exist_ok=True)
+    _gen_sobol_data()
+    cxx = shutil.which("g++") or "g++"
+    flags = [f for f in CXX_FLAGS if f != "-O2"] + ["-O1", *SANITIZE]
+    _run([cxx, *flags, f"-I{INCLUDE}", f"-I{CSRC}", f"-I{BUILD}", "-shared", "-o", str(out),
+          *[str(CSRC / s) for s in CXX_SOURCES], "-lpthread"])
+    return out
 
 
 def build_oracle() -> Path:

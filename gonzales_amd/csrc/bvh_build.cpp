@@ -22,6 +22,13 @@
 
 #include "gonzales_amd.h"
 
+namespace gz {
+void set_error(const char *fmt, ...);
+void clear_error();
+void alloc_fault_arm();
+bool alloc_fault(const char *what);
+}  // namespace gz
+
 namespace {
 
 struct B3 {
@@ -243,11 +250,20 @@ struct Builder {
 }  // namespace
 
 extern "C" int64_t gz_bvh2_build(const float *primBounds, int64_t n, BVH2Node *nodesOut, int64_t *primOrderOut) {
-        if (n < 0 || (n > 0 && (!primBounds || !nodesOut || !primOrderOut))) return GZ_E_INVALID;
+        gz::clear_error();
+        gz::alloc_fault_arm();
+        if (n < 0 || (n > 0 && (!primBounds || !nodesOut || !primOrderOut))) {
+                gz::set_error("gz_bvh2_build: bad arguments");
+                return GZ_E_INVALID;
+        }
         if (n == 0) return 0;
-        if (n > (int64_t)INT32_MAX / 2) return GZ_E_INVALID;  // node offsets are int32
+        if (n > (int64_t)INT32_MAX / 2) {  // node offsets are int32
+                gz::set_error("gz_bvh2_build: %lld primitives exceed the int32 node offsets", (long long)n);
+                return GZ_E_INVALID;
+        }
         try {
                 Builder b;
+                if (gz::alloc_fault("SAH build primitives")) return GZ_E_ALLOC;
                 b.prims.resize((size_t)n);
                 for (int64_t i = 0; i < n; ++i) {
                         Prim &p = b.prims[(size_t)i];
@@ -266,10 +282,14 @@ extern "C" int64_t gz_bvh2_build(const float *primBounds, int64_t n, BVH2Node *n
                 b.nodes = nodesOut;
                 b.capacity = 2 * n - 1;
                 b.build(0, n);
-                if (b.failed) return GZ_E_INVALID;
+                if (b.failed) {
+                        gz::set_error("gz_bvh2_build: more nodes than 2n - 1 (non-finite bounds?)");
+                        return GZ_E_INVALID;
+                }
                 for (int64_t k = 0; k < n; ++k) primOrderOut[k] = b.prims[(size_t)k].index;
                 return b.nodeCount;
         } catch (const std::bad_alloc &) {
+                gz::set_error("gz_bvh2_build: out of host memory");
                 return GZ_E_ALLOC;
         }
 }

@@ -110,6 +110,11 @@ struct Scene {
 void set_error(const char *fmt, ...);
 void clear_error();
 bool hip_ok(hipError_t e, const char *what);
+// Allocation-failure injection (gz_error.cpp): alloc_fault_arm() at the start of an entry point
+// reads GZ_FAIL_ALLOC_AFTER; alloc_fault(what) before each allocation returns true (error set) on
+// the armed one.
+void alloc_fault_arm();
+bool alloc_fault(const char *what);
 
 // Makes the scene's device current; returns false (error set) on failure.
 bool enter_device(Scene *s);

@@ -21,19 +21,6 @@
 
 namespace gz {
 
-static thread_local std::string t_error;
-
-void set_error(const char *fmt, ...) {
-        char buf[1024];
-        va_list ap;
-        va_start(ap, fmt);
-        vsnprintf(buf, sizeof buf, fmt, ap);
-        va_end(ap);
-        t_error = buf;
-}
-
-void clear_error() { t_error.clear(); }
-
 bool hip_ok(hipError_t e, const char *what) {
         if (e == hipSuccess) return true;
         set_error("%s: %s", what, hipGetErrorString(e));
@@ -65,7 +52,7 @@ StreamSlot *stream_slot(Scene *s, hipStream_t stream) {
         if (it != s->slots.end()) return &it->second;
         // 64 bytes: the flag at 0, the counter at 8 (zeroed before any launch can use them)
         void *p = nullptr;
-        if (!hip_ok(hipMalloc(&p, 64), "hipMalloc(stream slot)")) return nullptr;
+        if (alloc_fault("stream slot") || !hip_ok(hipMalloc(&p, 64), "hipMalloc(stream slot)")) return nullptr;
         if (!hip_ok(hipMemset(p, 0, 64), "hipMemset(stream slot)")) {
                 (void)hipFree(p);
                 return nullptr;
@@ -117,7 +104,7 @@ static void free_scene(Scene *s) {
 template <class T>
 static bool upload(T **dst, const T *src, int64_t count, Scene *s, const char *what) {
         size_t bytes = sizeof(T) * (size_t)(count > 0 ? count : 1);
-        if (!hip_ok(hipMalloc((void **)dst, bytes), what)) return false;
+        if (alloc_fault(what) || !hip_ok(hipMalloc((void **)dst, bytes), what)) return false;
         s->deviceBytes += (int64_t)bytes;
         if (count > 0 && !hip_ok(hipMemcpy(*dst, src, sizeof(T) * (size_t)count, hipMemcpyHostToDevice), what))
                 return false;
@@ -186,8 +173,6 @@ static inline float as_float(int32_t i) {
 
 using namespace gz;
 
-extern "C" const char *gz_last_error(void) { return t_error.c_str(); }
-
 extern "C" bool gz_gpu_available(void) {
         int n = 0;
         if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return false;
@@ -203,6 +188,7 @@ extern "C" void *gz_gpu_upload_scene(const BVH2Node *nodes, int64_t nNodes, cons
                                      const int64_t *faceCounts, const int64_t *vtxCounts, const Material_C *materials,
                                      int64_t nMaterials) {
         clear_error();
+        alloc_fault_arm();
         if (nNodes < 0 || nPrims < 0 || nMeshes < 0 || nMaterials < 0) {
                 set_error("gz_gpu_upload_scene: negative count");
                 return nullptr;
@@ -240,6 +226,7 @@ extern "C" void *gz_gpu_upload_scene(const BVH2Node *nodes, int64_t nNodes, cons
         std::vector<NodePair> pairs;
         std::vector<TriRec> tris;
         try {
+                if (alloc_fault("host node and triangle records")) throw std::bad_alloc();
                 if (nNodes > 0) {
                         // Depth-first walk from node 0 (the reference's traversal start). Record ids are
                         // assigned on first visit; a node reachable twice (a DAG) keeps one record.
@@ -370,7 +357,7 @@ extern "C" void *gz_gpu_upload_scene(const BVH2Node *nodes, int64_t nNodes, cons
                         }
                 }
         } catch (const std::bad_alloc &) {
-                set_error("gz_gpu_upload_scene: out of host memory");
+                if (gz_last_error()[0] == 0) set_error("gz_gpu_upload_scene: out of host memory");
                 free_scene(s);
                 return nullptr;
         }

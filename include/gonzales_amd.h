@@ -184,6 +184,30 @@ void gz_gpu_shade_batch(void *handle, struct PathState_C *paths, int64_t count,
 void gz_gpu_free_scene(void *handle);
 
 /* ---------------------------------------------------------------------------------------- */
+/* Entry points that replace the reference's CPU half of mojo_kernel.h (host C++, no device). */
+/* Re-entrant, no global state: callers run them concurrently, one per tile task, as           */
+/* TileRenderer.swift:77-107 does. The 64-entry traversal stack is checked (the reference's is   */
+/* not, kernel.mojo:250): a ray that would overflow it stops with the closest hit found so far   */
+/* and gz_last_error() on the calling thread says so; otherwise gz_last_error() is "".          */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Replaces mojo_traverse_bvh2 (mojo_kernel.h:60-62; kernel.mojo:369-373): closest hit of one ray
+ * over the BVH2 / PrimId / TriangleMesh_C arrays the descriptor points at (traverse_bvh2_core,
+ * kernel.mojo:223-364). A descriptor with no nodes (bvh2Nodes NULL) gives a miss. */
+void gz_traverse_bvh2(const struct SceneDescriptor2_C *scene, const struct Ray_C *ray, float tMax,
+                      struct Intersection_C *result);
+
+/* Replaces mojo_cpu_traverse_batch (mojo_kernel.h:84-85; kernel.mojo:378-390): the same for
+ * `count` rays with per-ray tMax, one after another on the calling thread. */
+void gz_cpu_traverse_batch(const struct SceneDescriptor2_C *scene, const struct Ray_C *rays,
+                           const float *tMaxValues, int64_t count, struct Intersection_C *results);
+
+/* Replaces mojo_cpu_shade_batch (mojo_kernel.h:87-89; kernel.mojo:393-403): the diffuse shade
+ * step of Tile.render (shade_core, kernel.mojo:676-773) for `count` paths, in place. */
+void gz_cpu_shade_batch(struct PathState_C *paths, int64_t count, const struct Intersection_C *intersections,
+                        const struct TriangleMesh_C *meshes, const struct Material_C *materials);
+
+/* ---------------------------------------------------------------------------------------- */
 /* New entry points (status-returning).                                                        */
 /* ---------------------------------------------------------------------------------------- */
 

@@ -252,3 +252,46 @@ def test_concurrent_callers_share_one_handle(gpu):
     for k in range(len(batches)):
         assert out[k].tobytes() == ref[k].tobytes()
     bh.free()
+
+
+def test_injected_allocation_failures_leave_no_handle(gpu, monkeypatch):
+    """GZ_FAIL_ALLOC_AFTER=k fails the k-th allocation of gz_gpu_upload_scene (host records,
+    device buffers, the stream slot): every k gives NULL with the injected error and frees what
+    was allocated before it; device memory in use returns to where it was."""
+    import torch
+    bh = random_soup_hierarchy(3000, 12)
+    bh.prepare_materials()
+    lib = abi.load()
+    meshes_c = bh.scene.meshes_c()
+    pts = np.array([m.points.size for m in bh.scene.meshes], np.int64)
+    faces = np.array([m.face_indices.size for m in bh.scene.meshes], np.int64)
+    vtx = np.array([m.vertex_indices.size for m in bh.scene.meshes], np.int64)
+
+    def upload():
+        return lib.gz_gpu_upload_scene(abi.ptr(bh.nodes), len(bh.nodes), abi.ptr(bh.prim_ids), len(bh.prim_ids),
+                                       ctypes.addressof(meshes_c), len(bh.scene.meshes), abi.ptr(pts),
+                                       abi.ptr(faces), abi.ptr(vtx), abi.ptr(bh.materials_c), len(bh.materials_c))
+
+    lib.gz_gpu_free_scene(upload())  # first use: the runtime's own one-time allocations
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    failed = 0
+    for k in range(200):
+        monkeypatch.setenv("GZ_FAIL_ALLOC_AFTER", str(k))
+        h = upload()
+        if h:
+            lib.gz_gpu_free_scene(h)
+            break
+        assert "injected allocation failure" in abi.last_error(), abi.last_error()
+        failed += 1
+    monkeypatch.delenv("GZ_FAIL_ALLOC_AFTER")
+    assert failed >= 8  # host records, nodes, triangles, primIds, materials, 2 meshes x 3 arrays, slot
+    torch.cuda.synchronize()
+    assert abs(torch.cuda.mem_get_info()[0] - free0) <= 4 << 20
+    h = upload()
+    assert h
+    rays, tmax = random_rays(2000, 13)
+    out = np.zeros(len(rays), abi.INTERSECTION)
+    lib.gz_gpu_traverse_batch(h, abi.ptr(rays), abi.ptr(tmax), len(rays), abi.ptr(out))
+    assert_isect_equal(out, OracleView(bh).closest(rays, tmax)[0])
+    lib.gz_gpu_free_scene(h)
