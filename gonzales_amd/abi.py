@@ -69,6 +69,7 @@ GZ_E_HIP = -3
 GZ_E_OVERFLOW = -4
 GZ_E_ALLOC = -5
 GZ_STACK_CAPACITY = 64
+GZ_NOALLOC_MAX_RAYS = 524288
 
 
 class TriangleMeshC(ctypes.Structure):
@@ -163,6 +164,7 @@ SIGNATURES = {
     "gz_last_error": (ctypes.c_char_p, []),
     "gz_gpu_scene_info": (ctypes.c_int, [_P, _P]),
     "gz_gpu_occluded_batch": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
+    "gz_gpu_traverse_batch_noalloc": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
     "gz_gpu_traverse_closest_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
     "gz_gpu_traverse_any_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
     "gz_gpu_traverse_closest_stats_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P]),

@@ -224,15 +224,9 @@ static void launch_lds(Scene *s, const StreamSlot &slot, const Ray_C *d_rays, co
                 launch_variant<kAny, kStats, 0>(s, slot, d_rays, d_tMax, n, d_hits, d_occ, d_stats, stream);
 }
 
-int launch_trace(Scene *s, const Ray_C *d_rays, const float *d_tMax, int64_t n, GzHit *d_hits, uint8_t *d_occ,
-                 GzTraceStats *d_stats, bool anyHit, hipStream_t stream) {
-        if (n <= 0) return GZ_OK;
-        if (n > (int64_t)kBlock * 0x7fffffffLL) {
-                set_error("traversal batch of %lld rays is too large", (long long)n);
-                return GZ_E_INVALID;
-        }
-        const StreamSlot *slot = stream_slot(s, stream);
-        if (!slot) return GZ_E_ALLOC;
+// The launches of one traversal with the error flag and ray counter of `slot`.
+static int launch_trace_slot(Scene *s, const StreamSlot *slot, const Ray_C *d_rays, const float *d_tMax, int64_t n,
+                             GzHit *d_hits, uint8_t *d_occ, GzTraceStats *d_stats, bool anyHit, hipStream_t stream) {
         if (s->empty) {
                 hipLaunchKernelGGL(k_miss, dim3(grid_for(n)), dim3(kBlock), 0, stream, d_tMax, n,
                                    anyHit ? nullptr : d_hits, anyHit ? d_occ : nullptr, d_stats);
@@ -248,6 +242,21 @@ int launch_trace(Scene *s, const Ray_C *d_rays, const float *d_tMax, int64_t n, 
                         launch_lds<false, false>(s, *slot, d_rays, d_tMax, n, d_hits, nullptr, nullptr, stream);
         }
         return hip_ok(hipGetLastError(), "traversal launch") ? GZ_OK : GZ_E_HIP;
+}
+
+static bool trace_size_ok(int64_t n) {
+        if (n <= (int64_t)kBlock * 0x7fffffffLL) return true;
+        set_error("traversal batch of %lld rays is too large", (long long)n);
+        return false;
+}
+
+int launch_trace(Scene *s, const Ray_C *d_rays, const float *d_tMax, int64_t n, GzHit *d_hits, uint8_t *d_occ,
+                 GzTraceStats *d_stats, bool anyHit, hipStream_t stream) {
+        if (n <= 0) return GZ_OK;
+        if (!trace_size_ok(n)) return GZ_E_INVALID;
+        const StreamSlot *slot = stream_slot(s, stream);
+        if (!slot) return GZ_E_ALLOC;
+        return launch_trace_slot(s, slot, d_rays, d_tMax, n, d_hits, d_occ, d_stats, anyHit, stream);
 }
 
 int launch_expand(Scene *s, const GzHit *d_hits, const float *d_tMax, int64_t n, Intersection_C *d_out,
@@ -406,6 +415,112 @@ extern "C" int gz_gpu_occluded_batch(void *handle, const Ray_C *rays, const floa
         rc = check_overflow_sync(s);
         if (rc != GZ_OK) return rc;
         if (!hip_ok(hipMemcpyAsync(occluded, dO, n, hipMemcpyDeviceToHost, st), "copy occluded") ||
+            !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+                return GZ_E_HIP;
+        return GZ_OK;
+}
+
+// ---- Allocation-free host-buffer batch ----------------------------------------------------------
+//
+// gz_gpu_traverse_batch_noalloc stages its rays, tMax, hits and records, its error flag and the
+// persistent kernel's ray counter in `scratch`, a file-scope device array of fixed size that every
+// GPU the library runs on holds (its own copy, at its own address) until the process ends. In host
+// code `scratch` is a stand-in for the symbol, not a device address: it is never passed to a
+// kernel or a copy. hipGetSymbolAddress gives each device's address once, with that device
+// current, and only that address is used. Layout (each part 256-byte aligned):
+//   [0, 256) : the error flag (int at 0) and the ray counter (u64 at 8)
+//   rays, tMax, hits, Intersection_C records for up to GZ_NOALLOC_MAX_RAYS rays
+namespace gz {
+
+static constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct NoAllocLayout {
+        size_t rays, tMax, hits, records, end;
+};
+
+static constexpr NoAllocLayout noalloc_layout(size_t n) {
+        const size_t r = 256;
+        const size_t t = align256(r + n * sizeof(Ray_C));
+        const size_t h = align256(t + n * sizeof(float));
+        const size_t i = align256(h + n * sizeof(GzHit));
+        return NoAllocLayout{r, t, h, i, i + n * sizeof(Intersection_C)};
+}
+
+constexpr size_t kNoAllocScratchBytes = align256(noalloc_layout((size_t)GZ_NOALLOC_MAX_RAYS).end);
+static_assert(kNoAllocScratchBytes <= (size_t)48 << 20, "scratch stays within 48 MiB");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex gScratchMu[kMaxDevices];   // one call at a time per device owns its scratch
+static void *gScratchPtr[kMaxDevices];       // device address of `scratch` per device, or null
+
+__global__ void k_clear_flag(int *flag) { *flag = 0; }
+
+// The device address of `scratch` on the current device `dev` (the caller holds gScratchMu[dev]).
+static void *scratch_address(int dev) {
+        if (!gScratchPtr[dev]) {
+                void *p = nullptr;
+                if (!hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)") || !p)
+                        return nullptr;
+                gScratchPtr[dev] = p;
+        }
+        return gScratchPtr[dev];
+}
+
+}  // namespace gz
+
+extern "C" int gz_gpu_traverse_batch_noalloc(void *handle, const Ray_C *rays, const float *tMaxValues, int64_t count,
+                                             Intersection_C *results) {
+        clear_error();
+        Scene *s = scene_of(handle, "gz_gpu_traverse_batch_noalloc");
+        if (!s) return GZ_E_INVALID;
+        if (count < 0 || (count > 0 && (!rays || !tMaxValues || !results))) {
+                set_error("gz_gpu_traverse_batch_noalloc: bad arguments");
+                return GZ_E_INVALID;
+        }
+        if (count > GZ_NOALLOC_MAX_RAYS) {
+                set_error("gz_gpu_traverse_batch_noalloc: %lld rays exceed the fixed staging of %d rays",
+                          (long long)count, GZ_NOALLOC_MAX_RAYS);
+                return GZ_E_INVALID;
+        }
+        if (count == 0) return GZ_OK;
+        if (s->device < 0 || s->device >= kMaxDevices) {
+                set_error("gz_gpu_traverse_batch_noalloc: device %d is beyond the %d this entry serves", s->device,
+                          kMaxDevices);
+                return GZ_E_INVALID;
+        }
+        std::lock_guard<std::mutex> dlock(gScratchMu[s->device]);
+        std::lock_guard<std::mutex> lock(s->mu);
+        if (!enter_device(s)) return GZ_E_HIP;
+        char *base = static_cast<char *>(scratch_address(s->device));
+        if (!base) return GZ_E_HIP;
+        const size_t n = (size_t)count;
+        const NoAllocLayout L = noalloc_layout(n);
+        StreamSlot slot{reinterpret_cast<int *>(base), reinterpret_cast<unsigned long long *>(base + 8)};
+        Ray_C *dR = reinterpret_cast<Ray_C *>(base + L.rays);
+        float *dT = reinterpret_cast<float *>(base + L.tMax);
+        GzHit *dH = reinterpret_cast<GzHit *>(base + L.hits);
+        Intersection_C *dI = reinterpret_cast<Intersection_C *>(base + L.records);
+        hipStream_t st = s->stream;
+        hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, st, slot.dErr);
+        if (!hip_ok(hipGetLastError(), "clear error flag") ||
+            !hip_ok(hipMemcpyAsync(dR, rays, n * sizeof(Ray_C), hipMemcpyHostToDevice, st), "copy rays") ||
+            !hip_ok(hipMemcpyAsync(dT, tMaxValues, n * sizeof(float), hipMemcpyHostToDevice, st), "copy tMax"))
+                return GZ_E_HIP;
+        int rc = launch_trace_slot(s, &slot, dR, dT, count, dH, nullptr, nullptr, false, st);
+        if (rc != GZ_OK) return rc;
+        rc = launch_expand(s, dH, dT, count, dI, st);
+        if (rc != GZ_OK) return rc;
+        int flag = 0;
+        if (!hip_ok(hipMemcpyAsync(&flag, slot.dErr, sizeof(int), hipMemcpyDeviceToHost, st), "read error flag") ||
+            !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+                return GZ_E_HIP;
+        if (flag & 1) {
+                set_error("traversal stack overflow: a ray needed more than %d entries", GZ_STACK_CAPACITY);
+                return GZ_E_OVERFLOW;
+        }
+        if (!hip_ok(hipMemcpyAsync(results, dI, n * sizeof(Intersection_C), hipMemcpyDeviceToHost, st), "copy results") ||
             !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
                 return GZ_E_HIP;
         return GZ_OK;

@@ -225,6 +225,18 @@ int gz_gpu_scene_info(void *handle, struct GzSceneInfo *info);
 int gz_gpu_occluded_batch(void *handle, const struct Ray_C *rays, const float *tMaxValues,
                           int64_t count, uint8_t *occluded);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (serving
+ * loops and harnesses that forbid hipMalloc): gz_gpu_traverse_batch's closest hits, the same
+ * records, with every device buffer the call needs (rays, tMax, hits, records, the error flag and
+ * the ray counter) in a fixed 46 MiB array that the library holds on each GPU it runs on until
+ * the process ends, instead of the handle's grow-only staging. At most GZ_NOALLOC_MAX_RAYS rays
+ * per call: more is GZ_E_INVALID and nothing runs. Calls on one GPU share its array and run one
+ * at a time (a lock per GPU); the array keeps the last call's contents. Synchronous, host
+ * buffers; returns GZ_OK, GZ_E_OVERFLOW (no results written), GZ_E_INVALID or GZ_E_HIP. */
+#define GZ_NOALLOC_MAX_RAYS 524288
+int gz_gpu_traverse_batch_noalloc(void *handle, const struct Ray_C *rays, const float *tMaxValues,
+                                  int64_t count, struct Intersection_C *results);
+
 /* Device-resident closest hit: d_rays, d_tMax, d_hits are device pointers; asynchronous on
  * `stream`. The batch form behind mojo_gpu_traverse_batch, without the host round trip. */
 int gz_gpu_traverse_closest_device(void *handle, const struct Ray_C *d_rays, const float *d_tMax,

@@ -64,6 +64,7 @@ def test_device_entry_points_fail_cleanly_without_a_handle():
     assert "null scene handle" in abi.last_error()
     assert lib.gz_gpu_scene_info(None, None) == abi.GZ_E_INVALID
     lib.gz_gpu_traverse_batch(None, None, None, 10, None)  # NULL handle: no-op (kernel.mojo:616-618)
+    assert lib.gz_gpu_traverse_batch_noalloc(None, None, None, 10, None) == abi.GZ_E_INVALID
     lib.gz_gpu_shade_batch(None, None, 10, None)
     lib.gz_gpu_free_scene(None)
     c = abi.GzMisLaunch(1, 256, 1, 256)
