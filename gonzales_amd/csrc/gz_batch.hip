@@ -650,13 +650,14 @@ static bool ensure_wave(Batch *b) {
         return ok;
 }
 
-// Persistent grid of a batch kernel: gridScale x its resident blocks per CU x CUs.
+// Persistent grid of a batch kernel that takes `chunk`-entry chunks, one per wave:
+// gridScale x its resident blocks per CU x CUs, or fewer when a round has fewer chunks.
 template <typename K>
-static unsigned batch_grid(const Batch *b, K kern, int block) {
+static unsigned batch_grid(const Batch *b, K kern, int block, unsigned chunk) {
         int perCU = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, block, 0) != hipSuccess || perCU < 1) perCU = 1;
         const int64_t want = (int64_t)b->cuCount * perCU * b->launch.gridScale;
-        const int64_t need = ((int64_t)b->roundPaths + b->launch.chunk - 1) / b->launch.chunk * 64 / block + 1;
+        const int64_t need = ((int64_t)b->roundPaths + chunk - 1) / chunk * 64 / block + 1;
         return (unsigned)std::max<int64_t>(1, std::min(want, need));
 }
 
@@ -687,17 +688,19 @@ static bool enqueue_round(Batch *b, hipStream_t st) {
                 hipLaunchKernelGGL(k_bw_clear, dim3(1), dim3(256), 0, st, b->wctr);
                 auto kd = k_bw_phase<kBsdfDiffuse>;
                 auto ka = b->bsdfSet == kBsdfNoLayer ? k_bw_phase<kBsdfNoLayer> : k_bw_phase<kBsdfAll>;
-                const unsigned gt = batch_grid(b, k_bw_rays<0>, block),
+                // the ray kernels take 64-entry chunks; the shade and finish kernels chunks of at
+                // least 256 entries (their work per entry is short next to the per-chunk item lookup
+                // and queue appends)
+                const unsigned pchunk = std::max(256u, chunk);
+                const unsigned gt = batch_grid(b, k_bw_rays<0>, block, 64u),
                                gsplit = (unsigned)std::max<int64_t>(
                                        1, std::min<int64_t>((int64_t)b->cuCount * 8,
                                                             ((int64_t)b->roundPaths + kSplitTile - 1) / kSplitTile + 64)),
-                               gd = batch_grid(b, kd, block), ga = batch_grid(b, ka, block),
-                               gany = batch_grid(b, k_bw_rays<1>, block), gclose = batch_grid(b, k_bw_rays<2>, block),
-                               gfin = batch_grid(b, k_bw_finish, block);
+                               gd = batch_grid(b, kd, block, pchunk), ga = batch_grid(b, ka, block, pchunk),
+                               gany = batch_grid(b, k_bw_rays<1>, block, 64u),
+                               gclose = batch_grid(b, k_bw_rays<2>, block, 64u),
+                               gfin = batch_grid(b, k_bw_finish, block, pchunk);
                 const int *c = b->dCursor;
-                // the shade and finish kernels over chunks of at least 256 entries: their work per
-                // entry is short next to the per-chunk item lookup and queue appends
-                const unsigned pchunk = std::max(256u, chunk);
                 for (int level = 0; level < b->levels; ++level) {
                         unsigned *lbase = b->ctr.base + (size_t)level * (b->ctr.maxItems + 1);
                         if (drainLevel > 0 && level >= drainLevel) {

@@ -92,6 +92,49 @@ class DiffRenderer:
         return grad[:, :3].clone(), grad[:, 3].clone()
 
 
+def central_difference_check(make_renderer, albedo, roughness, target, rough_ids, albedo_ids, seeds, h=1e-3):
+    """Adjoint gradients against central differences (step h, common random numbers: the +h and -h
+    renders reuse every Z-Sobol index and roulette and BSDF stream) of the float64 L2 loss against
+    `target`, for renderers make_renderer(seed) over `seeds`. Parameters: roughness of the materials
+    rough_ids, reflectance channels albedo_ids [(material, channel)]. Returns per-seed arrays
+    (adjoint, fd), shape (len(seeds), len(rough_ids) + len(albedo_ids)), roughness first."""
+    import torch
+
+    def loss(img):
+        d = img.double() - target
+        return float((d * d).sum())
+
+    adj, fd = [], []
+    for seed in seeds:
+        r = make_renderer(seed)
+        r.set_parameters(albedo, roughness)
+        img = r.render()
+        ga, gr = r.backward(2 * (img.double() - target))
+        ga, gr = ga.cpu().numpy(), gr.cpu().numpy()
+        adj.append(np.concatenate([gr[list(rough_ids)], [ga[k, c] for k, c in albedo_ids]]))
+        row = []
+        for k in rough_ids:
+            v = []
+            for s in (1, -1):
+                rr = np.array(roughness, np.float32, copy=True)
+                rr[k] += s * h
+                r.set_parameters(albedo, rr)
+                v.append(loss(r.render()))
+            row.append((v[0] - v[1]) / (2 * h))
+        for k, c in albedo_ids:
+            v = []
+            for s in (1, -1):
+                aa = np.array(albedo, np.float32, copy=True)
+                aa[k, c] += s * h
+                r.set_parameters(aa, roughness)
+                v.append(loss(r.render()))
+            row.append((v[0] - v[1]) / (2 * h))
+        fd.append(np.array(row))
+        del r
+        torch.cuda.empty_cache()
+    return np.array(adj), np.array(fd)
+
+
 def render_function():
     """torch.autograd.Function: image = f(albedo (M, 3), roughness (M,)) for a DiffRenderer."""
     import torch
