@@ -8,9 +8,11 @@ Workload (one "step" = one launch of the closest-hit kernel over the whole batch
          (seed 3), tMax U[0.1, 2] (seed 4); inputs resident in HBM before timing starts.
 Prints ONE JSON line. ``--gpus N`` with N > 1 is not supported (one GPU per box).
 
-roofline.achieved = algorithmic bytes of the launch / average launch time, with the per-ray
-figure B = 28 (ray + tMax) + 16 (hit record) + 64 x interior visits + 40 x triangle tests,
-the visit and test counts taken from the counting variant of the same kernel over the same rays
+value = rays / the whole call's time (binning the batch by origin cell + the traversal kernel).
+roofline.achieved = algorithmic bytes of the traversal kernel / its average duration (HIP events
+the library records around it), with the per-ray figure B = 32 (the sorted ray record; 28 for
+ray + tMax when not binned) + 16 (hit record) + 64 x interior visits + 40 x triangle tests, the
+visit and test counts taken from the counting variant of the same kernel over the same rays
 (checked against the CPU oracle on the baseline sample). cpu_baseline = the CPU oracle (a port of
 the reference's mojo_cpu_traverse_batch) on the host cores, on a bounded sample of the same rays.
 
@@ -875,6 +877,9 @@ def main() -> int:
 
     best, timings = autotune(bh, d_rays, d_tmax, any_hit=False, sample=n, reps=2)
     log(f"autotuned launch config: {best}")
+    binned = best.bin_min_rays > 0 and n >= best.bin_min_rays
+    bh.set_trace_config(best.persistent, best.lds_stack, best.refill_lanes, best.grid_scale, best.chunk,
+                        best.bin_min_rays, record_times=1)
 
     # Work counts for the algorithmic bytes (untimed counting pass).
     bh.closest_device(d_rays, d_tmax, d_hits, d_stats)
@@ -882,7 +887,10 @@ def main() -> int:
     st = d_stats.view(torch.int32).view(n, 2).to(torch.int64)
     sum_interior = int(st[:, 0].sum())
     sum_tris = int(st[:, 1].sum())
-    algo_bytes = 44 * n + 64 * sum_interior + 40 * sum_tris
+    # per ray: its 24-byte Ray_C and 4-byte tMax (as a 32-byte sorted record when binned) read, the
+    # 16-byte hit written, 64 B per interior visit, 40 B per triangle test
+    ray_bytes = (32 if binned else 28) + 16
+    algo_bytes = ray_bytes * n + 64 * sum_interior + 40 * sum_tris
     hit_rate = float((d_hits.view(torch.int32).view(n, 4)[:, 0] >= 0).float().mean())
     log(f"interior/ray {sum_interior / n:.1f}, tris/ray {sum_tris / n:.2f}, bytes/ray {algo_bytes / n:.0f}, "
         f"hit rate {hit_rate:.3f}")
@@ -902,13 +910,29 @@ def main() -> int:
     bh.check_traversal()
     ms = ev0.elapsed_time(ev1) / args.steps
     mrays = n / (ms * 1e-3) / 1e6
-    achieved = algo_bytes / (ms * 1e-3) / 1e9
-    log(f"closest: {ms:.3f} ms/launch, {mrays:.1f} Mrays/s (wall {n * args.steps / wall / 1e6:.1f})")
+    # the traversal kernel alone (the roofline's kernel): HIP events the library records around the
+    # binning and the traversal launch on this stream, read after each of args.steps more launches
+    kbin, ktrace = [], []
+    for _ in range(args.steps):
+        bh.closest_device(d_rays, d_tmax, d_hits)
+        b_ms, t_ms = bh.trace_times()
+        kbin.append(b_ms)
+        ktrace.append(t_ms)
+    bh.check_traversal()
+    trace_ms = float(np.mean(ktrace))
+    bin_ms = float(np.mean(kbin))
+    achieved = algo_bytes / (trace_ms * 1e-3) / 1e9
+    log(f"closest: {ms:.3f} ms/launch, {mrays:.1f} Mrays/s (wall {n * args.steps / wall / 1e6:.1f}); "
+        f"binning {bin_ms:.3f} ms + traversal kernel {trace_ms:.3f} ms")
 
     extra = {"host_build_s": round(t_build, 3), "upload_s": round(t_upload, 3),
              "bytes_per_ray": round(algo_bytes / n, 1), "interior_per_ray": round(sum_interior / n, 2),
              "tris_per_ray": round(sum_tris / n, 3), "hit_rate": round(hit_rate, 4),
              "wall_mrays_per_s": round(n * args.steps / wall / 1e6, 2),
+             "binned": binned, "binning_ms": round(bin_ms, 4), "traversal_kernel_ms": round(trace_ms, 4),
+             # binning traffic per ray: the count kernel reads the 12-byte origin and writes a 2-byte key, the
+             # place kernel reads key, ray and tMax and writes the 32-byte sorted record
+             "binning_bytes_per_ray": (12 + 2 + 2 + 28 + 32) if binned else 0,
              "launch_config": describe({best: timings[best]})[0] if best in timings else None,
              "autotune_ms": describe(timings)}
 
@@ -940,7 +964,8 @@ def main() -> int:
         extra["coherent_closest_mrays_per_s"] = round(nc / (ms_c * 1e-3) / 1e6, 2)
         extra["coherent_anyhit_mrays_per_s"] = round(nc / (ms_ca * 1e-3) / 1e6, 2)
         extra["coherent_launch_config"] = describe({cbest: 0.0})[0]
-        bh.set_trace_config(best.persistent, best.lds_stack, best.refill_lanes, best.grid_scale, best.chunk)
+        bh.set_trace_config(best.persistent, best.lds_stack, best.refill_lanes, best.grid_scale, best.chunk,
+                            best.bin_min_rays, record_times=1)
         extra["device_build"] = device_build_leg(args, bh, d_rays, d_tmax, n, stream, mrays)
         # End to end through host buffers (H2D + trace + Intersection_C expansion + D2H), 1M rays.
         # (steady state: the first call grows the staging buffers and is not timed)

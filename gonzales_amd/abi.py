@@ -89,7 +89,8 @@ class GzSceneInfo(ctypes.Structure):
 
 class GzTraceConfig(ctypes.Structure):
     _fields_ = [("persistent", ctypes.c_int32), ("ldsStack", ctypes.c_int32), ("refillLanes", ctypes.c_int32),
-                ("gridScale", ctypes.c_int32), ("chunk", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("gridScale", ctypes.c_int32), ("chunk", ctypes.c_int32), ("binMinRays", ctypes.c_int32),
+                ("recordTimes", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class GzZSobolParams(ctypes.Structure):
@@ -165,6 +166,7 @@ SIGNATURES = {
     "gz_gpu_scene_info": (ctypes.c_int, [_P, _P]),
     "gz_gpu_occluded_batch": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
     "gz_gpu_traverse_batch_noalloc": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
+    "gz_gpu_trace_times": (ctypes.c_int, [_P, _P, _P, _P]),
     "gz_gpu_traverse_closest_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
     "gz_gpu_traverse_any_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
     "gz_gpu_traverse_closest_stats_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P]),

@@ -288,10 +288,18 @@ class BoundingHierarchy:
                                                       abi.ptr(occ_t), abi.ptr(stats_t), abi.stream_ptr(stream))
         abi.check(rc, "any_device")
 
-    def set_trace_config(self, persistent: int = 1, lds_stack: int = 16, refill_lanes: int = 16,
-                         grid_scale: int = 1, chunk: int = 128) -> None:
-        cfg = abi.GzTraceConfig(persistent, lds_stack, refill_lanes, grid_scale, chunk)
+    def set_trace_config(self, persistent: int = 3, lds_stack: int = 16, refill_lanes: int = 32,
+                         grid_scale: int = 1, chunk: int = 64, bin_min_rays: int = 1 << 18,
+                         record_times: int = 0) -> None:
+        cfg = abi.GzTraceConfig(persistent, lds_stack, refill_lanes, grid_scale, chunk, bin_min_rays, record_times)
         abi.check(abi.load().gz_gpu_set_trace_config(self.handle, ctypes.byref(cfg)), "gz_gpu_set_trace_config")
+
+    def trace_times(self, stream=None) -> tuple[float, float]:
+        """(binning ms, traversal kernel ms) of the last traversal on `stream` (needs record_times)."""
+        b, t = ctypes.c_float(), ctypes.c_float()
+        abi.check(abi.load().gz_gpu_trace_times(self.handle, abi.stream_ptr(stream), ctypes.byref(b), ctypes.byref(t)),
+                  "gz_gpu_trace_times")
+        return b.value, t.value
 
     def trace_config(self) -> abi.GzTraceConfig:
         cfg = abi.GzTraceConfig()

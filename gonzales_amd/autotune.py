@@ -18,22 +18,31 @@ from pathlib import Path
 
 @dataclass(frozen=True)
 class TraceConfig:
-    persistent: int = 1
+    persistent: int = 3
     lds_stack: int = 16
-    refill_lanes: int = 56
+    refill_lanes: int = 32
     grid_scale: int = 1
     chunk: int = 64
+    bin_min_rays: int = 1 << 18
 
 
+# binned by origin cell (the default bin_min_rays) and in the given order (bin_min_rays 0)
 CANDIDATES = (
-    TraceConfig(0, 16, 16, 1, 64),
-    TraceConfig(1, 16, 16, 1, 64),
+    TraceConfig(3, 16, 32, 1, 64),
+    TraceConfig(3, 16, 16, 1, 64),
+    TraceConfig(3, 16, 48, 1, 64),
+    TraceConfig(3, 8, 32, 1, 64),
     TraceConfig(1, 16, 32, 1, 64),
-    TraceConfig(1, 16, 48, 1, 64),
-    TraceConfig(1, 16, 56, 1, 64),
-    TraceConfig(1, 8, 56, 1, 64),
+    TraceConfig(0, 16, 16, 1, 64),
     TraceConfig(2, 16, 16, 2, 64),
+    TraceConfig(1, 16, 56, 1, 64, 0),
+    TraceConfig(3, 16, 56, 1, 64, 0),
+    TraceConfig(1, 16, 16, 1, 64, 0),
 )
+
+
+def _apply(bh, cfg: TraceConfig) -> None:
+    bh.set_trace_config(cfg.persistent, cfg.lds_stack, cfg.refill_lanes, cfg.grid_scale, cfg.chunk, cfg.bin_min_rays)
 
 
 def scene_signature(bh) -> str:
@@ -71,7 +80,7 @@ def autotune(bh, d_rays, d_tmax, any_hit: bool = False, sample: int = 1 << 21, r
         saved = json.loads(Path(cache).read_text())
         if key in saved:
             cfg = TraceConfig(**saved[key])
-            bh.set_trace_config(cfg.persistent, cfg.lds_stack, cfg.refill_lanes, cfg.grid_scale, cfg.chunk)
+            _apply(bh, cfg)
             return cfg, {}
     n = int(d_tmax.numel())
     m = min(n, sample)
@@ -80,7 +89,7 @@ def autotune(bh, d_rays, d_tmax, any_hit: bool = False, sample: int = 1 << 21, r
     out = torch.empty(m * (1 if any_hit else 16), dtype=torch.uint8, device=d_tmax.device)
     timings = {}
     for cfg in candidates:
-        bh.set_trace_config(cfg.persistent, cfg.lds_stack, cfg.refill_lanes, cfg.grid_scale, cfg.chunk)
+        _apply(bh, cfg)
         if any_hit:
             ms = _time(lambda: bh.any_device(rays, tmax, out), reps)
         else:
@@ -88,7 +97,7 @@ def autotune(bh, d_rays, d_tmax, any_hit: bool = False, sample: int = 1 << 21, r
         timings[cfg] = ms
     bh.check_traversal()
     best = min(timings, key=timings.get)
-    bh.set_trace_config(best.persistent, best.lds_stack, best.refill_lanes, best.grid_scale, best.chunk)
+    _apply(bh, best)
     if cache is not None:
         p = Path(cache)
         saved = json.loads(p.read_text()) if p.exists() else {}

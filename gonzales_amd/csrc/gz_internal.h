@@ -61,11 +61,19 @@ void release_mis_queues(MisQueues *m);
 //   dErr     : bit 0 = a traversal stack overflowed, bit 1 = a shading kernel met a bad index;
 //              raised by the launches on the stream, read and cleared by
 //              gz_gpu_check_traversal(stream), so concurrent streams never see each other's flag;
-//   dCounter : the ray counter of the persistent traversal kernel, reset by a kernel at the start
-//              of each launch; launches on one stream run in order, so they can share it.
+//   dCounter : the ray counters of the persistent traversal kernels (one, or one per XCD: eight),
+//              reset by a kernel at the start of each launch; launches on one stream run in order,
+//              so they can share them.
+//   dBin     : the ray binning's storage (sorted batch, keys, counts), grow-only, binRays rays;
+//   ev       : with recordTimes, events before the binning, before and after the traversal kernel
+//              of the last launch (created at first use); timed: whether they hold one.
 struct StreamSlot {
         int *dErr = nullptr;
         unsigned long long *dCounter = nullptr;
+        uint32_t *dBin = nullptr;
+        int64_t binRays = 0;
+        hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+        bool timed = false;
 };
 
 struct Scene {
@@ -102,7 +110,7 @@ struct Scene {
         size_t stageBytes = 0;
 
         // Traversal launch configuration.
-        GzTraceConfig traceConfig{1, 16, 56, 1, 64, {0, 0, 0}};
+        GzTraceConfig traceConfig{3, 16, 32, 1, 64, 1 << 18, 0, 0};
         bool boundsSafe = false;  // every node box coordinate is finite with magnitude <= 2^40
         int cuCount = 256;
 };

@@ -50,15 +50,15 @@ bool ensure_stage(Scene *s, size_t bytes) {
 StreamSlot *stream_slot(Scene *s, hipStream_t stream) {
         auto it = s->slots.find(stream);
         if (it != s->slots.end()) return &it->second;
-        // 64 bytes: the flag at 0, the counter at 8 (zeroed before any launch can use them)
+        // 128 bytes: the flag at 0, the eight ray counters at 64 (zeroed before any launch can use them)
         void *p = nullptr;
-        if (alloc_fault("stream slot") || !hip_ok(hipMalloc(&p, 64), "hipMalloc(stream slot)")) return nullptr;
-        if (!hip_ok(hipMemset(p, 0, 64), "hipMemset(stream slot)")) {
+        if (alloc_fault("stream slot") || !hip_ok(hipMalloc(&p, 128), "hipMalloc(stream slot)")) return nullptr;
+        if (!hip_ok(hipMemset(p, 0, 128), "hipMemset(stream slot)")) {
                 (void)hipFree(p);
                 return nullptr;
         }
-        s->deviceBytes += 64;
-        StreamSlot slot{static_cast<int *>(p), reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 8)};
+        s->deviceBytes += 128;
+        StreamSlot slot{static_cast<int *>(p), reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 64)};
         return &s->slots.emplace(stream, slot).first->second;
 }
 
@@ -88,8 +88,12 @@ static void free_scene(Scene *s) {
         if (s->stream) (void)hipStreamSynchronize(s->stream);
         release_mis(s->mis);
         release_mis_queues(s->misQueues);
-        for (auto &kv : s->slots)
+        for (auto &kv : s->slots) {
                 if (kv.second.dErr) (void)hipFree(kv.second.dErr);
+                if (kv.second.dBin) (void)hipFree(kv.second.dBin);
+                for (hipEvent_t e : kv.second.ev)
+                        if (e) (void)hipEventDestroy(e);
+        }
         void *ptrs[] = {s->dNodes, s->dTris, s->dPrims, s->dMeshes, s->dMaterials, s->dStage, s->dBuildOrder};
         for (void *p : ptrs)
                 if (p) (void)hipFree(p);
@@ -418,12 +422,12 @@ extern "C" int gz_gpu_set_trace_config(void *handle, const GzTraceConfig *config
                 set_error("gz_gpu_set_trace_config: null argument");
                 return GZ_E_INVALID;
         }
-        if (config->persistent < 0 || config->persistent > 2 ||
+        if (config->persistent < 0 || config->persistent > 3 ||
             (config->ldsStack != 0 && config->ldsStack != 8 && config->ldsStack != 16) || config->refillLanes < 1 ||
             config->refillLanes > 64 || config->gridScale < 0 || config->gridScale > 16 || config->chunk < 1 ||
-            config->chunk > 4096) {
-                set_error("gz_gpu_set_trace_config: persistent 0..2, ldsStack 0/8/16, refillLanes 1..64, "
-                          "gridScale 0..16, chunk 1..4096");
+            config->chunk > 4096 || config->binMinRays < 0 || config->recordTimes < 0 || config->recordTimes > 1) {
+                set_error("gz_gpu_set_trace_config: persistent 0..3, ldsStack 0/8/16, refillLanes 1..64, "
+                          "gridScale 0..16, chunk 1..4096, binMinRays >= 0, recordTimes 0/1");
                 return GZ_E_INVALID;
         }
         Scene *s = static_cast<Scene *>(handle);

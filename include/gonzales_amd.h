@@ -266,22 +266,39 @@ int gz_gpu_expand_hits_device(void *handle, const struct GzHit *d_hits, const fl
 /* Launch configuration of the traversal kernels (no reference counterpart: the reference launches
  * one thread per ray in blocks of 256, kernel.mojo:649-662). Results do not depend on it.
  *   persistent  : 0 = one lane per ray; 1 = persistent lanes refilled from an atomic ray
- *                 counter; 2 = persistent lanes that trace a fixed stride of rays;
+ *                 counter; 2 = persistent lanes that trace a fixed stride of rays; 3 = as 1 with
+ *                 the batch cut into eight ranges, one per XCD, each with its own counter (a
+ *                 wave draws from its XCD's range first, then from the others);
  *   ldsStack    : stack entries kept in LDS per lane (0, 8 or 16; the rest in private memory);
- *   refillLanes : idle lanes (1..64) that trigger a refill of a wave's lanes (mode 1);
+ *   refillLanes : idle lanes (1..64) that trigger a refill of a wave's lanes (modes 1, 3);
  *   gridScale   : persistent grid = gridScale x the blocks resident per CU x CUs (0 = 1);
- *   chunk       : rays a wave takes from the global counter per atomic (mode 1, 1..4096). */
+ *   chunk       : rays a wave takes from a counter per atomic (modes 1, 3; 1..4096);
+ *   binMinRays  : batches of at least this many rays (and at least 1) are traced in the order of
+ *                 their origin and direction cells (a counting sort on the device, into storage
+ *                 the handle keeps per stream, 8 bytes per ray + 1 MiB), so that the waves in
+ *                 flight read overlapping parts of the tree; each ray's result still goes to its
+ *                 own index. 0 = always in the given order;
+ *   recordTimes : 1 = record HIP events around the binning and the traversal kernel of each
+ *                 launch, for gz_gpu_trace_times (0 = none). */
 struct GzTraceConfig {
         int32_t persistent;
         int32_t ldsStack;
         int32_t refillLanes;
         int32_t gridScale;
         int32_t chunk;
-        int32_t reserved[3];
+        int32_t binMinRays;
+        int32_t recordTimes;
+        int32_t reserved;
 };
 
 int gz_gpu_set_trace_config(void *handle, const struct GzTraceConfig *config);
 int gz_gpu_get_trace_config(void *handle, struct GzTraceConfig *config);
+
+/* With recordTimes set: waits for `stream` and returns in *binMs and *traceMs the milliseconds of
+ * the binning kernels (0 if the batch was traced as given) and of the traversal kernel of the last
+ * closest-hit or any-hit launch on `stream`. GZ_E_INVALID if none was recorded. No counterpart in
+ * the reference (for per-kernel rates). */
+int gz_gpu_trace_times(void *handle, void *stream, float *binMs, float *traceMs);
 
 /* Waits for `stream` and returns GZ_E_OVERFLOW if any traversal launched on `stream` since the
  * last call stopped on a full stack (GZ_E_INVALID if a shading kernel met an index out of range),

@@ -165,6 +165,36 @@ def test_axis_aligned_and_degenerate_rays(gpu):
     bh.free()
 
 
+@pytest.mark.parametrize("persistent", [0, 1, 2, 3])
+def test_binned_order_gives_the_same_records(gpu, persistent):
+    """Rays traced in the order of their origin cells (the binning counting sort) give the records,
+    occlusion flags and work counters of the oracle, ray for ray, including origins outside the
+    scene box, NaN origins and zero directions (all land in some cell), for a batch above and one
+    below the binning threshold."""
+    bh = random_soup_hierarchy(100_000, 31, scale=0.03)
+    bh.upload_to_gpu()
+    view = OracleView(bh)
+    for n, bin_min in ((300_001, 1 << 18), (1000, 1)):
+        rays, tmax = random_rays(n, 32, lo=-0.5, hi=1.5)
+        rays["orgX"][::101] = np.nan
+        rays["orgY"][::103] = 1e30
+        rays["dirZ"][::107] = 0.0
+        rays["dirX"][::109] = 0.0
+        rays["dirY"][::109] = 0.0
+        rays["dirZ"][::109] = 0.0
+        bh.set_trace_config(persistent, 16, 16, 1, 64, bin_min)
+        ref, ref_st = view.closest(rays, tmax, nthreads=16, stats=True)
+        hits, st = _device_closest(bh, rays, tmax, stats=True)
+        _assert_hits_equal(hits, ref)
+        assert st.tobytes() == ref_st.tobytes()
+        occ, _ = _device_any(bh, rays, tmax)
+        assert np.array_equal(occ, ref["hit"].astype(bool))
+        bh.set_trace_config(persistent, 16, 16, 1, 64, 0)
+        plain, _ = _device_closest(bh, rays, tmax)
+        assert plain.tobytes() == hits.tobytes()
+    bh.free()
+
+
 def test_empty_scene_and_empty_batch(gpu):
     mesh = TriangleMesh.from_xyz(np.zeros((3, 3), np.float32), np.arange(3))
     bh = BoundingHierarchy(Scene([mesh], np.zeros(0, abi.MATERIAL)), np.zeros(0, abi.PRIM_ID),

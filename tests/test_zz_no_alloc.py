@@ -54,7 +54,7 @@ def test_noalloc_equals_the_staged_batch(gpu):
     box.free()
 
 
-@pytest.mark.parametrize("persistent", [0, 1, 2])
+@pytest.mark.parametrize("persistent", [0, 1, 2, 3])
 def test_noalloc_in_every_launch_mode(gpu, persistent):
     lib = abi.load()
     soup = random_soup_hierarchy(50_000, seed=21)
