@@ -47,8 +47,9 @@ PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_k_trace_closest_incoherent_1M.
 
 
 def measured_traffic() -> float | None:
-    """HBM bytes per closest-hit launch on this workload, from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy); None if it is absent."""
+    """Bytes per launch of the traversal kernel that leave L2 for the memory side (the MALL or HBM),
+    from the committed rocprofv3 --pmc summary: 128/64/32 B x the sized read requests
+    (TCC_EA0_RDREQ_128B/64B/32B) plus the write requests (TCC_EA0_WRREQ, 64B); None if absent."""
     try:
         with open(PMC_SUMMARY) as f:
             for line in f:

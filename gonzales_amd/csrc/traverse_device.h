@@ -72,6 +72,45 @@ __device__ __forceinline__ bool slab(float mnx, float mny, float mnz, float mxx,
         return slab_exact(mnx, mny, mnz, mxx, mxy, mxz, r, tMax, tNearOut);
 }
 
+// v_max3_f32 / v_min3_f32 of three values none of which is NaN. fmaxf and fminf of the same values
+// give the same results (up to the sign of a zero), but the compiler quiets each operand with an
+// extra v_max_f32 first (IEEE mode), which doubles the work of the slab test.
+__device__ __forceinline__ float max3_finite(float a, float b, float c) {
+        float r;
+        asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+        return r;
+}
+__device__ __forceinline__ float min3_finite(float a, float b, float c) {
+        float r;
+        asm("v_min3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+        return r;
+}
+
+// v_min_f32 of a finite value and a value that is finite, infinite or a quiet NaN (which it ignores,
+// as fminf does); a signalling NaN would not be ignored, so tMax enters as Lane::tBest, quieted when
+// the lane starts.
+__device__ __forceinline__ float min_quiet(float a, float q) {
+        float r;
+        asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(q));
+        return r;
+}
+
+// slab_fast for the two child boxes of an interior record (safe rays only).
+__device__ __forceinline__ void slab_pair_fast(const float4 &a, const float4 &b, const float4 &c, const RayState &r,
+                                               float tMax, bool &lh, float &tl, bool &rh, float &tr) {
+        // left: (a.x, a.y, a.z) - (a.w, b.x, b.y); right: (b.z, b.w, c.x) - (c.y, c.z, c.w)
+        const float lNx = (r.nx ? a.x : a.w) * r.rdx - r.ordx, lFx = (r.nx ? a.w : a.x) * r.rdx - r.ordx;
+        const float lNy = (r.ny ? a.y : b.x) * r.rdy - r.ordy, lFy = (r.ny ? b.x : a.y) * r.rdy - r.ordy;
+        const float lNz = (r.nz ? a.z : b.y) * r.rdz - r.ordz, lFz = (r.nz ? b.y : a.z) * r.rdz - r.ordz;
+        const float rNx = (r.nx ? b.z : c.y) * r.rdx - r.ordx, rFx = (r.nx ? c.y : b.z) * r.rdx - r.ordx;
+        const float rNy = (r.ny ? b.w : c.z) * r.rdy - r.ordy, rFy = (r.ny ? c.z : b.w) * r.rdy - r.ordy;
+        const float rNz = (r.nz ? c.x : c.w) * r.rdz - r.ordz, rFz = (r.nz ? c.w : c.x) * r.rdz - r.ordz;
+        tl = max3_finite(lNx, lNy, max3_finite(lNz, 0.0f, 0.0f));
+        tr = max3_finite(rNx, rNy, max3_finite(rNz, 0.0f, 0.0f));
+        lh = tl <= min3_finite(lFx, lFy, min_quiet(lFz, tMax)) * 1.0000003f;
+        rh = tr <= min3_finite(rFx, rFy, min_quiet(rFz, tMax)) * 1.0000003f;
+}
+
 // kernel.mojo:122-156 (cross :84-90 as a.yzx*b.zxy - a.zxy*b.yzx, dot :92-95 left to right).
 __device__ __forceinline__ bool tri_test(const RayState &r, float4 a, float4 b, float4 c, float tMax, float &tOut,
                                          float &uOut, float &vOut) {
@@ -139,7 +178,7 @@ __device__ __forceinline__ void lane_init(Lane &L, const Ray_C ray, float tMax, 
         r.safe = sceneSafe && ax >= lo && ay >= lo && az >= lo && ax <= hi && ay <= hi && az <= hi &&
                  fabsf(r.ox) <= hi && fabsf(r.oy) <= hi && fabsf(r.oz) <= hi;
         L.hitPrim = -1;
-        L.tBest = L.tMax;
+        L.tBest = __builtin_canonicalizef(L.tMax);  // a signalling NaN quieted: the slab tests ignore it
         L.bu = 0.0f;
         L.bv = 0.0f;
         L.cur = rootDesc;
@@ -217,8 +256,13 @@ __device__ __forceinline__ int step(Lane &L, Stack<kLds> &S, const NodePair *__r
         if (kStats) ++L.nInterior;
         const NodePair np = nodes[L.cur];
         float tl, tr;
-        const bool lh = slab(np.a.x, np.a.y, np.a.z, np.a.w, np.b.x, np.b.y, L.r, L.tBest, tl);
-        const bool rh = slab(np.b.z, np.b.w, np.c.x, np.c.y, np.c.z, np.c.w, L.r, L.tBest, tr);
+        bool lh, rh;
+        if (L.r.safe) {
+                slab_pair_fast(np.a, np.b, np.c, L.r, L.tBest, lh, tl, rh, tr);
+        } else {
+                lh = slab_exact(np.a.x, np.a.y, np.a.z, np.a.w, np.b.x, np.b.y, L.r, L.tBest, tl);
+                rh = slab_exact(np.b.z, np.b.w, np.c.x, np.c.y, np.c.z, np.c.w, L.r, L.tBest, tr);
+        }
         if (lh && rh) {
                 if (L.sp == GZ_STACK_CAPACITY) return kStepOverflow;
                 const bool leftFirst = tl <= tr;
