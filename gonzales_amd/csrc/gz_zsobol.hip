@@ -3,6 +3,7 @@
 // fixed sequence of get1D/get2D/get3D calls). One lane per (pixel, sample index).
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "gz_internal.h"
 #include "zsobol_device.h"
@@ -73,7 +74,24 @@ __global__ __launch_bounds__(kZsBlock) void k_zsobol(ZSobolParams p, const uint3
 static std::mutex g_table_mu;
 static uint32_t *g_dev_table[64] = {nullptr};
 
-// Device copy of the 1024 x 52 matrix table (213 KB), made once per device.
+// The byte table of the 1024 x 52 matrix table (zsobol_device.h), on the host.
+static std::vector<uint32_t> byte_table() {
+        const uint32_t *cols = sobol_table();
+        std::vector<uint32_t> t(kSobolByteTableWords, 0u);
+        for (int d = 0; d < kSobolDims; ++d)
+                for (int k = 0; k < kSobolBytes; ++k)
+                        for (int v = 0; v < 256; ++v) {
+                                uint32_t acc = 0;
+                                for (int j = 0; j < 8; ++j) {
+                                        const int c = 8 * k + j;
+                                        if (((v >> j) & 1) && c < kSobolCols) acc ^= cols[d * kSobolCols + c];
+                                }
+                                t[((size_t)d * kSobolBytes + k) * 256 + v] = acc;
+                        }
+        return t;
+}
+
+// Device copy of the matrices in their byte-table form (7 MiB), made once per device.
 const uint32_t *device_sobol_table() {
         int dev = 0;
         if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
@@ -84,9 +102,10 @@ const uint32_t *device_sobol_table() {
         std::lock_guard<std::mutex> lock(g_table_mu);
         if (!g_dev_table[dev]) {
                 uint32_t *d = nullptr;
-                const size_t bytes = sizeof(uint32_t) * kSobolDims * kSobolCols;
+                const size_t bytes = sizeof(uint32_t) * kSobolByteTableWords;
                 if (!hip_ok(hipMalloc((void **)&d, bytes), "hipMalloc(sobol table)")) return nullptr;
-                if (!hip_ok(hipMemcpy(d, sobol_table(), bytes, hipMemcpyHostToDevice), "upload sobol table")) {
+                const std::vector<uint32_t> host = byte_table();
+                if (!hip_ok(hipMemcpy(d, host.data(), bytes, hipMemcpyHostToDevice), "upload sobol table")) {
                         (void)hipFree(d);
                         return nullptr;
                 }

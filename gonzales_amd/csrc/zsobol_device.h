@@ -15,6 +15,11 @@ namespace gz {
 
 constexpr int kSobolDims = 1024;
 constexpr int kSobolCols = 52;
+// The device form of the matrices: per dimension, for each of the 7 bytes of a 52-bit index and
+// each value of that byte, the XOR of the columns of its set bits (bits 52..55 of byte 6 have no
+// column: 0). 1024 x 7 x 256 words, 7 MiB.
+constexpr int kSobolBytes = 7;
+constexpr size_t kSobolByteTableWords = (size_t)kSobolDims * kSobolBytes * 256;
 
 struct ZSobolParams {
         int64_t seed;
@@ -60,12 +65,27 @@ __device__ __forceinline__ uint64_t zs_morton2(uint32_t x, uint32_t y) {
 // Swift's `>>` on UInt64 yields 0 for shifts of 64 or more.
 __device__ __forceinline__ uint64_t zs_shr(uint64_t v, int s) { return s >= 64 ? 0ull : v >> s; }
 
-__device__ __constant__ static const uint8_t kZsPerm[24][4] = {
+// The 24 permutations of the base-4 digits (ZSobolSampler.swift:148-156), 2 bits per entry and a
+// byte per permutation, packed into three 64-bit words so that a lookup is shifts and masks.
+constexpr uint8_t kZsPermTable[24][4] = {
         {0, 1, 2, 3}, {0, 1, 3, 2}, {0, 2, 1, 3}, {0, 2, 3, 1}, {0, 3, 2, 1}, {0, 3, 1, 2},
         {1, 0, 2, 3}, {1, 0, 3, 2}, {1, 2, 0, 3}, {1, 2, 3, 0}, {1, 3, 2, 0}, {1, 3, 0, 2},
         {2, 1, 0, 3}, {2, 1, 3, 0}, {2, 0, 1, 3}, {2, 0, 3, 1}, {2, 3, 0, 1}, {2, 3, 1, 0},
         {3, 1, 2, 0}, {3, 1, 0, 2}, {3, 2, 1, 0}, {3, 2, 0, 1}, {3, 0, 2, 1}, {3, 0, 1, 2},
 };
+
+constexpr uint64_t zs_perm_word(int w) {
+        uint64_t x = 0;
+        for (int p = 0; p < 8; ++p)
+                for (int d = 0; d < 4; ++d) x |= (uint64_t)kZsPermTable[8 * w + p][d] << (8 * p + 2 * d);
+        return x;
+}
+
+__device__ __forceinline__ uint32_t zs_perm(int perm, int digit) {
+        constexpr uint64_t w0 = zs_perm_word(0), w1 = zs_perm_word(1), w2 = zs_perm_word(2);
+        const uint64_t w = perm < 8 ? w0 : perm < 16 ? w1 : w2;
+        return (uint32_t)(w >> (((perm & 7) << 3) + (digit << 1))) & 3u;
+}
 
 __device__ __forceinline__ uint64_t zs_sample_index(const ZSobolParams &p, uint64_t morton, int dim) {
         uint64_t out = 0;
@@ -78,7 +98,7 @@ __device__ __forceinline__ uint64_t zs_sample_index(const ZSobolParams &p, uint6
                 const uint64_t higher = zs_shr(morton, shift + 2);
                 const uint32_t h = zs_mix_bits(higher ^ dimMix);
                 const int perm = (int)((h >> 24) % 24u);
-                out |= (shift < 64 ? (uint64_t)kZsPerm[perm][digit] << shift : 0ull);
+                out |= (shift < 64 ? (uint64_t)zs_perm(perm, digit) << shift : 0ull);
         }
         if (odd) {
                 int digit = (int)(morton & 1);
@@ -90,16 +110,15 @@ __device__ __forceinline__ uint64_t zs_sample_index(const ZSobolParams &p, uint6
 }
 
 // sobolSample: XOR of the columns of the set bits among the 52 lowest, Owen-scrambled, scaled
-// by 2^-32 (u32 -> f32 rounds to nearest even) and clamped to 1 - 1e-6.
+// by 2^-32 (u32 -> f32 rounds to nearest even) and clamped to 1 - 1e-6. The XOR is taken a byte
+// of the index at a time from the byte table (the same XOR regrouped): seven loads that do not
+// depend on each other, where a walk over the set bits chains one dependent load per bit.
 __device__ __forceinline__ float zs_sobol(uint64_t index, int dim, uint32_t seed, const uint32_t *__restrict__ table) {
-        const uint32_t *col = table + dim * kSobolCols;
-        uint32_t acc = 0;
-        uint64_t bits = index & ((1ull << kSobolCols) - 1);
-        while (bits) {
-                const int b = __builtin_ctzll(bits);
-                acc ^= col[b];
-                bits &= bits - 1;
-        }
+        const uint32_t *t = table + (size_t)dim * (kSobolBytes * 256);
+        const uint64_t x = index & ((1ull << kSobolCols) - 1);
+        uint32_t acc = t[(uint32_t)x & 255u];
+#pragma unroll
+        for (int k = 1; k < kSobolBytes; ++k) acc ^= t[k * 256 + ((uint32_t)(x >> (8 * k)) & 255u)];
         const float f = (float)zs_owen(acc, seed) * 2.32830643653869628906e-10f;
         const float oneMinusEps = 1.0f - 1e-6f;
         return oneMinusEps < f ? oneMinusEps : f;
