@@ -99,6 +99,11 @@ struct Scene {
         TriRec *dTris = nullptr;
         uint32_t *dBuildOrder = nullptr;  // device build: caller's primitive index per leaf position
         PrimId_C *dPrims = nullptr;
+        // The three vertices of every primId in primId order (3 float4 per primitive, p0.w = 1 for a
+        // primitive whose mesh or vertex indices are out of range), gathered at the first
+        // gz_gpu_set_mis_scene: the shading kernels read a hit's vertices with one load, not
+        // through the primId -> mesh -> vertex index -> point chain.
+        float4 *dPrimTris = nullptr;
         DevMesh *dMeshes = nullptr;
         Material_C *dMaterials = nullptr;
         void **meshBuffers = nullptr;  // per-mesh device arrays to free

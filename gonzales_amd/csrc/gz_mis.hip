@@ -752,8 +752,42 @@ static bool put(T **dst, const T *src, size_t count, const char *what) {
                (count == 0 || hip_ok(hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice), what));
 }
 
+__global__ __launch_bounds__(kBlock) void k_prim_tris(const PrimId_C *__restrict__ prims, int64_t n,
+                                                      const DevMesh *__restrict__ meshes, int64_t meshCount,
+                                                      float4 *__restrict__ out) {
+        const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        if (i >= n) return;
+        f3 p0, p1, p2;
+        const bool ok = tri_points(meshes, meshCount, prims[i], p0, p1, p2);
+        out[3 * i] = ok ? make_float4(p0.x, p0.y, p0.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        out[3 * i + 1] = ok ? make_float4(p1.x, p1.y, p1.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        out[3 * i + 2] = ok ? make_float4(p2.x, p2.y, p2.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Fills Scene::dPrimTris once per handle (synchronously, on the handle's stream).
+static int ensure_prim_tris(Scene *s) {
+        if (s->dPrimTris || s->primCount <= 0) return GZ_OK;
+        const size_t bytes = sizeof(float4) * 3 * (size_t)s->primCount;
+        void *p = nullptr;
+        if (alloc_fault("primitive vertices") || !hip_ok(hipMalloc(&p, bytes), "hipMalloc(primitive vertices)"))
+                return GZ_E_ALLOC;
+        float4 *d = static_cast<float4 *>(p);
+        hipLaunchKernelGGL(k_prim_tris, dim3((unsigned)((s->primCount + kBlock - 1) / kBlock)), dim3(kBlock), 0, s->stream,
+                           s->dPrims, s->primCount, s->dMeshes, s->meshCount, d);
+        if (!hip_ok(hipGetLastError(), "primitive vertices launch") ||
+            !hip_ok(hipStreamSynchronize(s->stream), "hipStreamSynchronize")) {
+                (void)hipFree(p);
+                return GZ_E_HIP;
+        }
+        s->dPrimTris = d;
+        s->deviceBytes += (int64_t)bytes;
+        return GZ_OK;
+}
+
 static int set_mis_scene(Scene *s, const GzMaterial *materials, int32_t materialCount, const GzAreaLight *lights,
                          int32_t lightCount, const float *skyRadiance) {
+        const int trc = ensure_prim_tris(s);
+        if (trc != GZ_OK) return trc;
         std::vector<DevLight> dl((size_t)lightCount);
         std::vector<float> power;
         // Light powers and the power CDF in the reference's float operation order
@@ -893,6 +927,7 @@ bool gz::mis_args(Scene *s, const GzMisParams *params, int *err, MisArgs &a) {
         a.empty = s->empty;
         a.prims = s->dPrims;
         a.primCount = s->primCount;
+        a.primTris = s->dPrimTris;
         a.meshes = s->dMeshes;
         a.meshCount = s->meshCount;
         a.mats = ms->dMats;

@@ -94,7 +94,8 @@ static void free_scene(Scene *s) {
                 for (hipEvent_t e : kv.second.ev)
                         if (e) (void)hipEventDestroy(e);
         }
-        void *ptrs[] = {s->dNodes, s->dTris, s->dPrims, s->dMeshes, s->dMaterials, s->dStage, s->dBuildOrder};
+        void *ptrs[] = {s->dNodes, s->dTris, s->dPrims, s->dMeshes, s->dMaterials, s->dStage, s->dBuildOrder,
+                        s->dPrimTris};
         for (void *p : ptrs)
                 if (p) (void)hipFree(p);
         for (int64_t i = 0; i < s->meshBufferCount; ++i)

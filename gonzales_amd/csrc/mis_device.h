@@ -51,6 +51,7 @@ struct MisArgs {
         bool empty;
         const PrimId_C *prims;
         int64_t primCount;
+        const float4 *primTris;  // Scene::dPrimTris
         const DevMesh *meshes;
         int64_t meshCount;
         const GzMaterial *mats;
@@ -113,7 +114,10 @@ struct Interaction {
         int areaLight;
 };
 
-__device__ __forceinline__ bool tri_points(const MisArgs &a, const PrimId_C &pr, f3 &p0, f3 &p1, f3 &p2) {
+// The vertices of primitive pr through its mesh arrays (BoundingHierarchy.swift's triangle
+// lookup); false for a mesh or index out of range. Used once per scene to fill Scene::dPrimTris.
+__device__ __forceinline__ bool tri_points(const DevMesh *__restrict__ meshes, int64_t meshCount, const PrimId_C &pr,
+                                           f3 &p0, f3 &p1, f3 &p2) {
         int64_t mesh, base;
         if (pr.type == 0) {
                 mesh = pr.id1;
@@ -122,8 +126,8 @@ __device__ __forceinline__ bool tri_points(const MisArgs &a, const PrimId_C &pr,
                 mesh = pr.id2 >> 32;
                 base = (pr.id2 & 0xFFFFFFFFLL) * 3;
         }
-        if (mesh < 0 || mesh >= a.meshCount) return false;
-        const DevMesh m = a.meshes[mesh];
+        if (mesh < 0 || mesh >= meshCount) return false;
+        const DevMesh m = meshes[mesh];
         if (base < 0 || base + 2 >= m.vertexIndexCount) return false;
         const int64_t i0 = m.vertexIndices[base], i1 = m.vertexIndices[base + 1], i2 = m.vertexIndices[base + 2];
         if (i0 < 0 || i1 < 0 || i2 < 0 || i0 * 4 + 2 >= m.pointFloats || i1 * 4 + 2 >= m.pointFloats ||
@@ -139,12 +143,15 @@ __device__ __forceinline__ bool tri_points(const MisArgs &a, const PrimId_C &pr,
 // normals or uvs (Triangle.swift:363-474). false where the reference returns nil.
 __device__ bool surface_interaction(const MisArgs &a, const Ray_C &ray, int prim, float u, float v, Interaction &si) {
         if (prim < 0 || prim >= a.primCount) return false;
+        // the primId and the vertices: two loads that do not wait on each other
         const PrimId_C pr = a.prims[prim];
-        f3 p0, p1, p2;
-        if (!tri_points(a, pr, p0, p1, p2)) {
+        const float4 q0 = a.primTris[3 * (int64_t)prim], q1 = a.primTris[3 * (int64_t)prim + 1],
+                     q2 = a.primTris[3 * (int64_t)prim + 2];
+        if (q0.w != 0.0f) {
                 atomicOr(a.err, 2);
                 return false;
         }
+        const f3 p0 = mk(q0.x, q0.y, q0.z), p1 = mk(q1.x, q1.y, q1.z), p2 = mk(q2.x, q2.y, q2.z);
         const float b0 = 1.0f - u - v, b1 = u, b2 = v;
         const f3 dp02 = sub(p0, p2), dp12 = sub(p1, p2);
         const f3 pHit = add(add(mul(p0, b0), mul(p1, b1)), mul(p2, b2));
