@@ -195,6 +195,26 @@ def test_binned_order_gives_the_same_records(gpu, persistent):
     bh.free()
 
 
+def test_recorded_kernel_times(gpu):
+    """With recordTimes the library times the binning and the traversal kernel of the last launch
+    on a stream; without a timed launch the query is refused."""
+    import torch
+
+    bh = random_soup_hierarchy(20_000, 41)
+    bh.upload_to_gpu()
+    lib = abi.load()
+    b, t = ctypes.c_float(), ctypes.c_float()
+    assert lib.gz_gpu_trace_times(bh.handle, None, ctypes.byref(b), ctypes.byref(t)) == abi.GZ_E_INVALID
+    assert "recordTimes" in abi.last_error()
+    rays, tmax = random_rays(300_000, 42)
+    for bin_min, binned in ((1 << 18, True), (0, False)):
+        bh.set_trace_config(3, 16, 32, 1, 64, bin_min, record_times=1)
+        hits, _ = _device_closest(bh, rays, tmax)
+        b_ms, t_ms = bh.trace_times(torch.cuda.current_stream())
+        assert t_ms > 0.0 and (b_ms > 0.0 if binned else b_ms < 0.05 * t_ms), (b_ms, t_ms)
+    bh.free()
+
+
 def test_empty_scene_and_empty_batch(gpu):
     mesh = TriangleMesh.from_xyz(np.zeros((3, 3), np.float32), np.arange(3))
     bh = BoundingHierarchy(Scene([mesh], np.zeros(0, abi.MATERIAL)), np.zeros(0, abi.PRIM_ID),
