@@ -844,7 +844,7 @@ __device__ __forceinline__ bool hit_needs_all_bsdfs(const MisArgs &a, int prim) 
 }
 
 // Waves per SIMD a bounce kernel compiled with BSDF set kAll asks of the compiler.
-constexpr int mis_min_waves(int kAll) { return kAll == kBsdfAll ? 3 : kAll == kBsdfNoLayer ? 3 : 4; }
+constexpr int mis_min_waves(int kAll) { return kAll == kBsdfDiffuse ? 4 : 3; }
 
 // ---- Queue appends ----------------------------------------------------------------------------
 
