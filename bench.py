@@ -132,6 +132,8 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=1)
     ap.add_argument("--batch-round-paths", type=int, default=1 << 27,
                     help="slots per round of the batch, and paths per launch sequence of the one-at-a-time loop")
+    ap.add_argument("--batch-solo-paths", type=int, default=None,
+                    help="scenes of the batch with more paths render alone (default GZ_BATCH_SOLO_PATHS; 0: none)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the inverse-rendering adjoint leg")
     ap.add_argument("--adj-res", type=int, default=512)
     ap.add_argument("--adj-spp", type=int, default=32)
@@ -458,8 +460,10 @@ def batch_leg(args, dev, stream, host: dict, run_cpu: bool) -> dict:
     """64 scenes of uneven size in one launch sequence: triangle counts log-uniform in [1e4, 2e6]
     (displaced spheres and triangle soups), resolutions 256^2 / 512^2 / 1024^2, 4 / 16 / 64 spp,
     patches of diffuse / dielectric / GGX conductor materials, an area light and a sky, max depth 6
-    (seed 7). One step = every path sample of every scene: rounds of 8M path slots, each round one
-    replay of a captured HIP graph (camera rays, bounce levels over device path queues, film).
+    (seed 7). One step = every path sample of every scene: the scenes of at most 4M paths in rounds
+    of 128M path slots, each round one replay of a captured HIP graph (camera rays, bounce levels
+    over device path queues, film), then each larger scene alone with the single-scene launch
+    sequence (one queue storage for all of them).
     The launch shape is autotuned on 128x128 crops of the same scenes. For comparison, the same
     frames rendered one scene at a time (the single-scene loop) are timed too."""
     from concurrent.futures import ThreadPoolExecutor
@@ -502,7 +506,9 @@ def batch_leg(args, dev, stream, host: dict, run_cpu: bool) -> dict:
     tune = SceneBatch(tune_entries, round_paths=1 << 23)
     best, timings = autotune_batch(tune, BATCH_CANDIDATES, reps=1)
     tune.close()
-    batch = SceneBatch(entries, round_paths=args.batch_round_paths)
+    from gonzales_amd import abi
+    solo = abi.GZ_BATCH_SOLO_PATHS if args.batch_solo_paths is None else args.batch_solo_paths
+    batch = SceneBatch(entries, round_paths=args.batch_round_paths, solo_paths=solo)
     batch.set_launch(best.block, best.grid_scale, best.chunk, best.drain_level, best.queued)
     info = batch.info()
     n = info.totalPaths
@@ -517,11 +523,11 @@ def batch_leg(args, dev, stream, host: dict, run_cpu: bool) -> dict:
     b.record(stream)
     torch.cuda.synchronize()
     ms = a.elapsed_time(b) / args.batch_steps
+    batch.check(stream)
     for bh in hs:
         bh.check_traversal()
     # the same frames, one scene at a time through the single-scene loop (queued trace), and
     # (untimed) each scene's work counts for the algorithmic bytes
-    from gonzales_amd import abi
     seq_t, sums = [], torch.zeros(3, dtype=torch.int64, device=dev)
     d_st = None
     for e in entries:
@@ -562,7 +568,7 @@ def batch_leg(args, dev, stream, host: dict, run_cpu: bool) -> dict:
         idx = np.nonzero(sel)[0]
         if len(idx) == 0:
             continue
-        sub = SceneBatch([entries[k] for k in idx], round_paths=args.batch_round_paths)
+        sub = SceneBatch([entries[k] for k in idx], round_paths=args.batch_round_paths, solo_paths=solo)
         sub.set_launch(best.block, best.grid_scale, best.chunk, best.drain_level, best.queued)
         sub.render(True)
         torch.cuda.synchronize()
@@ -581,6 +587,7 @@ def batch_leg(args, dev, stream, host: dict, run_cpu: bool) -> dict:
                        "log-uniform 1e4..2e6 per scene, displaced spheres and soups, 256^2/512^2/1024^2, "
                        "4/16/64 spp, diffuse/dielectric/GGX conductor patches, area light + sky, max depth 6",
            "path_samples": n, "rounds": info.rounds, "round_paths": info.roundPaths,
+           "solo_paths": solo, "solo_scenes": info.soloScenes,
            "path_samples_per_s": round(n / (ms * 1e-3), 1), "ms_per_batch": round(ms, 2),
            "one_scene_at_a_time_ms": round(seq_ms, 2), "batch_speedup": round(seq_ms / ms, 3),
            "by_size_class": by_class,

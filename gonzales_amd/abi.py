@@ -70,6 +70,7 @@ GZ_E_OVERFLOW = -4
 GZ_E_ALLOC = -5
 GZ_STACK_CAPACITY = 64
 GZ_NOALLOC_MAX_RAYS = 524288
+GZ_BATCH_SOLO_PATHS = 4194304
 
 
 class TriangleMeshC(ctypes.Structure):
@@ -135,7 +136,7 @@ class GzBatchScene(ctypes.Structure):
 class GzBatchInfo(ctypes.Structure):
     _fields_ = [("totalPaths", ctypes.c_int64), ("rounds", ctypes.c_int32), ("items", ctypes.c_int32),
                 ("roundPaths", ctypes.c_int32), ("maxItemsPerRound", ctypes.c_int32), ("levels", ctypes.c_int32),
-                ("allBsdfs", ctypes.c_int32)]
+                ("allBsdfs", ctypes.c_int32), ("soloScenes", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 GZ_FILTER_GAUSSIAN = 0
@@ -195,7 +196,7 @@ SIGNATURES = {
     "gz_ply_parse": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P]),
     "gz_gpu_build_scene": (ctypes.c_void_p, [_P, _I64, _P, _I64, _P, _P, _P, _P, _I64, _P]),
     "gz_gpu_export_bvh2": (_I64, [_P, _P, _I64, _P]),
-    "gz_batch_create": (ctypes.c_void_p, [_P, ctypes.c_int32, _I64]),
+    "gz_batch_create": (ctypes.c_void_p, [_P, ctypes.c_int32, _I64, _I64]),
     "gz_batch_info": (ctypes.c_int, [_P, _P]),
     "gz_batch_set_launch": (ctypes.c_int, [_P, _P]),
     "gz_batch_render_device": (ctypes.c_int, [_P, ctypes.c_int32, _P]),

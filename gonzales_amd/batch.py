@@ -25,7 +25,12 @@ class BatchEntry:
 
 
 class SceneBatch:
-    def __init__(self, entries: list[BatchEntry], round_paths: int = 1 << 25):
+    """Scenes rendered together (gz_batch_*): the scenes with at most `solo_paths` paths are
+    packed into rounds of at most `round_paths` path slots, each round one launch sequence (a
+    replayed HIP graph); a scene with more paths fills the GPU alone and renders with the
+    single-scene launch sequence after the rounds. solo_paths = 0 puts every scene in the rounds."""
+
+    def __init__(self, entries: list[BatchEntry], round_paths: int = 1 << 25, solo_paths: int = abi.GZ_BATCH_SOLO_PATHS):
         if not entries:
             raise ValueError("a batch needs at least one scene")
         self.entries = list(entries)
@@ -42,7 +47,7 @@ class SceneBatch:
             arr[k].d_albedo = abi.ptr(e.film.albedo)
             arr[k].d_normal = abi.ptr(e.film.normal)
         self._scenes = arr
-        h = abi.load().gz_batch_create(arr, len(entries), int(round_paths))
+        h = abi.load().gz_batch_create(arr, len(entries), int(round_paths), int(solo_paths))
         if not h:
             raise RuntimeError(f"gz_batch_create: {abi.last_error()}")
         self.handle = h

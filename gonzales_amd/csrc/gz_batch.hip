@@ -16,6 +16,13 @@
 //   next    the round cursor moves on.
 // Captured once as a HIP graph, the sequence is replayed once per round with no host work
 // between rounds; the eager mode launches the same sequence directly.
+//
+// A scene with more than `soloPaths` paths fills the chip by itself: it stays out of the rounds
+// and renders alone, with the launch sequence of the single-scene path (camera kernel,
+// gz_gpu_mis_trace_device's kernels under the scene's own launch shape, film), in pieces of whole
+// pixels, all such scenes one after another with one queue storage. The rounds' kernels pay a
+// per-chunk item lookup and MisArgs read from memory that the single-scene kernels (MisArgs as a
+// kernel argument) do not; that only pays for scenes too small to fill the chip alone.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -596,6 +603,16 @@ struct Batch {
         // which gz_gpu_set_mis_scene frees and replaces.
         int *dErr = nullptr;
         std::vector<uint64_t> misGeneration;
+
+        // Scenes that render alone (more than soloPaths paths), in scene order; their pieces use
+        // dPaths (pathSlots >= soloSlots) and one queue storage shared by all of them.
+        std::vector<int> solo;
+        std::vector<GzMisParams> params;
+        std::vector<GzCamera> hCams;
+        std::vector<BatchFilm> hFilms;
+        int64_t soloSlots = 0;
+        int64_t pathSlots = 0;
+        MisQueues *soloQueues = nullptr;
 };
 
 static void release_graph(Batch *b) {
@@ -618,6 +635,7 @@ static void free_batch(Batch *b) {
                         b->ws.q[3]};
         for (void *p : ptrs)
                 if (p) (void)hipFree(p);
+        release_mis_queues(b->soloQueues);
         delete b;
 }
 
@@ -797,10 +815,11 @@ static bool capture(Batch *b) {
 
 using namespace gz;
 
-extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount, int64_t roundPaths) {
+extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount, int64_t roundPaths,
+                                 int64_t soloPaths) {
         clear_error();
-        if (!scenes || sceneCount <= 0 || roundPaths <= 0 || roundPaths > 0x7fffffffLL) {
-                set_error("gz_batch_create: need scenes and 0 < roundPaths < 2^31");
+        if (!scenes || sceneCount <= 0 || roundPaths <= 0 || roundPaths > 0x7fffffffLL || soloPaths < 0) {
+                set_error("gz_batch_create: need scenes, 0 < roundPaths < 2^31 and soloPaths >= 0");
                 return nullptr;
         }
         Batch *b = new Batch;
@@ -809,7 +828,7 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
         std::vector<MisArgs> args((size_t)sceneCount);
         std::vector<GzCamera> cams((size_t)sceneCount);
         std::vector<BatchFilm> films((size_t)sceneCount);
-        std::vector<char> waveScene((size_t)sceneCount, 0);
+        std::vector<char> waveScene((size_t)sceneCount, 0), soloScene((size_t)sceneCount, 0);
         int maxDepth = 0;
         b->misGeneration.assign((size_t)sceneCount, 0);
         for (int32_t s = 0; s < sceneCount; ++s) {
@@ -855,6 +874,18 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
                         free_batch(b);
                         return nullptr;
                 }
+                cams[(size_t)s] = bs.camera;
+                films[(size_t)s] = BatchFilm{bs.d_light,       bs.d_weight,         bs.d_albedo,  bs.d_normal,
+                                             bs.iso,           bs.maxComponentValue, bs.camera.resX, bs.camera.resY,
+                                             bs.camera.spp,    0};
+                b->params.push_back(bs.params);
+                const int64_t paths = crop_paths(&bs.camera);
+                if (soloPaths > 0 && paths > soloPaths) {
+                        soloScene[(size_t)s] = 1;
+                        b->solo.push_back(s);
+                        b->soloSlots = std::max(b->soloSlots, std::min<int64_t>(paths, roundPaths));
+                        continue;
+                }
                 if (!mis_all_diffuse(sc)) b->all = true;
                 const int set = mis_bsdf_set(sc);
                 if (set == kBsdfAll || (set == kBsdfNoLayer && b->bsdfSet == kBsdfDiffuse)) b->bsdfSet = set;
@@ -866,12 +897,10 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
                 else
                         b->anyMega = true;
                 maxDepth = std::max(maxDepth, (int)bs.params.maxDepth);
-                cams[(size_t)s] = bs.camera;
-                films[(size_t)s] = BatchFilm{bs.d_light,       bs.d_weight,         bs.d_albedo,  bs.d_normal,
-                                             bs.iso,           bs.maxComponentValue, bs.camera.resX, bs.camera.resY,
-                                             bs.camera.spp,    0};
         }
         b->levels = maxDepth + 1;
+        b->hCams = cams;
+        b->hFilms = films;
         // Rounds of whole pixels, scenes in order.
         int used = 0, pixels = 0, firstItem = 0;
         auto close_round = [&]() {
@@ -885,9 +914,10 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
         };
         for (int32_t s = 0; s < sceneCount; ++s) {
                 const GzCamera &c = cams[(size_t)s];
+                b->totalPaths += crop_paths(&c);
+                if (soloScene[(size_t)s]) continue;
                 const int64_t cropPixels = crop_paths(&c) / c.spp;
                 int64_t done = 0;
-                b->totalPaths += crop_paths(&c);
                 while (done < cropPixels) {
                         const int64_t room = (b->roundPaths - used) / c.spp;
                         if (room == 0) {
@@ -907,6 +937,7 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
         int maxRound = 1;
         for (const BatchRound &r : b->hRounds) maxRound = std::max(maxRound, (int)r.total);
         b->roundPaths = std::min(b->roundPaths, maxRound);
+        b->pathSlots = std::max<int64_t>(b->roundPaths, b->soloSlots);
         b->ctr.maxItems = std::max(b->maxItems, 1);
         b->ctr.levels = b->levels;
         const size_t slots = (size_t)b->roundPaths;
@@ -918,7 +949,7 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
             !upload(&b->dCams, cams.data(), cams.size(), "batch cameras") ||
             !upload(&b->dFilms, films.data(), films.size(), "batch films") ||
             !hip_ok(hipMalloc(&b->dCursor, sizeof(int)), "batch cursor") ||
-            !hip_ok(hipMalloc(&b->dPaths, slots * sizeof(GzPath)), "batch paths") ||
+            !hip_ok(hipMalloc(&b->dPaths, (size_t)b->pathSlots * sizeof(GzPath)), "batch paths") ||
             !hip_ok(hipMalloc(&b->dQ[0], slots * sizeof(int)), "batch queue") ||
             !hip_ok(hipMalloc(&b->dQ[1], slots * sizeof(int)), "batch queue") ||
             !hip_ok(hipMalloc(&b->ctr.n, (size_t)(b->levels + 1) * b->ctr.maxItems * sizeof(unsigned)), "batch counters") ||
@@ -953,6 +984,8 @@ extern "C" int gz_batch_info(void *batch, GzBatchInfo *info) {
         info->maxItemsPerRound = b->maxItems;
         info->levels = b->levels;
         info->allBsdfs = b->all ? 1 : 0;
+        info->soloScenes = (int32_t)b->solo.size();
+        info->reserved = 0;
         return GZ_OK;
 }
 
@@ -1003,14 +1036,33 @@ extern "C" int gz_batch_render_device(void *batch, int32_t useGraph, void *strea
         if (!hip_ok(hipGetDevice(&cur), "get device") || (cur != b->device && !hip_ok(hipSetDevice(b->device), "set device")))
                 return GZ_E_HIP;
         const hipStream_t st = static_cast<hipStream_t>(stream);
-        if (useGraph && !b->exec && !capture(b)) return GZ_E_HIP;
-        hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, b->dCursor);
-        if (!hip_ok(hipGetLastError(), "batch begin")) return GZ_E_HIP;
-        for (size_t r = 0; r < b->hRounds.size(); ++r) {
-                if (useGraph) {
-                        if (!hip_ok(hipGraphLaunch(b->exec, st), "graph launch")) return GZ_E_HIP;
-                } else if (!enqueue_round(b, st)) {
-                        return GZ_E_HIP;
+        if (!b->hRounds.empty()) {
+                if (useGraph && !b->exec && !capture(b)) return GZ_E_HIP;
+                hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, b->dCursor);
+                if (!hip_ok(hipGetLastError(), "batch begin")) return GZ_E_HIP;
+                for (size_t r = 0; r < b->hRounds.size(); ++r) {
+                        if (useGraph) {
+                                if (!hip_ok(hipGraphLaunch(b->exec, st), "graph launch")) return GZ_E_HIP;
+                        } else if (!enqueue_round(b, st)) {
+                                return GZ_E_HIP;
+                        }
+                }
+        }
+        // The scenes that render alone, in pieces of whole pixels: the single-scene launch sequence.
+        for (const int k : b->solo) {
+                const GzCamera &c = b->hCams[(size_t)k];
+                const BatchFilm &f = b->hFilms[(size_t)k];
+                const int64_t n = crop_paths(&c), piece = b->soloSlots / c.spp * c.spp;
+                for (int64_t first = 0; first < n; first += piece) {
+                        const int64_t count = std::min(piece, n - first);
+                        int rc = gz_gpu_camera_paths_device(&c, first, count, b->dPaths, stream);
+                        if (rc == GZ_OK)
+                                rc = mis_trace_shared(b->scenes[(size_t)k], b->dPaths, count, &b->params[(size_t)k], st,
+                                                      &b->soloQueues, b->dErr);
+                        if (rc == GZ_OK)
+                                rc = gz_gpu_film_add_device(&c, b->dPaths, count, f.iso, f.maxComponent, f.light, f.weight,
+                                                            f.albedo, f.normal, stream);
+                        if (rc != GZ_OK) return rc;
                 }
         }
         return GZ_OK;

@@ -56,6 +56,13 @@ void release_mis(MisScene *m);
 // Device path queues of the MIS trace, per stream (gz_mis.hip).
 struct MisQueues;
 void release_mis_queues(MisQueues *m);
+struct Scene;
+// gz_gpu_mis_trace_device over one scene with the queue storage taken from *queues (created on
+// first use, grow-only, freed with release_mis_queues) instead of the handle's own, and traversal
+// overflows raised in *err; for a caller that renders several scenes one after another on one
+// stream and keeps one storage for all of them.
+int mis_trace_shared(Scene *s, GzPath *d_paths, int64_t count, const GzMisParams *params, hipStream_t stream,
+                     MisQueues **queues, int *err);
 
 // Per-stream device state of a scene handle (allocated zeroed at the stream's first use):
 //   dErr     : bit 0 = a traversal stack overflowed, bit 1 = a shading kernel met a bad index;

@@ -593,10 +593,11 @@ void release_mis_queues(MisQueues *m) {
         delete m;
 }
 
-// The queue storage of `stream` with room for `count` paths and `levels` counters.
-static MisQueue *mis_queue(Scene *s, hipStream_t stream, size_t count, int levels) {
-        if (!s->misQueues) s->misQueues = new MisQueues;
-        MisQueue &q = s->misQueues->perStream[stream];
+// The queue storage of `stream` in `owner` (a handle's own, or one a batch shares between its
+// scenes; created on first use) with room for `count` paths and `levels` counters.
+static MisQueue *mis_queue(MisQueues *&owner, hipStream_t stream, size_t count, int levels) {
+        if (!owner) owner = new MisQueues;
+        MisQueue &q = owner->perStream[stream];
         if (q.cap < count) {
                 for (int k = 0; k < 2; ++k) {
                         if (q.q[k]) (void)hipFree(q.q[k]);
@@ -619,8 +620,8 @@ static MisQueue *mis_queue(Scene *s, hipStream_t stream, size_t count, int level
 }
 
 // mis_queue plus the split wavefront's class queues, hits and level counters.
-static MisQueue *wf_queue(Scene *s, hipStream_t stream, size_t count, int levels) {
-        MisQueue *q = mis_queue(s, stream, count, levels);
+static MisQueue *wf_queue(MisQueues *&owner, hipStream_t stream, size_t count, int levels) {
+        MisQueue *q = mis_queue(owner, stream, count, levels);
         if (!q) return nullptr;
         if (q->wfCap < count) {
                 void *old[] = {q->qd, q->qa, q->hits};
@@ -646,8 +647,8 @@ static MisQueue *wf_queue(Scene *s, hipStream_t stream, size_t count, int levels
 }
 
 // wf_queue plus the full wavefront's ray queues, pending terms, rays and results.
-static MisQueue *full_queue(Scene *s, hipStream_t stream, size_t count, int levels) {
-        MisQueue *q = wf_queue(s, stream, count, levels);
+static MisQueue *full_queue(MisQueues *&owner, hipStream_t stream, size_t count, int levels) {
+        MisQueue *q = wf_queue(owner, stream, count, levels);
         if (!q) return nullptr;
         if (q->fullCap < count) {
                 void *old[] = {q->qn, q->qs, q->qm, q->pend, q->srays, q->mrays, q->occ, q->mhits};
@@ -953,11 +954,11 @@ bool gz::mis_args(Scene *s, const GzMisParams *params, int *err, MisArgs &a) {
 // the counters, then one persistent launch per level. `launch(level, grid, qin, qout, ctl)`
 // enqueues one level's kernel.
 template <typename K, typename Launch>
-static int queued_levels(Scene *s, K kern, int64_t count, int maxDepth, hipStream_t st, const char *who,
-                         Launch &&launch, const std::function<void(unsigned, const int *, unsigned *, unsigned *,
-                                                                   unsigned)> &drain = {}) {
+static int queued_levels(Scene *s, MisQueues *&owner, K kern, int64_t count, int maxDepth, hipStream_t st,
+                         const char *who, Launch &&launch,
+                         const std::function<void(unsigned, const int *, unsigned *, unsigned *, unsigned)> &drain = {}) {
         const int levels = maxDepth + 2;
-        MisQueue *q = mis_queue(s, st, (size_t)count, levels);
+        MisQueue *q = mis_queue(owner, st, (size_t)count, levels);
         if (!q) return GZ_E_ALLOC;
         const GzMisLaunch &c = s->misLaunch;
         const unsigned chunk = (unsigned)c.chunk;
@@ -993,8 +994,11 @@ static void refresh_default_launch(Scene *s) {
                                             : GzMisLaunch{1, 256, 1, 256, 3, {0, 0, 0}};
 }
 
+// queues: the queue storage to use (null: the handle's own); err: the flag traversal overflows
+// raise (null: the handle's, per stream).
 static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisParams *params,
-                      GzPathStats *d_stats, void *stream, bool toEnd, const char *who) {
+                      GzPathStats *d_stats, void *stream, bool toEnd, const char *who, MisQueues **queues = nullptr,
+                      int *err = nullptr) {
         clear_error();
         if (!handle || count < 0 || (count > 0 && !d_paths)) {
                 set_error("%s: bad arguments", who);
@@ -1006,8 +1010,9 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
         const StreamSlot *slot = stream_slot(s, static_cast<hipStream_t>(stream));
         if (!slot) return GZ_E_ALLOC;
         MisArgs a;
-        if (!mis_args(s, params, slot->dErr, a)) return GZ_E_INVALID;
+        if (!mis_args(s, params, err ? err : slot->dErr, a)) return GZ_E_INVALID;
         if (count == 0) return GZ_OK;
+        MisQueues *&owner = queues ? *queues : s->misQueues;
         const int64_t blocks = (count + kMisBlock - 1) / kMisBlock;
         if (blocks > 0x7fffffffLL) {
                 set_error("%s: %lld paths exceed one launch", who, (long long)count);
@@ -1024,7 +1029,7 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                         return GZ_E_INVALID;
                 }
                 const int levels = params->maxDepth + 2;
-                MisQueue *q = full_queue(s, st, (size_t)count, levels);
+                MisQueue *q = full_queue(owner, st, (size_t)count, levels);
                 if (!q) return GZ_E_ALLOC;
                 const GzMisLaunch &c = s->misLaunch;
                 const GzTraceConfig &tc = s->traceConfig;
@@ -1103,7 +1108,7 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                         return GZ_E_INVALID;
                 }
                 const int levels = params->maxDepth + 2;
-                MisQueue *q = wf_queue(s, st, (size_t)count, levels);
+                MisQueue *q = wf_queue(owner, st, (size_t)count, levels);
                 if (!q) return GZ_E_ALLOC;
                 const GzMisLaunch &c = s->misLaunch;
                 const GzTraceConfig &tc = s->traceConfig;
@@ -1164,7 +1169,7 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                                                    stats);
                         };
                 return queued_levels(
-                        s, kern, count, params->maxDepth, st, who,
+                        s, owner, kern, count, params->maxDepth, st, who,
                         [&](int level, unsigned grid, const int *qin, int *qout, QueueLevel *ctl, unsigned chunk) {
                                 hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, a, d_paths, (unsigned)count, qin,
                                                    qout, ctl, level, chunk, stats);
@@ -1248,6 +1253,11 @@ extern "C" int gz_gpu_mis_trace_device(void *handle, GzPath *d_paths, int64_t co
         return mis_launch(handle, d_paths, count, params, nullptr, stream, true, "gz_gpu_mis_trace_device");
 }
 
+int gz::mis_trace_shared(Scene *s, GzPath *d_paths, int64_t count, const GzMisParams *params, hipStream_t stream,
+                         MisQueues **queues, int *err) {
+        return mis_launch(s, d_paths, count, params, nullptr, stream, true, "batch scene trace", queues, err);
+}
+
 extern "C" int gz_gpu_mis_trace_stats_device(void *handle, GzPath *d_paths, int64_t count,
                                              const GzMisParams *params, GzPathStats *d_stats, void *stream) {
         if (count > 0 && !d_stats) {
@@ -1296,7 +1306,7 @@ extern "C" int gz_gpu_mis_adjoint_device(void *handle, GzPath *d_paths, const Gz
                                                    adj);
                         };
                 return queued_levels(
-                        s, kern, count, params->maxDepth, st, "MIS adjoint launch",
+                        s, s->misQueues, kern, count, params->maxDepth, st, "MIS adjoint launch",
                         [&](int level, unsigned grid, const int *qin, int *qout, QueueLevel *ctl, unsigned chunk) {
                                 hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, a, d_paths, (unsigned)count, qin,
                                                    qout, ctl, level, chunk, adj);

@@ -560,17 +560,26 @@ struct GzBatchInfo {
         int32_t roundPaths;      /* path slots of a round: the largest round's paths (<= the roundPaths asked) */
         int32_t maxItemsPerRound;
         int32_t levels;          /* bounce levels of a round: 1 + the largest maxDepth */
-        int32_t allBsdfs;        /* 1 if some scene has a non-diffuse material */
+        int32_t allBsdfs;        /* 1 if some scene of the rounds has a non-diffuse material */
+        int32_t soloScenes;      /* scenes that render alone (more than soloPaths paths) */
+        int32_t reserved;
 };
 
 /* Plans the batch: the paths of all scenes (in scene order, whole pixels) are cut into rounds of
  * at most roundPaths path slots, and device tables are built for them; device state (about 280 B
- * per slot) is allocated for the largest round only. The batch snapshots each
- * scene's MIS tables and parameters: a gz_gpu_set_mis_scene on one of its scenes afterwards makes
- * gz_batch_render_device refuse to run (GZ_E_INVALID; create the batch again). Free the batch
- * before its scenes. All scenes must be on one device. NULL on failure, with gz_last_error()
- * set. */
-void *gz_batch_create(const struct GzBatchScene *scenes, int32_t sceneCount, int64_t roundPaths);
+ * per slot) is allocated for the largest round only. A scene with more than soloPaths paths
+ * (soloPaths > 0; GZ_BATCH_SOLO_PATHS is the measured break-even on MI355X) fills the chip alone:
+ * it stays out of the rounds and each render runs it with the single-scene launch sequence
+ * (gz_gpu_camera_paths_device, gz_gpu_mis_trace_device under the scene's own GzMisLaunch,
+ * gz_gpu_film_add_device) in pieces of at most roundPaths paths, with one queue storage for all
+ * such scenes (allocated by the first render); soloPaths = 0 puts every scene in the rounds.
+ * The batch snapshots each scene's MIS tables and parameters: a gz_gpu_set_mis_scene on one of
+ * its scenes afterwards makes gz_batch_render_device refuse to run (GZ_E_INVALID; create the
+ * batch again). Free the batch before its scenes. All scenes must be on one device. NULL on
+ * failure, with gz_last_error() set. */
+#define GZ_BATCH_SOLO_PATHS 4194304
+void *gz_batch_create(const struct GzBatchScene *scenes, int32_t sceneCount, int64_t roundPaths,
+                      int64_t soloPaths);
 int gz_batch_info(void *batch, struct GzBatchInfo *info);
 /* Launch shape of the batch's bounce levels (queued 1 or 3, as in GzMisLaunch). queued 3 bounces
  * the items of the scenes whose own default launch is the wavefront with the wavefront kernels and
@@ -581,8 +590,9 @@ int gz_batch_set_launch(void *batch, const struct GzMisLaunch *launch);
  * `stream`. Each round is the same launch sequence (camera rays, device path queues per bounce
  * level, film) driven by device tables and a device round cursor. useGraph = 1 captures that
  * sequence once as a HIP graph (on the first call after creation or gz_batch_set_launch) and
- * replays it once per round; 0 launches it directly. Both give identical films. Calls on one
- * batch must not overlap (they share its path slots and queues). */
+ * replays it once per round; 0 launches it directly. Both give identical films. The scenes that
+ * render alone follow the rounds, launched directly. Calls on one batch must not overlap (they
+ * share its path slots and queues). */
 int gz_batch_render_device(void *batch, int32_t useGraph, void *stream);
 /* gz_gpu_check_traversal for a batch: waits for `stream`, returns GZ_E_OVERFLOW if a traversal
  * of any scene's paths stopped on a full stack since the last call (the batch's own flag; then
@@ -728,7 +738,7 @@ static_assert(sizeof(struct GzCamera) == 200, "GzCamera layout");
 static_assert(sizeof(struct GzPathAdjoint) == 24, "GzPathAdjoint layout");
 static_assert(sizeof(struct GzMisLaunch) == 32, "GzMisLaunch layout");
 static_assert(sizeof(struct GzBatchScene) == 280, "GzBatchScene layout");
-static_assert(sizeof(struct GzBatchInfo) == 32, "GzBatchInfo layout");
+static_assert(sizeof(struct GzBatchInfo) == 40, "GzBatchInfo layout");
 static_assert(sizeof(struct GzBuildOptions) == 32, "GzBuildOptions layout");
 static_assert(sizeof(struct GzPlyInfo) == 40, "GzPlyInfo layout");
 #endif

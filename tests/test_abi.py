@@ -71,6 +71,9 @@ def test_device_entry_points_fail_cleanly_without_a_handle():
     assert lib.gz_gpu_set_mis_launch(None, ctypes.byref(c)) == abi.GZ_E_INVALID
     assert lib.gz_gpu_get_mis_launch(None, ctypes.byref(c)) == abi.GZ_E_INVALID
     assert ctypes.sizeof(abi.GzMisLaunch) == 32
+    assert lib.gz_batch_create(None, 0, 1 << 20, abi.GZ_BATCH_SOLO_PATHS) is None
+    assert "soloPaths" in abi.last_error()
+    assert ctypes.sizeof(abi.GzBatchInfo) == 40
 
 
 def test_zsobol_rejects_bad_patterns():

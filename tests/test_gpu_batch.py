@@ -62,7 +62,7 @@ def test_batch_films_equal_single_scene_renders(gpu):
     batch = SceneBatch(entries, round_paths=3000)
     info = batch.info()
     assert info.totalPaths == sum(cam.path_count for _, cam in sc)
-    assert info.rounds > 3 and info.levels == 7 and info.allBsdfs == 1
+    assert info.rounds > 3 and info.levels == 7 and info.allBsdfs == 1 and info.soloScenes == 0
     for mode in (False, True, True):
         batch.clear_films()
         batch.render(use_graph=mode)
@@ -89,6 +89,42 @@ def test_batch_films_equal_single_scene_renders(gpu):
         img = e.film.resolve()
         assert torch.isfinite(img).all()
     batch.close()
+
+
+def test_batch_scenes_that_render_alone(gpu):
+    """Scenes above the solo threshold render with the single-scene launch sequence after the
+    rounds, in pieces of at most round_paths paths, with one queue storage for all of them: films
+    bit-identical to FrameRenderer's, with some scenes alone and the rest in rounds, and with every
+    scene alone (no rounds at all)."""
+    import torch
+    sc = _scenes()
+    refs = []
+    for integ, cam in sc:
+        f = DeviceFilm(cam)
+        FrameRenderer(cam, integ, f, batch_paths=1 << 12).render()
+        torch.cuda.synchronize()
+        refs.append(np.concatenate([t.cpu().numpy() for t in (f.light, f.weight, f.albedo, f.normal)]))
+    counts = [cam.path_count for _, cam in sc]
+    for solo_paths, want_solo in ((2000, sum(c > 2000 for c in counts)), (1, len(sc))):
+        entries = [BatchEntry(integ, cam, DeviceFilm(cam)) for integ, cam in sc]
+        # round_paths 3000: the 6400-path scene renders alone in pieces of 3000, 3000 and 400
+        batch = SceneBatch(entries, round_paths=3000, solo_paths=solo_paths)
+        info = batch.info()
+        assert info.soloScenes == want_solo and 0 < want_solo
+        assert info.totalPaths == sum(counts)
+        assert (info.rounds == 0) == (want_solo == len(sc))
+        for mode in (False, True, True):
+            batch.clear_films()
+            batch.render(use_graph=mode)
+            junk = torch.randn(1 << 16, device="cuda").sum()
+            torch.cuda.synchronize()
+            assert float(junk) == float(junk)
+            for k, (got, ref) in enumerate(zip(_films(entries), refs)):
+                assert got.tobytes() == ref.tobytes(), (solo_paths, mode, k)
+        batch.check()
+        batch.close()
+    with pytest.raises(RuntimeError, match="soloPaths"):
+        SceneBatch([BatchEntry(integ, cam, DeviceFilm(cam)) for integ, cam in sc], solo_paths=-1)
 
 
 def test_batch_rejects_bad_input(gpu):
