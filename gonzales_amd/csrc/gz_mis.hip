@@ -705,6 +705,7 @@ struct MisScene {
         float totalPower = 0.0f;
         bool hasSky = false;
         float sky[3] = {0, 0, 0};
+        uint8_t *dPrimClass = nullptr;  // per primitive: prim_class_of (depends on the materials)
         bool allDiffuse = true;  // picks the kernel built with the diffuse BSDF only
         bool anyLayered = false;  // some material is the layered (coated conductor) BSDF
         bool anyOther = false;    // some material is non-diffuse and not layered
@@ -712,7 +713,7 @@ struct MisScene {
 
 void release_mis(MisScene *m) {
         if (!m) return;
-        void *ptrs[] = {m->dMats, m->dLights, m->dPower, m->dCumulative};
+        void *ptrs[] = {m->dMats, m->dLights, m->dPower, m->dCumulative, m->dPrimClass};
         for (void *p : ptrs)
                 if (p) (void)hipFree(p);
         delete m;
@@ -763,6 +764,15 @@ __global__ __launch_bounds__(kBlock) void k_prim_tris(const PrimId_C *__restrict
         out[3 * i] = ok ? make_float4(p0.x, p0.y, p0.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
         out[3 * i + 1] = ok ? make_float4(p1.x, p1.y, p1.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         out[3 * i + 2] = ok ? make_float4(p2.x, p2.y, p2.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_prim_class(const PrimId_C *__restrict__ prims, int64_t n,
+                                                       const float4 *__restrict__ primTris,
+                                                       const GzMaterial *__restrict__ mats, int nMats,
+                                                       uint8_t *__restrict__ out) {
+        const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        if (i >= n) return;
+        out[i] = prim_class_of(prims[i], primTris + 3 * i, mats, nMats);
 }
 
 // Fills Scene::dPrimTris once per handle (synchronously, on the handle's stream).
@@ -837,10 +847,21 @@ static int set_mis_scene(Scene *s, const GzMaterial *materials, int32_t material
         }
         if (skyRadiance)
                 for (int k = 0; k < 3; ++k) m->sky[k] = skyRadiance[k];
-        const bool ok = put(&m->dMats, materials, (size_t)materialCount, "upload materials") &&
-                        put(&m->dLights, dl.data(), dl.size(), "upload lights") &&
-                        put(&m->dPower, power.data(), power.size(), "upload light powers") &&
-                        put(&m->dCumulative, cum.data(), cum.size(), "upload light CDF");
+        bool ok = put(&m->dMats, materials, (size_t)materialCount, "upload materials") &&
+                  put(&m->dLights, dl.data(), dl.size(), "upload lights") &&
+                  put(&m->dPower, power.data(), power.size(), "upload light powers") &&
+                  put(&m->dCumulative, cum.data(), cum.size(), "upload light CDF");
+        if (ok && s->primCount > 0) {
+                ok = !alloc_fault("primitive classes") &&
+                     hip_ok(hipMalloc((void **)&m->dPrimClass, (size_t)s->primCount), "hipMalloc(primitive classes)");
+                if (ok) {
+                        hipLaunchKernelGGL(k_prim_class, dim3((unsigned)((s->primCount + kBlock - 1) / kBlock)), dim3(kBlock),
+                                           0, s->stream, s->dPrims, s->primCount, s->dPrimTris, m->dMats, materialCount,
+                                           m->dPrimClass);
+                        ok = hip_ok(hipGetLastError(), "primitive classes launch") &&
+                             hip_ok(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+                }
+        }
         if (!ok) {
                 release_mis(m);
                 return GZ_E_HIP;
@@ -929,6 +950,7 @@ bool gz::mis_args(Scene *s, const GzMisParams *params, int *err, MisArgs &a) {
         a.prims = s->dPrims;
         a.primCount = s->primCount;
         a.primTris = s->dPrimTris;
+        a.primClass = ms->dPrimClass;
         a.meshes = s->dMeshes;
         a.meshCount = s->meshCount;
         a.mats = ms->dMats;

@@ -52,6 +52,7 @@ struct MisArgs {
         const PrimId_C *prims;
         int64_t primCount;
         const float4 *primTris;  // Scene::dPrimTris
+        const uint8_t *primClass;  // per primitive: prim_class_of (set with the materials)
         const DevMesh *meshes;
         int64_t meshCount;
         const GzMaterial *mats;
@@ -182,6 +183,37 @@ __device__ bool surface_interaction(const MisArgs &a, const Ray_C &ray, int prim
                 si.material = (int)pr.materialIndex;
         }
         return true;
+}
+
+// The class byte of a primitive (MisScene::dPrimClass, rebuilt by every gz_gpu_set_mis_scene):
+// bits 0-1 the BSDF set its hits shade with (hit_bsdf_set: the material's, materials[0] for an
+// area-light triangle, an index out of range read as 0), kClsLight for an area-light triangle,
+// kClsFull where surface_interaction does not return an ordinary surface (vertex indices out of
+// range, or a zero geometric normal: the same float operations as there).
+enum : uint8_t { kClsSetMask = 3, kClsLight = 4, kClsFull = 8 };
+
+__device__ __forceinline__ uint8_t prim_class_of(const PrimId_C &pr, const float4 *q, const GzMaterial *mats,
+                                                 int nMats) {
+        int m = pr.type == 3 ? 0 : (int)pr.materialIndex;
+        if (m < 0 || m >= nMats) m = 0;
+        const int t = mats[m].type;
+        uint8_t c = (uint8_t)(t == GZ_MAT_DIFFUSE ? kBsdfDiffuse : t == GZ_MAT_COATED_CONDUCTOR ? kBsdfAll : kBsdfNoLayer);
+        if (pr.type == 3) c |= kClsLight;
+        const float4 q0 = q[0], q1 = q[1], q2 = q[2];
+        if (q0.w != 0.0f) return (uint8_t)(c | kClsFull);
+        const f3 p0 = mk(q0.x, q0.y, q0.z), p1 = mk(q1.x, q1.y, q1.z), p2 = mk(q2.x, q2.y, q2.z);
+        const f3 gn = cross(sub(p2, p0), sub(p1, p0));
+        if (len2(gn) == 0.0f) c |= kClsFull;
+        return c;
+}
+
+// Whether the closest hit `prim` of a BSDF-sampled ray needs its surface interaction for the BSDF
+// strategy's term: an area-light triangle (its one-sided emission), or a primitive on which
+// surface_interaction fails (the term then falls back to the sky, as for a miss). Any other hit
+// adds nothing, and the class byte says so without loading the hit's primId and vertices.
+__device__ __forceinline__ bool bsdf_hit_needs_interaction(const MisArgs &a, int prim) {
+        if (prim < 0 || prim >= a.primCount) return true;
+        return (a.primClass[prim] & (kClsLight | kClsFull)) != 0;
 }
 
 // The material's BSDF in its shading frame (BsdfFrame.swift:96-104, ShadingFrame.swift:27-77,
@@ -559,8 +591,10 @@ __device__ __forceinline__ bool bounce_rest(const MisArgs &a, GzPath *__restrict
                         Interaction bsi;
                         f3 bL = mk(0.0f, 0.0f, 0.0f);
                         bool ok = false;
-                        if (trace_ray<false, kStats>(a, r, INFINITY, bprim, bt, bu, bv, w) &&
-                            surface_interaction(a, r, bprim, bu, bv, bsi)) {
+                        const bool bhit = trace_ray<false, kStats>(a, r, INFINITY, bprim, bt, bu, bv, w);
+                        if (bhit && !bsdf_hit_needs_interaction(a, bprim)) {
+                                // a surface that emits nothing: no term
+                        } else if (bhit && surface_interaction(a, r, bprim, bu, bv, bsi)) {
                                 if (bsi.areaLight >= 0 && bsi.areaLight < a.nLights) {
                                         const float4 Lb = a.lights[bsi.areaLight].L;
                                         bL = dot(bsi.normal, bwi) > 0.0f ? mk(Lb.x, Lb.y, Lb.z) : mk(0.0f, 0.0f, 0.0f);
@@ -786,7 +820,9 @@ __device__ __forceinline__ void finish_phase(const MisArgs &a, GzPath *__restric
                 Interaction bsi;
                 f3 bL = mk(0.0f, 0.0f, 0.0f);
                 bool ok = false;
-                if (mh.prim >= 0 && surface_interaction(a, mray, mh.prim, mh.u, mh.v, bsi)) {
+                if (mh.prim >= 0 && !bsdf_hit_needs_interaction(a, mh.prim)) {
+                        // a surface that emits nothing: no term
+                } else if (mh.prim >= 0 && surface_interaction(a, mray, mh.prim, mh.u, mh.v, bsi)) {
                         if (bsi.areaLight >= 0 && bsi.areaLight < a.nLights) {
                                 const float4 Lb = a.lights[bsi.areaLight].L;
                                 bL = dot(bsi.normal, bwi) > 0.0f ? mk(Lb.x, Lb.y, Lb.z) : mk(0.0f, 0.0f, 0.0f);
@@ -832,11 +868,7 @@ __device__ __forceinline__ bool bounce_one(const MisArgs &a, GzPath *__restrict_
 // the layered BSDF, kBsdfNoLayer for the other BSDFs.
 __device__ __forceinline__ int hit_bsdf_set(const MisArgs &a, int prim) {
         if (prim < 0 || prim >= a.primCount) return kBsdfDiffuse;
-        const PrimId_C pr = a.prims[prim];
-        int m = pr.type == 3 ? 0 : (int)pr.materialIndex;
-        if (m < 0 || m >= a.nMats) m = 0;
-        const int t = a.mats[m].type;
-        return t == GZ_MAT_DIFFUSE ? kBsdfDiffuse : t == GZ_MAT_COATED_CONDUCTOR ? kBsdfAll : kBsdfNoLayer;
+        return a.primClass[prim] & kClsSetMask;
 }
 // Whether the hit needs a kernel with more than the diffuse BSDF.
 __device__ __forceinline__ bool hit_needs_all_bsdfs(const MisArgs &a, int prim) {
