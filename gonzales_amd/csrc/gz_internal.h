@@ -71,14 +71,14 @@ int mis_trace_shared(Scene *s, GzPath *d_paths, int64_t count, const GzMisParams
 //   dCounter : the ray counters of the persistent traversal kernels (one, or one per XCD: eight),
 //              reset by a kernel at the start of each launch; launches on one stream run in order,
 //              so they can share them.
-//   dBin     : the ray binning's storage (sorted batch, keys, counts), grow-only, binRays rays;
+//   dBin     : the ray binning's storage (sorted batch, keys, counts), grow-only, binBytes bytes;
 //   ev       : with recordTimes, events before the binning, before and after the traversal kernel
 //              of the last launch (created at first use); timed: whether they hold one.
 struct StreamSlot {
         int *dErr = nullptr;
         unsigned long long *dCounter = nullptr;
         uint32_t *dBin = nullptr;
-        int64_t binRays = 0;
+        size_t binBytes = 0;
         hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
         bool timed = false;
 };
