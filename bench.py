@@ -238,6 +238,7 @@ def path_cpu_leg(args, bh, integ, cam, sky, host: dict, dev, label: str, seconds
 
 
 PATH_RECORD_BYTES = 224  # GzPath read + write per bounce
+RECORD_BYTES = 33  # replay record per path and level: path-ray hit, BSDF-ray hit, shadow visibility
 SHADING_FETCH_BYTES = 160  # PrimId 32 + mesh view 40 + 3 vertex indices 24 + 3 points 48 + material 16
 
 
@@ -742,9 +743,10 @@ def adjoint_gradient_check(args, bh, gz, ta, tr, albedo, rough, dev) -> dict:
 def adjoint_leg(args, dev, stream) -> dict:
     """Inverse rendering: the Cornell box with three displaced occluders (200,320 triangles) whose
     patches carry 32 diffuse (reflectance) and 32 conductor (roughness) parameters; 512 x 512 x 32
-    spp, max depth 5. Forward = one frame (camera + bounces + film); backward = the per-path
-    adjoints from dLoss/dImage of an L2 loss against a 256-spp target image + the path-replay
-    adjoint of every path."""
+    spp, max depth 5. Forward = one frame (camera + bounces + film), keeping the replay record of
+    the wavefront levels; backward = the per-path adjoints from dLoss/dImage of an L2 loss against a
+    256-spp target image + the path-replay adjoint of every path from that record (the rays of the
+    levels past the record traced again). The backward that traces every ray again is timed too."""
     import torch
 
     from gonzales_amd import scenes
@@ -786,6 +788,20 @@ def adjoint_leg(args, dev, stream) -> dict:
         bwd.append(ev[1].elapsed_time(ev[2]))
     loss = float(((img.double() - target) ** 2).sum())
     f, b = float(np.median(fwd)), float(np.median(bwd))
+    rec_levels = r.recorded_levels
+    # the same backward without the record: every ray traced again
+    r.record = False
+    img = r.render()
+    g_img = 2 * (img.double() - target)
+    bwd_traced = []
+    for _ in range(args.adj_steps):
+        ev[1].record(stream)
+        r.backward(g_img)
+        ev[2].record(stream)
+        torch.cuda.synchronize()
+        bwd_traced.append(ev[1].elapsed_time(ev[2]))
+    b_traced = float(np.median(bwd_traced))
+    r.record = True
     grad_check = adjoint_gradient_check(args, bh, gz, ta, tr, albedo, rough, dev) if args.adj_check_seeds > 0 else None
     # work counts of the frame's paths (untimed): the replay traces the same rays and reads and
     # writes the same records as the forward trace
@@ -795,16 +811,31 @@ def adjoint_leg(args, dev, stream) -> dict:
     r.d_paths.copy_(r.d_start)
     r.integrator.trace_stats(r.d_paths, d_st)
     st = d_st.view(n, 3).to(torch.int64).sum(0).tolist()
+    # the levels past the record: the same paths advanced by the recorded levels' bounces first
+    r.d_paths.copy_(r.d_start)
+    for _ in range(rec_levels):
+        r.integrator.bounce(r.d_paths)
+    d_st.zero_()
+    r.integrator.trace_stats(r.d_paths, d_st)
+    tail = d_st.view(n, 3).to(torch.int64).sum(0).tolist()
     bh.check_traversal()
-    trace_bytes = (PATH_RECORD_BYTES + SHADING_FETCH_BYTES) * st[2] + 64 * st[0] + 40 * st[1]
-    # forward: camera records written + trace + film read; backward: the start records copied,
-    # the per-path adjoint input (radiance + pixel adjoint, 24 B written and read) + the replay
-    fwd_bytes = 2 * abi.GZ_PATH.itemsize * n + trace_bytes
-    bwd_bytes = 2 * abi.GZ_PATH.itemsize * n + 48 * n + trace_bytes
+    shade_bytes = (PATH_RECORD_BYTES + SHADING_FETCH_BYTES) * st[2]
+    trace_bytes = shade_bytes + 64 * st[0] + 40 * st[1]
+    # forward: camera records written + trace + film read + the record (33 B per path and recorded
+    # level, cleared and written); backward: the start records copied, the per-path adjoint input
+    # (radiance + pixel adjoint, 24 B written and read), the replay's shading, the record read for
+    # the recorded bounces and the rays of the later levels traced again
+    fwd_bytes = 2 * abi.GZ_PATH.itemsize * n + trace_bytes + 2 * RECORD_BYTES * n * rec_levels
+    bwd_bytes = (2 * abi.GZ_PATH.itemsize * n + 48 * n + shade_bytes + RECORD_BYTES * (st[2] - tail[2]) +
+                 64 * tail[0] + 40 * tail[1])
+    bwd_traced_bytes = 2 * abi.GZ_PATH.itemsize * n + 48 * n + trace_bytes
     res = {"workload": f"inverse rendering: Cornell box + 2 displaced icospheres ({len(bh.prim_ids)} triangles), "
                        f"32 diffuse + 32 conductor parameter materials, {args.adj_res}x{args.adj_res} x "
                        f"{args.adj_spp} spp, max depth 5, L2 loss vs a {args.adj_target_spp}-spp target",
            "forward_ms": round(f, 3), "backward_ms": round(b, 3), "adjoint_over_forward": round(b / f, 3),
+           "recorded_levels": rec_levels, "record_bytes": RECORD_BYTES * n * rec_levels,
+           "backward_traced_ms": round(b_traced, 3), "adjoint_over_forward_traced": round(b_traced / f, 3),
+           "backward_traced_bytes_per_path": round(bwd_traced_bytes / n, 1),
            "loss": loss, "grad_albedo_norm": float(ga.norm()), "grad_roughness_norm": float(gr.norm()),
            "bounces_per_path": round(st[2] / n, 3), "forward_bytes_per_path": round(fwd_bytes / n, 1),
            "backward_bytes_per_path": round(bwd_bytes / n, 1),

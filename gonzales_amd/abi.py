@@ -187,6 +187,9 @@ SIGNATURES = {
     "gz_gpu_mis_trace_device": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
     "gz_gpu_mis_trace_stats_device": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
     "gz_gpu_mis_adjoint_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P]),
+    "gz_mis_record_bytes": (ctypes.c_int64, [_I64, ctypes.c_int32]),
+    "gz_gpu_mis_trace_record_device": (ctypes.c_int, [_P, _P, _I64, _P, _P, ctypes.c_int32, _P]),
+    "gz_gpu_mis_adjoint_record_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P, ctypes.c_int32, _P]),
     "gz_gpu_set_mis_launch": (ctypes.c_int, [_P, _P]),
     "gz_gpu_get_mis_launch": (ctypes.c_int, [_P, _P]),
     "gz_gpu_release_mis_queues": (ctypes.c_int, [_P]),

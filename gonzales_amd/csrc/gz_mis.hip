@@ -149,6 +149,16 @@ struct WfLevel {
 };
 static_assert(sizeof(WfLevel) == 64, "WfLevel");
 
+// Marks every entry of a replay record as not traced (the forward pass then fills the rays it traces).
+__global__ __launch_bounds__(kBlock) void k_record_clear(GzHit *__restrict__ hits, GzHit *__restrict__ mhits,
+                                                         uint8_t *__restrict__ occ, int64_t n) {
+        for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock) {
+                hits[j] = GzHit{kRecNone, 0.0f, 0.0f, 0.0f};
+                mhits[j] = GzHit{kRecNone, 0.0f, 0.0f, 0.0f};
+                occ[j] = kRecNoOcc;
+        }
+}
+
 __global__ void k_wf_clear(WfLevel *ctl, int levels) {
         for (int k = threadIdx.x; k < levels; k += blockDim.x) {
                 WfLevel z;
@@ -1016,11 +1026,29 @@ static void refresh_default_launch(Scene *s) {
                                             : GzMisLaunch{1, 256, 1, 256, 3, {0, 0, 0}};
 }
 
+constexpr int kMaxRecordLevels = 1025;  // maxDepth + 1 for the largest maxDepth (1024)
+
+// The three arrays of a replay record of `levels` levels of `count` paths at `base`.
+struct MisRecord {
+        GzHit *hits = nullptr, *mhits = nullptr;
+        uint8_t *occ = nullptr;
+        int levels = 0;
+};
+static MisRecord record_at(void *base, int64_t count, int levels) {
+        MisRecord r;
+        r.hits = static_cast<GzHit *>(base);
+        r.mhits = r.hits + (size_t)levels * (size_t)count;
+        r.occ = reinterpret_cast<uint8_t *>(r.mhits + (size_t)levels * (size_t)count);
+        r.levels = levels;
+        return r;
+}
+
 // queues: the queue storage to use (null: the handle's own); err: the flag traversal overflows
-// raise (null: the handle's, per stream).
+// raise (null: the handle's, per stream); rec: a replay record to fill (full wavefront levels
+// below the drain level; every other entry stays marked as not traced).
 static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisParams *params,
                       GzPathStats *d_stats, void *stream, bool toEnd, const char *who, MisQueues **queues = nullptr,
-                      int *err = nullptr) {
+                      int *err = nullptr, const MisRecord *rec = nullptr) {
         clear_error();
         if (!handle || count < 0 || (count > 0 && !d_paths)) {
                 set_error("%s: bad arguments", who);
@@ -1042,6 +1070,13 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
         }
         refresh_default_launch(s);
         const hipStream_t st = static_cast<hipStream_t>(stream);
+        if (rec && rec->levels > 0) {
+                const int64_t entries = (int64_t)rec->levels * count;
+                const unsigned g = (unsigned)std::max<int64_t>(
+                        1, std::min<int64_t>((int64_t)s->cuCount * 8, (entries + kBlock - 1) / kBlock));
+                hipLaunchKernelGGL(k_record_clear, dim3(g), dim3(kBlock), 0, st, rec->hits, rec->mhits, rec->occ, entries);
+                if (!hip_ok(hipGetLastError(), who)) return GZ_E_HIP;
+        }
         uint32_t *stats = reinterpret_cast<uint32_t *>(d_stats);
         const bool all = !s->mis->allDiffuse;
         const int bset = mis_bsdf_set(s);
@@ -1091,35 +1126,41 @@ static int mis_launch(void *handle, GzPath *d_paths, int64_t count, const GzMisP
                         const int *qin = level == 0 ? nullptr : q->q[level & 1];
                         int *qout = q->q[(level + 1) & 1];
                         WfLevel *lv = q->wctl + level;
+                        // this level's closest hits, shadow results and BSDF-ray hits: in the replay
+                        // record when it keeps the level, else in the queue storage
+                        const bool keep = rec && level < rec->levels;
+                        GzHit *hits = keep ? rec->hits + (size_t)level * (size_t)count : q->hits;
+                        GzHit *mhits = keep ? rec->mhits + (size_t)level * (size_t)count : q->mhits;
+                        uint8_t *occ = keep ? rec->occ + (size_t)level * (size_t)count : q->occ;
                         if (c.drainLevel > 0 && level >= c.drainLevel) {
                                 hipLaunchKernelGGL(kdrain, dim3(gdrain), dim3(c.block), 0, st, a, d_paths, qin, &lv->qn,
                                                    &lv->qhead, chunk, stats);
                                 return hip_ok(hipGetLastError(), who) ? GZ_OK : GZ_E_HIP;
                         }
                         hipLaunchKernelGGL(ktrace, dim3(tgrid), dim3(kBlock), 0, st, a, d_paths, (unsigned)count, qin,
-                                           q->wctl, level, q->hits, refill, tchunk, stats);
+                                           q->wctl, level, hits, refill, tchunk, stats);
                         hipLaunchKernelGGL(k_wf_split, dim3(spgrid), dim3(kMisBlock), 0, st, a, (unsigned)count, qin,
-                                           q->wctl, level, q->hits, q->qd, q->qa, q->qn, classes);
-                        hipLaunchKernelGGL(kd, dim3(gd), dim3(c.block), 0, st, a, d_paths, q->hits, q->qd, q->wctl, level,
+                                           q->wctl, level, hits, q->qd, q->qa, q->qn, classes);
+                        hipLaunchKernelGGL(kd, dim3(gd), dim3(c.block), 0, st, a, d_paths, hits, q->qd, q->wctl, level,
                                            qout, q->qs, q->qm, q->pend, q->srays, q->mrays, chunk, stats);
                         if (all && layered)
-                                hipLaunchKernelGGL(ka, dim3(ga), dim3(c.block), 0, st, a, d_paths, q->hits, q->qa, q->wctl,
+                                hipLaunchKernelGGL(ka, dim3(ga), dim3(c.block), 0, st, a, d_paths, hits, q->qa, q->wctl,
                                                    level, qout, q->qs, q->qm, q->pend, q->srays, q->mrays, chunk, stats);
                         if (all && other)
-                                hipLaunchKernelGGL(kn, dim3(gn), dim3(c.block), 0, st, a, d_paths, q->hits, q->qn, q->wctl,
+                                hipLaunchKernelGGL(kn, dim3(gn), dim3(c.block), 0, st, a, d_paths, hits, q->qn, q->wctl,
                                                    level, qout, q->qs, q->qm, q->pend, q->srays, q->mrays, chunk, stats);
                         hipLaunchKernelGGL(kany, dim3(agrid), dim3(kBlock), 0, st, a, d_paths, (unsigned)count, q->qs,
-                                           &lv->sn, q->srays, q->occ, nullptr, stats);
+                                           &lv->sn, q->srays, occ, nullptr, stats);
                         hipLaunchKernelGGL(kclose, dim3(cgrid), dim3(kBlock), 0, st, a, d_paths, (unsigned)count, q->qm,
-                                           &lv->mn, q->mrays, nullptr, q->mhits, stats);
+                                           &lv->mn, q->mrays, nullptr, mhits, stats);
                         hipLaunchKernelGGL(k_wf_finish, dim3(sgrid), dim3(kMisBlock), 0, st, a, d_paths, q->qd, &lv->dn,
-                                           q->pend, q->occ, q->mrays, q->mhits);
+                                           q->pend, occ, q->mrays, mhits);
                         if (all && layered)
                                 hipLaunchKernelGGL(k_wf_finish, dim3(sgrid), dim3(kMisBlock), 0, st, a, d_paths, q->qa,
-                                                   &lv->an, q->pend, q->occ, q->mrays, q->mhits);
+                                                   &lv->an, q->pend, occ, q->mrays, mhits);
                         if (all && other)
                                 hipLaunchKernelGGL(k_wf_finish, dim3(sgrid), dim3(kMisBlock), 0, st, a, d_paths, q->qn,
-                                                   &lv->nn, q->pend, q->occ, q->mrays, q->mhits);
+                                                   &lv->nn, q->pend, occ, q->mrays, mhits);
                         if (!hip_ok(hipGetLastError(), who)) return GZ_E_HIP;
                 }
                 return GZ_OK;
@@ -1289,8 +1330,9 @@ extern "C" int gz_gpu_mis_trace_stats_device(void *handle, GzPath *d_paths, int6
         return mis_launch(handle, d_paths, count, params, d_stats, stream, true, "gz_gpu_mis_trace_stats_device");
 }
 
-extern "C" int gz_gpu_mis_adjoint_device(void *handle, GzPath *d_paths, const GzPathAdjoint *d_adjoint,
-                                         int64_t count, const GzMisParams *params, double *d_grad, void *stream) {
+// The adjoint replay; rec: the forward pass's replay record (null: every ray is traced again).
+static int mis_adjoint(void *handle, GzPath *d_paths, const GzPathAdjoint *d_adjoint, int64_t count,
+                       const GzMisParams *params, double *d_grad, void *stream, const MisRecord *rec) {
         clear_error();
         if (!handle || count < 0 || (count > 0 && (!d_paths || !d_adjoint || !d_grad))) {
                 set_error("gz_gpu_mis_adjoint_device: bad arguments");
@@ -1310,7 +1352,14 @@ extern "C" int gz_gpu_mis_adjoint_device(void *handle, GzPath *d_paths, const Gz
                 return GZ_E_INVALID;
         }
         const hipStream_t st = static_cast<hipStream_t>(stream);
-        const AdjArgs adj{d_adjoint, d_grad};
+        AdjArgs adj{d_adjoint, d_grad};
+        if (rec && rec->levels > 0) {
+                adj.recHits = rec->hits;
+                adj.recMhits = rec->mhits;
+                adj.recOcc = rec->occ;
+                adj.recLevels = rec->levels;
+                adj.recCount = count;
+        }
         refresh_default_launch(s);
         if (s->misLaunch.queued) {
                 if (count > 0x7fffffffLL) {
@@ -1341,4 +1390,38 @@ extern "C" int gz_gpu_mis_adjoint_device(void *handle, GzPath *d_paths, const Gz
                 if (!hip_ok(hipGetLastError(), "MIS adjoint launch")) return GZ_E_HIP;
         }
         return GZ_OK;
+}
+
+extern "C" int gz_gpu_mis_adjoint_device(void *handle, GzPath *d_paths, const GzPathAdjoint *d_adjoint,
+                                         int64_t count, const GzMisParams *params, double *d_grad, void *stream) {
+        return mis_adjoint(handle, d_paths, d_adjoint, count, params, d_grad, stream, nullptr);
+}
+
+extern "C" int64_t gz_mis_record_bytes(int64_t count, int32_t levels) {
+        if (count < 0 || levels < 0 || levels > kMaxRecordLevels) return -1;
+        return count * (int64_t)levels * (int64_t)(2 * sizeof(GzHit) + 1);
+}
+
+extern "C" int gz_gpu_mis_trace_record_device(void *handle, GzPath *d_paths, int64_t count, const GzMisParams *params,
+                                              void *d_record, int32_t levels, void *stream) {
+        if (levels < 0 || levels > kMaxRecordLevels || (levels > 0 && count > 0 && !d_record)) {
+                clear_error();
+                set_error("gz_gpu_mis_trace_record_device: need 0 <= levels <= %d and a record", kMaxRecordLevels);
+                return GZ_E_INVALID;
+        }
+        const MisRecord rec = record_at(d_record, count, levels);
+        return mis_launch(handle, d_paths, count, params, nullptr, stream, true, "gz_gpu_mis_trace_record_device",
+                          nullptr, nullptr, &rec);
+}
+
+extern "C" int gz_gpu_mis_adjoint_record_device(void *handle, GzPath *d_paths, const GzPathAdjoint *d_adjoint,
+                                                int64_t count, const GzMisParams *params, double *d_grad,
+                                                const void *d_record, int32_t levels, void *stream) {
+        if (levels < 0 || levels > kMaxRecordLevels || (levels > 0 && count > 0 && !d_record)) {
+                clear_error();
+                set_error("gz_gpu_mis_adjoint_record_device: need 0 <= levels <= %d and a record", kMaxRecordLevels);
+                return GZ_E_INVALID;
+        }
+        const MisRecord rec = record_at(const_cast<void *>(d_record), count, levels);
+        return mis_adjoint(handle, d_paths, d_adjoint, count, params, d_grad, stream, &rec);
 }

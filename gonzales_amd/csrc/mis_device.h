@@ -398,10 +398,27 @@ __device__ __forceinline__ float rr_draw(uint64_t seed, int px, int py, int si, 
 
 // Path-replay adjoint state (gz_gpu_mis_adjoint_device): per path its forward radiance and the
 // adjoint of its radiance; gradients accumulate per material as (dL/dR r, g, b, dL/droughness).
+// A replay record of the forward pass (gz_gpu_mis_trace_record_device): for bounce level L < recLevels
+// and path i, at index L * recCount + i, the closest hit of the path's ray (recHits), the closest hit
+// of its BSDF-sampled ray (recMhits) and its shadow ray's visibility (recOcc); prim kRecNone or occ
+// kRecNoOcc where the forward pass did not trace that ray (the replay traces it then).
+constexpr int kRecNone = -2;
+constexpr uint8_t kRecNoOcc = 2;
+
 struct AdjArgs {
         const GzPathAdjoint *paths;
         double *grad;  // [materials][4] in global memory
+        const GzHit *recHits = nullptr;
+        const GzHit *recMhits = nullptr;
+        const uint8_t *recOcc = nullptr;
+        int recLevels = 0;
+        int64_t recCount = 0;
 };
+
+// The record's entry of bounce `bounce` of path i, or -1 when the record has no such level.
+__device__ __forceinline__ int64_t rec_index(const AdjArgs &adj, int bounce, int64_t i) {
+        return bounce < adj.recLevels ? (int64_t)bounce * adj.recCount + i : -1;
+}
 constexpr int kAdjLdsMats = 128;
 
 // Log-derivatives in alpha of an isotropic Trowbridge-Reitz conductor (MicrofacetReflection.swift:
@@ -553,7 +570,13 @@ __device__ __forceinline__ bool bounce_rest(const MisArgs &a, GzPath *__restrict
                 if (kAdj) {
                         // the reference's order: the shadow ray first (the gradient of a zero term
                         // can be non-zero, so the adjoint needs its visibility)
-                        if (!cblack(ls.L) && !isinf(ls.pdf) && !occluded<kStats>(a, si.position, ls.target, w)) {
+                        bool vis = false;
+                        if (!cblack(ls.L) && !isinf(ls.pdf)) {
+                                const int64_t ri = rec_index(adj, p.bounce, i);
+                                const uint8_t o = ri >= 0 ? adj.recOcc[ri] : kRecNoOcc;
+                                vis = o != kRecNoOcc ? o == 0 : !occluded<kStats>(a, si.position, ls.target, w);
+                        }
+                        if (vis) {
                                 LayerRng r1 = layer_rng(a.rrSeed, p.pixelX, p.pixelY, p.sampleIndex, p.bounce, kCallLightF);
                                 const f3 scatter = mul(bsdf_eval_world<kAll>(b, si.outgoing, ls.dir, r1),
                                                        absdot(ls.dir, si.shadingNormal));
@@ -591,7 +614,17 @@ __device__ __forceinline__ bool bounce_rest(const MisArgs &a, GzPath *__restrict
                         Interaction bsi;
                         f3 bL = mk(0.0f, 0.0f, 0.0f);
                         bool ok = false;
-                        const bool bhit = trace_ray<false, kStats>(a, r, INFINITY, bprim, bt, bu, bv, w);
+                        bool bhit;
+                        const int64_t ri = kAdj ? rec_index(adj, p.bounce, i) : -1;
+                        const GzHit rh = ri >= 0 ? adj.recMhits[ri] : GzHit{kRecNone, 0.0f, 0.0f, 0.0f};
+                        if (rh.prim != kRecNone) {
+                                bhit = rh.prim >= 0;
+                                bprim = rh.prim;
+                                bu = rh.u;
+                                bv = rh.v;
+                        } else {
+                                bhit = trace_ray<false, kStats>(a, r, INFINITY, bprim, bt, bu, bv, w);
+                        }
                         if (bhit && !bsdf_hit_needs_interaction(a, bprim)) {
                                 // a surface that emits nothing: no term
                         } else if (bhit && surface_interaction(a, r, bprim, bu, bv, bsi)) {
@@ -858,8 +891,18 @@ __device__ __forceinline__ bool bounce_one(const MisArgs &a, GzPath *__restrict_
         }
         Work w;
         int prim = -1;
-        float th, hu, hv;
-        const bool traced = trace_ray<false, kStats>(a, p.ray, INFINITY, prim, th, hu, hv, w);
+        float th, hu = 0.0f, hv = 0.0f;
+        bool traced;
+        const int64_t ri = kAdj ? rec_index(adj, p.bounce, i) : -1;
+        const GzHit rh = ri >= 0 ? adj.recHits[ri] : GzHit{kRecNone, 0.0f, 0.0f, 0.0f};
+        if (rh.prim != kRecNone) {
+                traced = rh.prim >= 0;
+                prim = rh.prim;
+                hu = rh.u;
+                hv = rh.v;
+        } else {
+                traced = trace_ray<false, kStats>(a, p.ray, INFINITY, prim, th, hu, hv, w);
+        }
         return bounce_rest<kStats, kAll, kAdj>(a, paths, i, p, traced, prim, hu, hv, w, stats, adj, lds);
 }
 

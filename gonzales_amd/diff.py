@@ -1,6 +1,11 @@
 """Differentiable rendering: a torch.autograd.Function over the GPU MIS integrator whose backward
 pass is the path-replay adjoint kernel (gz_gpu_mis_adjoint_device).
 
+By default the forward pass also keeps a replay record in HBM (gz_gpu_mis_trace_record_device: each
+path's closest hit, BSDF-ray hit and shadow visibility for the wavefront levels, 33 B per path and
+level), and the backward pass replays the paths from it instead of tracing their rays again
+(gz_gpu_mis_adjoint_record_device); the gradients are the same.
+
 Parameters are the reflectance of diffuse materials and the (isotropic) roughness of conductor
 materials of the scene's GzMaterial table. forward renders the image (camera -> bounces -> film)
 and keeps the paths' starting records and radiance; backward turns dLoss/dImage into each path
@@ -22,7 +27,7 @@ class DiffRenderer:
     """Renders a full-frame camera (no crop) with the parameters written into the material table."""
 
     def __init__(self, bh, camera: PerspectiveCamera, materials: np.ndarray, options: MisOptions, sky=None,
-                 device="cuda"):
+                 device="cuda", record: bool = True):
         import torch
         c = camera.c
         if (c.x0, c.y0, c.x1, c.y1) != (0, 0, c.resX, c.resY):
@@ -36,6 +41,9 @@ class DiffRenderer:
         self.d_start = torch.empty(n * abi.GZ_PATH.itemsize, dtype=torch.uint8, device=device)
         self.d_paths = torch.empty_like(self.d_start)
         self.device = device
+        self.record = record
+        self.d_record = None
+        self.recorded_levels = 0  # levels the last render recorded (0: none; backward traces every ray)
 
     def set_parameters(self, albedo=None, roughness=None) -> None:
         """albedo: (M, 3) reflectances (used where the material is diffuse); roughness: (M,)
@@ -50,18 +58,41 @@ class DiffRenderer:
             sel = m["type"] == abi.GZ_MAT_CONDUCTOR
             m["roughness"][sel] = np.stack([r[sel], r[sel]], 1)
         self.integrator = VolumePathIntegrator(self.bh, self.options, sky=self.sky, materials=m)
+        self.recorded_levels = 0  # a record of the old parameters would not match the replay
 
     def render(self):
         """Forward pass: returns the (H, W, 3) image (a torch tensor on the device)."""
         import torch
         self.camera.paths_device(self.d_start)
         self.d_paths.copy_(self.d_start)
-        self.integrator.trace(self.d_paths)
+        levels = self._record_levels() if self.record else 0
+        if levels > 0:
+            lib = abi.load()
+            need = int(lib.gz_mis_record_bytes(self.n, levels))
+            if self.d_record is None or self.d_record.numel() < need:
+                self.d_record = None
+                self.d_record = torch.empty(need, dtype=torch.uint8, device=self.device)
+            abi.check(lib.gz_gpu_mis_trace_record_device(self.bh.handle, abi.ptr(self.d_paths), self.n,
+                                                         ctypes.byref(self.options.params()), abi.ptr(self.d_record),
+                                                         levels, abi.stream_ptr(None)),
+                      "gz_gpu_mis_trace_record_device")
+        else:
+            self.integrator.trace(self.d_paths)
+        self.recorded_levels = levels
         self.film.clear()
         self.film.add(self.d_paths, self.n)
         img = self.film.resolve()
         c = self.camera.c
         return img.view(c.resY, c.resX, 3).clone()
+
+    def _record_levels(self) -> int:
+        """Bounce levels the forward pass can record: those the full wavefront launch runs level by
+        level (below its drain level); none for the other launch shapes."""
+        queued, _, _, _, drain = self.integrator.launch()
+        if queued != 3:
+            return 0
+        levels = self.options.max_depth + 1
+        return min(levels, drain) if drain > 0 else levels
 
     def path_radiance(self):
         """(n, 3) radiance of each path sample of the last render (a strided view of the records)."""
@@ -85,9 +116,14 @@ class DiffRenderer:
         grad = torch.zeros(len(self.materials) * 4, dtype=torch.float64, device=self.device)
         self.d_paths.copy_(self.d_start)
         prm = self.options.params()
-        abi.check(abi.load().gz_gpu_mis_adjoint_device(self.bh.handle, abi.ptr(self.d_paths), abi.ptr(adj), self.n,
-                                                        ctypes.byref(prm), abi.ptr(grad), None),
-                  "gz_gpu_mis_adjoint_device")
+        if self.recorded_levels > 0:
+            abi.check(abi.load().gz_gpu_mis_adjoint_record_device(
+                self.bh.handle, abi.ptr(self.d_paths), abi.ptr(adj), self.n, ctypes.byref(prm), abi.ptr(grad),
+                abi.ptr(self.d_record), self.recorded_levels, None), "gz_gpu_mis_adjoint_record_device")
+        else:
+            abi.check(abi.load().gz_gpu_mis_adjoint_device(self.bh.handle, abi.ptr(self.d_paths), abi.ptr(adj),
+                                                            self.n, ctypes.byref(prm), abi.ptr(grad), None),
+                      "gz_gpu_mis_adjoint_device")
         grad = grad.view(-1, 4)
         return grad[:, :3].clone(), grad[:, 3].clone()
 

@@ -485,6 +485,25 @@ struct GzPathAdjoint {
 int gz_gpu_mis_adjoint_device(void *handle, struct GzPath *d_paths, const struct GzPathAdjoint *d_adjoint,
                               int64_t count, const struct GzMisParams *params, double *d_grad, void *stream);
 
+/* Replay record (no counterpart in the reference): the forward pass keeps, for each of the first
+ * `levels` bounce levels and each path, the closest hit of the path's ray, the closest hit of its
+ * BSDF-sampled ray and its shadow ray's visibility (33 B per path and level, in HBM), so that the
+ * backward pass replays the paths without tracing those rays again. gz_mis_record_bytes: the size
+ * of such a record (-1 for bad arguments).
+ * gz_gpu_mis_trace_record_device: gz_gpu_mis_trace_device that fills d_record. Only the full
+ * wavefront launch (GzMisLaunch.queued 3) records, and only its levels below drainLevel; every
+ * other entry is marked as not traced. Images and path records are those of
+ * gz_gpu_mis_trace_device.
+ * gz_gpu_mis_adjoint_record_device: gz_gpu_mis_adjoint_device over the paths of such a trace (the
+ * same starting records, count, parameters and record); it traces the rays the record lacks.
+ * Gradients equal gz_gpu_mis_adjoint_device's up to the order of the float64 additions. */
+int64_t gz_mis_record_bytes(int64_t count, int32_t levels);
+int gz_gpu_mis_trace_record_device(void *handle, struct GzPath *d_paths, int64_t count,
+                                   const struct GzMisParams *params, void *d_record, int32_t levels, void *stream);
+int gz_gpu_mis_adjoint_record_device(void *handle, struct GzPath *d_paths, const struct GzPathAdjoint *d_adjoint,
+                                     int64_t count, const struct GzMisParams *params, double *d_grad,
+                                     const void *d_record, int32_t levels, void *stream);
+
 /* ---------------------------------------------------------------------------------------- */
 /* Camera samples, primary rays and film (Core/Tile.swift:27-59, :170-219; Core/Film.swift).  */
 /* ---------------------------------------------------------------------------------------- */

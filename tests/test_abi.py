@@ -74,6 +74,9 @@ def test_device_entry_points_fail_cleanly_without_a_handle():
     assert lib.gz_batch_create(None, 0, 1 << 20, abi.GZ_BATCH_SOLO_PATHS) is None
     assert "soloPaths" in abi.last_error()
     assert ctypes.sizeof(abi.GzBatchInfo) == 40
+    assert lib.gz_mis_record_bytes(10, 3) == 10 * 3 * 33 and lib.gz_mis_record_bytes(-1, 3) == -1
+    assert lib.gz_gpu_mis_trace_record_device(None, None, 0, None, None, -1, None) == abi.GZ_E_INVALID
+    assert lib.gz_gpu_mis_adjoint_record_device(None, None, None, 0, None, None, None, 1, None) == abi.GZ_E_INVALID
 
 
 def test_zsobol_rejects_bad_patterns():

@@ -67,6 +67,39 @@ def test_replay_reproduces_the_forward_paths(gpu):
     assert torch.equal(r.d_paths, fwd)
 
 
+def test_recorded_replay_equals_traced_replay(gpu):
+    """The backward pass from the forward pass's replay record (gz_gpu_mis_adjoint_record_device: the
+    recorded levels' rays are not traced again) gives the traced replay's gradients and leaves the
+    same path records, with a record of the levels below the drain level, of every level, and with
+    a launch that records nothing; the forward images are the same with and without the record."""
+    import torch
+    res, spp, depth = 32, 8, 5
+    g = np.random.default_rng(7)
+    r, ta, tr = _setup(res, spp, depth)
+    albedo, rough = _params(ta, tr, g)
+    target = torch.from_numpy(g.uniform(0, 0.5, (res, res, 3))).cuda()
+    plain = DiffRenderer(r.bh, r.camera, r.materials, r.options, record=False)
+    for launch, want in (((3, 256, 1, 64, 3), 3), ((3, 256, 1, 64, 0), depth + 1), ((1, 256, 1, 256, 3), 0)):
+        for rr in (r, plain):
+            rr.set_parameters(albedo, rough)
+            rr.integrator.set_launch(*launch)
+        img = r.render()
+        img0 = plain.render()
+        assert r.recorded_levels == want and plain.recorded_levels == 0
+        assert torch.equal(img, img0), launch
+        ga, gr = r.backward(2 * (img.double() - target))
+        after = r.d_paths.clone()
+        ga0, gr0 = plain.backward(2 * (img0.double() - target))
+        torch.cuda.synchronize()
+        assert torch.equal(after, plain.d_paths), launch
+        for x, y in ((ga, ga0), (gr, gr0)):
+            x, y = x.cpu().numpy(), y.cpu().numpy()
+            assert np.abs(y).max() > 0
+            # the same terms, added in another order by the float64 atomics
+            np.testing.assert_allclose(x, y, rtol=1e-9, atol=1e-12 * np.abs(y).max(), err_msg=str(launch))
+    r.integrator.set_launch(3, 256, 1, 64, 3)
+
+
 def test_reflectance_gradient_matches_finite_differences(gpu):
     import torch
     r, ta, tr = _setup(48, 16, 2)
