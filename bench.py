@@ -743,8 +743,8 @@ def adjoint_gradient_check(args, bh, gz, ta, tr, albedo, rough, dev) -> dict:
 def adjoint_leg(args, dev, stream) -> dict:
     """Inverse rendering: the Cornell box with three displaced occluders (200,320 triangles) whose
     patches carry 32 diffuse (reflectance) and 32 conductor (roughness) parameters; 512 x 512 x 32
-    spp, max depth 5. Forward = one frame (camera + bounces + film), keeping the replay record of
-    the wavefront levels; backward = the per-path adjoints from dLoss/dImage of an L2 loss against a
+    spp, max depth 5. Forward = one frame (camera + bounces + film; the full wavefront for every
+    level), keeping the replay record of every level; backward = the per-path adjoints from dLoss/dImage of an L2 loss against a
     256-spp target image + the path-replay adjoint of every path from that record (the rays of the
     levels past the record traced again). The backward that traces every ray again is timed too."""
     import torch

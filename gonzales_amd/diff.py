@@ -24,7 +24,9 @@ from .integrator import MisOptions, VolumePathIntegrator
 
 
 class DiffRenderer:
-    """Renders a full-frame camera (no crop) with the parameters written into the material table."""
+    """Renders a full-frame camera (no crop) with the parameters written into the material table.
+    record (default): keep the forward pass's replay record and replay from it; this sets the scene
+    handle's launch to the full wavefront without a drain level (every level recorded)."""
 
     def __init__(self, bh, camera: PerspectiveCamera, materials: np.ndarray, options: MisOptions, sky=None,
                  device="cuda", record: bool = True):
@@ -44,6 +46,12 @@ class DiffRenderer:
         self.record = record
         self.d_record = None
         self.recorded_levels = 0  # levels the last render recorded (0: none; backward traces every ray)
+        if record:
+            # the full wavefront for every level (no drain), so that the record holds every level
+            # and the backward traces nothing: on the 512^2 x 32 spp inverse-rendering frame the
+            # forward takes 18.1 ms against 17.8 with the default drain from level 3, the backward
+            # 10.6 against 12.6 ms
+            self.integrator.set_launch(3, 256, 1, 64, 0)
 
     def set_parameters(self, albedo=None, roughness=None) -> None:
         """albedo: (M, 3) reflectances (used where the material is diffuse); roughness: (M,)
