@@ -141,6 +141,10 @@ def parse():
     ap.add_argument("--adj-steps", type=int, default=3)
     ap.add_argument("--adj-check-seeds", type=int, default=8,
                     help="Z-Sobol seeds of the adjoint's central-difference check (0: skip it)")
+    ap.add_argument("--trace-config", type=lambda t: tuple(int(x) for x in t.split(",")), default=None,
+                    help="persistent,lds_stack,refill_lanes,grid_scale,chunk,bin_min_rays for the traversal leg "
+                         "instead of the autotuner's choice (for profiler runs, where the timings it ranks by "
+                         "are distorted)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device copy (reference dispatch for HBM byte counters)")
     ap.add_argument("--only", choices=["path", "full", "batch", "adjoint"], default=None,
@@ -914,8 +918,13 @@ def main() -> int:
     # Launch configuration picked by the autotuner on this batch (untimed).
     from gonzales_amd.autotune import autotune, describe
 
-    best, timings = autotune(bh, d_rays, d_tmax, any_hit=False, sample=n, reps=2)
-    log(f"autotuned launch config: {best}")
+    if args.trace_config is not None:
+        from gonzales_amd.autotune import TraceConfig
+        best, timings = TraceConfig(*args.trace_config), {}
+        log(f"launch config given: {best}")
+    else:
+        best, timings = autotune(bh, d_rays, d_tmax, any_hit=False, sample=n, reps=2)
+        log(f"autotuned launch config: {best}")
     binned = best.bin_min_rays > 0 and n >= best.bin_min_rays
     bh.set_trace_config(best.persistent, best.lds_stack, best.refill_lanes, best.grid_scale, best.chunk,
                         best.bin_min_rays, record_times=1)
@@ -972,7 +981,7 @@ def main() -> int:
              # binning traffic per ray: the count kernel reads the 12-byte origin and writes a 2-byte key, the
              # place kernel reads key, ray and tMax and writes the 32-byte sorted record
              "binning_bytes_per_ray": (12 + 2 + 2 + 28 + 32) if binned else 0,
-             "launch_config": describe({best: timings[best]})[0] if best in timings else None,
+             "launch_config": describe({best: timings.get(best, 0.0)})[0],
              "autotune_ms": describe(timings)}
 
     def timed(fn, reps=5):

@@ -289,7 +289,7 @@ class BoundingHierarchy:
         abi.check(rc, "any_device")
 
     def set_trace_config(self, persistent: int = 3, lds_stack: int = 16, refill_lanes: int = 32,
-                         grid_scale: int = 1, chunk: int = 64, bin_min_rays: int = 1 << 18,
+                         grid_scale: int = 1, chunk: int = 128, bin_min_rays: int = 1 << 18,
                          record_times: int = 0) -> None:
         cfg = abi.GzTraceConfig(persistent, lds_stack, refill_lanes, grid_scale, chunk, bin_min_rays, record_times)
         abi.check(abi.load().gz_gpu_set_trace_config(self.handle, ctypes.byref(cfg)), "gz_gpu_set_trace_config")

@@ -22,12 +22,14 @@ class TraceConfig:
     lds_stack: int = 16
     refill_lanes: int = 32
     grid_scale: int = 1
-    chunk: int = 64
+    chunk: int = 128
     bin_min_rays: int = 1 << 18
 
 
 # binned by origin cell (the default bin_min_rays) and in the given order (bin_min_rays 0)
 CANDIDATES = (
+    TraceConfig(3, 16, 32, 1, 128),
+    TraceConfig(3, 8, 32, 1, 128),
     TraceConfig(3, 16, 32, 1, 64),
     TraceConfig(3, 16, 16, 1, 64),
     TraceConfig(3, 16, 48, 1, 64),

@@ -122,7 +122,7 @@ struct Scene {
         size_t stageBytes = 0;
 
         // Traversal launch configuration.
-        GzTraceConfig traceConfig{3, 16, 32, 1, 64, 1 << 18, 0, 0};
+        GzTraceConfig traceConfig{3, 16, 32, 1, 128, 1 << 18, 0, 0};
         bool boundsSafe = false;  // every node box coordinate is finite with magnitude <= 2^40
         int cuCount = 256;
 };

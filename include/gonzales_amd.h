@@ -272,12 +272,15 @@ int gz_gpu_expand_hits_device(void *handle, const struct GzHit *d_hits, const fl
  *   ldsStack    : stack entries kept in LDS per lane (0, 8 or 16; the rest in private memory);
  *   refillLanes : idle lanes (1..64) that trigger a refill of a wave's lanes (modes 1, 3);
  *   gridScale   : persistent grid = gridScale x the blocks resident per CU x CUs (0 = 1);
- *   chunk       : rays a wave takes from a counter per atomic (modes 1, 3; 1..4096);
+ *   chunk       : rays a wave takes from a counter per atomic (modes 1, 3; 1..4096); also the
+ *                 queue entries a wave of the MIS full wavefront's path-ray kernel takes per
+ *                 atomic (default 128);
  *   binMinRays  : batches of at least this many rays (and at least 1) are traced in the order of
- *                 their origin and direction cells (a counting sort on the device, into storage
- *                 the handle keeps per stream, 8 bytes per ray + 1 MiB), so that the waves in
- *                 flight read overlapping parts of the tree; each ray's result still goes to its
- *                 own index. 0 = always in the given order;
+ *                 their origin's cell in a 16^3 grid over the scene box (a counting sort on the
+ *                 device, into storage the handle keeps per stream: 34 bytes per ray plus 16 KiB
+ *                 per block of up to 65536 rays), so that the waves in flight read overlapping
+ *                 parts of the tree; each ray's result still goes to its own index. 0 = always in
+ *                 the given order;
  *   recordTimes : 1 = record HIP events around the binning and the traversal kernel of each
  *                 launch, for gz_gpu_trace_times (0 = none). */
 struct GzTraceConfig {
