@@ -92,6 +92,19 @@ __device__ __forceinline__ uint64_t zs_sample_index(const ZSobolParams &p, uint6
         const bool odd = (p.log2spp & 1) == 1;
         const int last = odd ? 1 : 0;
         const uint64_t dimMix = 0x55555555ull * (uint64_t)(int64_t)dim;
+        if (morton == 0) {
+                // Morton index 0 (the light chooser's sampler never starts a pixel sample): every
+                // digit and every higher part is 0, so every level hashes the same value and takes
+                // the same permutation of digit 0; the loop below gives the same bits.
+                const uint32_t h = zs_mix_bits(dimMix);
+                const uint64_t d = zs_perm((int)((h >> 24) % 24u), 0);
+                for (int i = p.nBase4Digits - 1; i >= last; --i) {
+                        const int shift = 2 * i - (odd ? 1 : 0);
+                        out |= (shift < 64 ? d << shift : 0ull);
+                }
+                if (odd) out |= (uint64_t)(h & 1u);
+                return out;
+        }
         for (int i = p.nBase4Digits - 1; i >= last; --i) {
                 const int shift = 2 * i - (odd ? 1 : 0);
                 const int digit = (int)(zs_shr(morton, shift) & 3);
