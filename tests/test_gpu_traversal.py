@@ -129,6 +129,24 @@ def test_soup_1m_coherent_subset_bit_exact(soup1m):
     _assert_hits_equal(hits, ref)
 
 
+@pytest.mark.skip(reason="a host-buffer batch in pieces of 262,144 rays through pinned staging on two "
+                         "streams went silent for 180 s in its first 600,000-ray call on the GPU and was "
+                         "killed; no cause was found, so that path is not built and this test stays off")
+def test_host_buffer_batch_in_pieces(soup1m):
+    """gz_gpu_traverse_batch over 600,000 host rays run in pieces of 262,144 through pinned staging
+    on two streams (two full pieces and a partial one): records byte-equal to two calls of 300,000
+    rays each."""
+    rays, tmax = scenes.incoherent_rays(1 << 24)
+    m, half = 600_000, 300_000
+    whole = soup1m.intersect_batch_gpu(rays[:m], tmax[:m])
+    parts = np.concatenate([soup1m.intersect_batch_gpu(rays[:half], tmax[:half]),
+                            soup1m.intersect_batch_gpu(rays[half:m], tmax[half:m])])
+    assert whole.tobytes() == parts.tobytes()
+    assert 0.2 < whole["hit"].mean() < 0.8
+    again = soup1m.intersect_batch_gpu(rays[:m], tmax[:m])  # staging reused
+    assert again.tobytes() == whole.tobytes()
+
+
 def test_soup_1m_full_batch_properties(soup1m):
     # 16M incoherent rays: every hit lies within (0, tMax], barycentrics are valid, misses carry
     # tMax, and any-hit agrees with "closest hit exists" ray for ray.
