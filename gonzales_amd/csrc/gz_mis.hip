@@ -1,6 +1,8 @@
 // Wavefront MIS path tracing on gfx950: the kernels and entry points of one scene's trace,
 // bounce and path-replay adjoint. The bounce itself is in mis_device.h.
 #include <algorithm>
+#include <cmath>
+#include <climits>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -494,10 +496,12 @@ __global__ __launch_bounds__(kBlock) void k_wf_rays(MisArgs a, const GzPath *__r
                         lane_init(L, rays[idx], kAny ? 1.0f - 0.00001f : INFINITY, a.rootDesc, a.sceneSafe);
                 }
                 if (trace && !a.empty) {
+                        // BSDF rays stop at a hit closer than the emitter box (bsdf_ray_stop)
+                        const float tStop = kMode == 2 ? bsdf_ray_stop(a, rays[idx]) : -1.0f;
                         int st;
                         do {
                                 st = step<kAny, kStats, kLdsMax>(L, S, a.nodes, a.tris);
-                        } while (st == kStepGo);
+                        } while (st == kStepGo && !(kMode == 2 && L.hitPrim >= 0 && L.tBest < tStop));
                         if (st == kStepOverflow) {
                                 atomicOr(a.err, 1);
                                 L.hitPrim = -1;
@@ -716,6 +720,7 @@ struct MisScene {
         bool hasSky = false;
         float sky[3] = {0, 0, 0};
         uint8_t *dPrimClass = nullptr;  // per primitive: prim_class_of (depends on the materials)
+        float emitLo[3] = {1.0f, 1.0f, 1.0f}, emitHi[3] = {0.0f, 0.0f, 0.0f};  // MisArgs::emitLo/Hi
         bool allDiffuse = true;  // picks the kernel built with the diffuse BSDF only
         bool anyLayered = false;  // some material is the layered (coated conductor) BSDF
         bool anyOther = false;    // some material is non-diffuse and not layered
@@ -783,6 +788,78 @@ __global__ __launch_bounds__(kBlock) void k_prim_class(const PrimId_C *__restric
         const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) return;
         out[i] = prim_class_of(prims[i], primTris + 3 * i, mats, nMats);
+}
+
+// Float keys whose integer order is the float order (for atomicMin / atomicMax).
+__device__ __forceinline__ int float_key(float f) {
+        const int b = __float_as_int(f);
+        return b >= 0 ? b : b ^ 0x7fffffff;
+}
+
+// Bounds of the vertices of the primitives of class kClsLight or kClsFull into box[0..5] (keys of
+// min x, y, z and max x, y, z); box[6] = 1 when such a primitive has no vertices (indices out of
+// range), whose hits then cannot be bounded.
+__global__ __launch_bounds__(kBlock) void k_emit_box(const uint8_t *__restrict__ cls, const float4 *__restrict__ primTris,
+                                                     int64_t n, int *__restrict__ box) {
+        for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+                if (!(cls[i] & (kClsLight | kClsFull))) continue;
+                const float4 q[3] = {primTris[3 * i], primTris[3 * i + 1], primTris[3 * i + 2]};
+                if (q[0].w != 0.0f) {
+                        atomicOr(box + 6, 1);
+                        continue;
+                }
+                for (int v = 0; v < 3; ++v) {
+                        const float c[3] = {q[v].x, q[v].y, q[v].z};
+                        for (int k = 0; k < 3; ++k) {
+                                atomicMin(box + k, float_key(c[k]));
+                                atomicMax(box + 3 + k, float_key(c[k]));
+                        }
+                }
+        }
+}
+
+static float key_float(int k) {
+        const int b = k >= 0 ? k : k ^ 0x7fffffff;
+        float f;
+        std::memcpy(&f, &b, sizeof f);
+        return f;
+}
+
+// MisScene::emitLo/Hi from its class array: the box, padded outward by 1e-4 of its largest
+// coordinate magnitude (far more than the rounding of a slab or triangle test), the whole space
+// when some such primitive cannot be bounded, empty when there is none.
+static bool emit_box(Scene *s, MisScene *m) {
+        int h[7] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN, 0};
+        int *d = nullptr;
+        if (alloc_fault("emitter box") || !hip_ok(hipMalloc((void **)&d, sizeof h), "hipMalloc(emitter box)")) return false;
+        const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)s->cuCount * 4,
+                                                                           (s->primCount + kBlock - 1) / kBlock));
+        bool ok = hip_ok(hipMemcpy(d, h, sizeof h, hipMemcpyHostToDevice), "emitter box");
+        if (ok) {
+                hipLaunchKernelGGL(k_emit_box, dim3(g), dim3(kBlock), 0, s->stream, m->dPrimClass, s->dPrimTris,
+                                   s->primCount, d);
+                ok = hip_ok(hipGetLastError(), "emitter box launch") &&
+                     hip_ok(hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, s->stream), "emitter box") &&
+                     hip_ok(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+        }
+        (void)hipFree(d);
+        if (!ok) return false;
+        if (h[6]) {
+                for (int k = 0; k < 3; ++k) {
+                        m->emitLo[k] = -INFINITY;
+                        m->emitHi[k] = INFINITY;
+                }
+                return true;
+        }
+        if (h[0] == INT_MAX) return true;  // none: the box stays empty
+        float big = 0.0f;
+        for (int k = 0; k < 6; ++k) big = std::max(big, std::fabs(key_float(h[k])));
+        const float pad = 1e-4f * std::max(big, 1.0f);
+        for (int k = 0; k < 3; ++k) {
+                m->emitLo[k] = key_float(h[k]) - pad;
+                m->emitHi[k] = key_float(h[3 + k]) + pad;
+        }
+        return true;
 }
 
 // Fills Scene::dPrimTris once per handle (synchronously, on the handle's stream).
@@ -871,6 +948,7 @@ static int set_mis_scene(Scene *s, const GzMaterial *materials, int32_t material
                         ok = hip_ok(hipGetLastError(), "primitive classes launch") &&
                              hip_ok(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
                 }
+                ok = ok && emit_box(s, m);
         }
         if (!ok) {
                 release_mis(m);
@@ -972,7 +1050,11 @@ bool gz::mis_args(Scene *s, const GzMisParams *params, int *err, MisArgs &a) {
         a.nChoices = ms->nChoices;
         a.totalPower = ms->totalPower;
         a.hasSky = ms->hasSky;
-        for (int k = 0; k < 3; ++k) a.sky[k] = ms->sky[k];
+        for (int k = 0; k < 3; ++k) {
+                a.sky[k] = ms->sky[k];
+                a.emitLo[k] = ms->emitLo[k];
+                a.emitHi[k] = ms->emitHi[k];
+        }
         a.sobol = table;
         a.zs = ZSobolParams{params->zsSeed, params->zsLog2Spp, params->zsBase4Digits};
         a.maxDepth = params->maxDepth;

@@ -53,6 +53,10 @@ struct MisArgs {
         int64_t primCount;
         const float4 *primTris;  // Scene::dPrimTris
         const uint8_t *primClass;  // per primitive: prim_class_of (set with the materials)
+        // Box around the primitives of class kClsLight or kClsFull, padded outward (lo > hi when
+        // there are none): a BSDF ray may stop at a hit closer than its entry into this box
+        // (bsdf_ray_stop).
+        float emitLo[3], emitHi[3];
         const DevMesh *meshes;
         int64_t meshCount;
         const GzMaterial *mats;
@@ -94,6 +98,57 @@ __device__ __forceinline__ bool trace_ray(const MisArgs &a, const Ray_C &ray, fl
         do {
                 st = step<kAny, kStats, kLdsMis>(L, w.S, a.nodes, a.tris);
         } while (st == kStepGo);
+        if (kStats) {
+                w.interior += L.nInterior;
+                w.tris += L.nTris;
+        }
+        if (st == kStepOverflow) {
+                atomicOr(a.err, 1);
+                return false;
+        }
+        prim = L.hitPrim;
+        t = L.tBest;
+        u = L.bu;
+        v = L.bv;
+        return prim >= 0;
+}
+
+// Where a BSDF-sampled ray may stop looking for its closest hit. The BSDF strategy needs only to
+// know whether that hit is an area light or a primitive whose surface interaction fails (both
+// inside the emitter box), or an ordinary surface, which adds nothing. Any hit closer than the
+// ray's entry into the padded box is closer than every such primitive, so the closest hit is an
+// ordinary surface once one such hit is found: the ray stops there (the hit it reports is then an
+// ordinary surface, not necessarily the closest). Returns the entry distance, +inf for a ray that
+// misses the box, 0 (never stop) where the slab test is not clear-cut.
+__device__ __forceinline__ float bsdf_ray_stop(const MisArgs &a, const Ray_C &r) {
+        if (!(a.emitLo[0] <= a.emitHi[0])) return INFINITY;  // no such primitives
+        const float o[3] = {r.orgX, r.orgY, r.orgZ}, d[3] = {r.dirX, r.dirY, r.dirZ};
+        float t0 = 0.0f, t1 = INFINITY;
+        for (int k = 0; k < 3; ++k) {
+                if (d[k] == 0.0f) {
+                        if (o[k] < a.emitLo[k] || o[k] > a.emitHi[k]) return INFINITY;
+                        continue;
+                }
+                const float ta = (a.emitLo[k] - o[k]) / d[k], tb = (a.emitHi[k] - o[k]) / d[k];
+                const float tn = ta < tb ? ta : tb, tf = ta < tb ? tb : ta;
+                if (!(tn == tn) || !(tf == tf)) return 0.0f;
+                t0 = tn > t0 ? tn : t0;
+                t1 = tf < t1 ? tf : t1;
+        }
+        return t0 <= t1 ? t0 : INFINITY;
+}
+
+// trace_ray's closest hit that stops once a hit closer than tStop is found (see bsdf_ray_stop).
+template <bool kStats>
+__device__ __forceinline__ bool trace_ray_until(const MisArgs &a, const Ray_C &ray, float tStop, int &prim, float &t,
+                                                float &u, float &v, Work &w) {
+        if (a.empty) return false;
+        Lane L;
+        lane_init(L, ray, INFINITY, a.rootDesc, a.sceneSafe);
+        int st;
+        do {
+                st = step<false, kStats, kLdsMis>(L, w.S, a.nodes, a.tris);
+        } while (st == kStepGo && !(L.hitPrim >= 0 && L.tBest < tStop));
         if (kStats) {
                 w.interior += L.nInterior;
                 w.tris += L.nTris;
@@ -623,7 +678,7 @@ __device__ __forceinline__ bool bounce_rest(const MisArgs &a, GzPath *__restrict
                                 bu = rh.u;
                                 bv = rh.v;
                         } else {
-                                bhit = trace_ray<false, kStats>(a, r, INFINITY, bprim, bt, bu, bv, w);
+                                bhit = trace_ray_until<kStats>(a, r, bsdf_ray_stop(a, r), bprim, bt, bu, bv, w);
                         }
                         if (bhit && !bsdf_hit_needs_interaction(a, bprim)) {
                                 // a surface that emits nothing: no term
