@@ -143,57 +143,32 @@ __device__ __forceinline__ GzPath camera_path(const GzCamera &c, const uint32_t 
         return p;
 }
 
-// Adds the spp consecutive samples of one pixel, in sample order, to the film sums. The sums of
-// the pixel being added to stay in registers while consecutive samples land on it (the camera
-// writes a pixel's samples next to each other) and go back to memory when the pixel changes: the
-// same float additions in the same order as adding each sample to memory.
+// Adds the spp consecutive samples of one pixel, in sample order, to the film sums.
 __device__ __forceinline__ void film_add_pixel(int resX, int resY, int spp, const GzPath *__restrict__ ps, float iso,
                                                float maxComponent, float *__restrict__ light,
                                                float *__restrict__ weight, float *__restrict__ albedo,
                                                float *__restrict__ normal) {
-        int64_t cur = -1;
-        float L[3] = {0.0f, 0.0f, 0.0f}, W = 0.0f, A[3] = {0.0f, 0.0f, 0.0f}, N[3] = {0.0f, 0.0f, 0.0f};
-        auto flush = [&]() {
-                if (cur < 0) return;
-                for (int ch = 0; ch < 3; ++ch) {
-                        light[3 * cur + ch] = L[ch];
-                        albedo[3 * cur + ch] = A[ch];
-                        normal[3 * cur + ch] = N[ch];
-                }
-                weight[cur] = W;
-        };
         for (int s = 0; s < spp; ++s) {
                 const GzPath &p = ps[s];
                 const int x = p.pixelX, y = p.pixelY;
                 if (!(x >= 0 && y >= 0 && x < resX && y < resY)) continue;
                 const int64_t idx = (int64_t)y * resX + x;
-                if (idx != cur) {
-                        flush();
-                        cur = idx;
-                        for (int ch = 0; ch < 3; ++ch) {
-                                L[ch] = light[3 * idx + ch];
-                                A[ch] = albedo[3 * idx + ch];
-                                N[ch] = normal[3 * idx + ch];
-                        }
-                        W = weight[idx];
-                }
                 const float k100 = iso / 100.0f;
                 f3 sc = mk(p.estimate[0] * k100, p.estimate[1] * k100, p.estimate[2] * k100);
                 if (maxComponent > 0.0f) {
                         const float mv = cmax(sc);
                         if (mv > maxComponent) sc = mul(sc, maxComponent / mv);
                 }
-                L[0] = L[0] + sc.x;
-                L[1] = L[1] + sc.y;
-                L[2] = L[2] + sc.z;
-                W = W + p.filterWeight;
-                for (int ch = 0; ch < 3; ++ch) A[ch] = A[ch] + p.albedo[ch];
+                light[3 * idx] = light[3 * idx] + sc.x;
+                light[3 * idx + 1] = light[3 * idx + 1] + sc.y;
+                light[3 * idx + 2] = light[3 * idx + 2] + sc.z;
+                weight[idx] = weight[idx] + p.filterWeight;
+                for (int ch = 0; ch < 3; ++ch) albedo[3 * idx + ch] = albedo[3 * idx + ch] + p.albedo[ch];
                 const f3 n = normalized(mk(p.firstNormal[0], p.firstNormal[1], p.firstNormal[2]));
-                N[0] = N[0] + fabsf(n.x);
-                N[1] = N[1] + fabsf(n.y);
-                N[2] = N[2] + fabsf(n.z);
+                normal[3 * idx] = normal[3 * idx] + fabsf(n.x);
+                normal[3 * idx + 1] = normal[3 * idx + 1] + fabsf(n.y);
+                normal[3 * idx + 2] = normal[3 * idx + 2] + fabsf(n.z);
         }
-        flush();
 }
 
 // Host side (gz_film.hip): camera validation (false with the error set) and the crop's path count.
