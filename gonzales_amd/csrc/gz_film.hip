@@ -26,9 +26,9 @@ __global__ __launch_bounds__(kFilmBlock) void k_camera_paths(GzCamera c, const u
         out[k] = camera_path(c, table, first + k);
 }
 
-// k_film_add with the records staged through LDS. One lane per pixel reading its own records
-// (7 KB from its neighbours' at 64 spp) fetched every line of the records 2.5 times on the
-// 5M-triangle frame. Here a 64-lane block takes 64 pixels; per round of kFilmStage samples its
+// Film accumulation with the records staged through LDS. One lane per pixel reading its own
+// records straight from memory (7 KB from its neighbours' at 64 spp) fetched every line of the
+// records 2.5 times on the 5M-triangle frame. Here a 64-lane block takes 64 pixels; per round of kFilmStage samples its
 // lanes load those samples of all 64 pixels with loads that run over each pixel's consecutive
 // records, then each lane adds its own pixel's staged samples in sample order, holding the
 // pixel's sums in registers while consecutive samples land on that pixel (the same additions, in
