@@ -136,7 +136,7 @@ class GzBatchScene(ctypes.Structure):
 class GzBatchInfo(ctypes.Structure):
     _fields_ = [("totalPaths", ctypes.c_int64), ("rounds", ctypes.c_int32), ("items", ctypes.c_int32),
                 ("roundPaths", ctypes.c_int32), ("maxItemsPerRound", ctypes.c_int32), ("levels", ctypes.c_int32),
-                ("allBsdfs", ctypes.c_int32), ("soloScenes", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("allBsdfs", ctypes.c_int32), ("soloScenes", ctypes.c_int32), ("sideScenes", ctypes.c_int32)]
 
 
 GZ_FILTER_GAUSSIAN = 0

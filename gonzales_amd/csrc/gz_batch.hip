@@ -20,7 +20,10 @@
 // A scene with more than `soloPaths` paths fills the chip by itself: it stays out of the rounds
 // and renders alone, with the launch sequence of the single-scene path (camera kernel,
 // gz_gpu_mis_trace_device's kernels under the scene's own launch shape, film), in pieces of whole
-// pixels, all such scenes one after another with one queue storage. The rounds' kernels pay a
+// pixels, in two lanes: lane 0 on the caller's stream after the rounds, lane 1 on a side stream
+// from the start of the render, each with its own path slots and queue storage, so that the level
+// tails of one scene (the chip draining a level's last waves) overlap the other lane's work. A
+// scene's pieces stay in one lane, and scenes that share a film buffer share a lane. The rounds' kernels pay a
 // per-chunk item lookup and MisArgs read from memory that the single-scene kernels (MisArgs as a
 // kernel argument) do not; that only pays for scenes too small to fill the chip alone.
 #include <algorithm>
@@ -604,15 +607,20 @@ struct Batch {
         int *dErr = nullptr;
         std::vector<uint64_t> misGeneration;
 
-        // Scenes that render alone (more than soloPaths paths), in scene order; their pieces use
-        // dPaths (pathSlots >= soloSlots) and one queue storage shared by all of them.
+        // Scenes that render alone (more than soloPaths paths), in scene order, and their lanes:
+        // lane 0 uses dPaths (pathSlots >= soloSlots) on the render's stream, lane 1 dPathsSide on
+        // the side stream; each lane has one queue storage shared by its scenes.
         std::vector<int> solo;
+        std::vector<int> soloLane;
+        GzPath *dPathsSide = nullptr;
+        hipStream_t side = nullptr;
+        hipEvent_t fork = nullptr, join = nullptr;
         std::vector<GzMisParams> params;
         std::vector<GzCamera> hCams;
         std::vector<BatchFilm> hFilms;
         int64_t soloSlots = 0;
         int64_t pathSlots = 0;
-        MisQueues *soloQueues = nullptr;
+        MisQueues *soloQueues[2] = {nullptr, nullptr};
 };
 
 static void release_graph(Batch *b) {
@@ -628,14 +636,18 @@ static void free_batch(Batch *b) {
         if (hipGetDevice(&cur) == hipSuccess && cur != b->device) (void)hipSetDevice(b->device);
         release_graph(b);
         if (b->captureStream) (void)hipStreamDestroy(b->captureStream);
+        if (b->side) (void)hipStreamDestroy(b->side);
+        if (b->fork) (void)hipEventDestroy(b->fork);
+        if (b->join) (void)hipEventDestroy(b->join);
         void *ptrs[] = {b->dErr,      b->dCursor,   b->dRounds,  b->dItems,   b->dArgs,     b->dCams,     b->dFilms,
                         b->dPaths,    b->dQ[0],    b->dQ[1],    b->ctr.n,     b->ctr.head,  b->ctr.base,
                         b->wctr.n,    b->wctr.base, b->ws.hits,  b->ws.pend,   b->ws.srays,
                         b->ws.mrays,  b->ws.occ,   b->ws.mhits, b->ws.q[0],  b->ws.q[1],   b->ws.q[2],
-                        b->ws.q[3]};
+                        b->ws.q[3],  b->dPathsSide};
         for (void *p : ptrs)
                 if (p) (void)hipFree(p);
-        release_mis_queues(b->soloQueues);
+        release_mis_queues(b->soloQueues[0]);
+        release_mis_queues(b->soloQueues[1]);
         delete b;
 }
 
@@ -967,6 +979,36 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
                 }
                 b->launch = GzMisLaunch{3, 256, 1, 64, 3, {0, 0, 0}};
         }
+        // Lanes of the scenes that render alone: each scene to the lane with fewer paths so far
+        // (lane 0 starts with the rounds' paths, which it runs first), except that a scene sharing
+        // a film buffer with an earlier solo scene joins that scene's lane, so that no two lanes
+        // add to one film.
+        int64_t load[2] = {b->totalPaths, 0};
+        for (const int k : b->solo) load[0] -= crop_paths(&b->hCams[(size_t)k]);
+        for (size_t j = 0; j < b->solo.size(); ++j) {
+                const BatchFilm &f = b->hFilms[(size_t)b->solo[j]];
+                int lane = load[1] < load[0] ? 1 : 0;
+                for (size_t i = 0; i < j; ++i) {
+                        const BatchFilm &g = b->hFilms[(size_t)b->solo[i]];
+                        const float *fp[4] = {f.light, f.weight, f.albedo, f.normal};
+                        const float *gp[4] = {g.light, g.weight, g.albedo, g.normal};
+                        bool shared = false;
+                        for (const float *x : fp)
+                                for (const float *y : gp) shared = shared || x == y;
+                        if (shared) lane = b->soloLane[i];
+                }
+                b->soloLane.push_back(lane);
+                load[lane] += crop_paths(&b->hCams[(size_t)b->solo[j]]);
+        }
+        if (std::find(b->soloLane.begin(), b->soloLane.end(), 1) != b->soloLane.end()) {
+                if (!hip_ok(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "batch side stream") ||
+                    !hip_ok(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "batch fork event") ||
+                    !hip_ok(hipEventCreateWithFlags(&b->join, hipEventDisableTiming), "batch join event") ||
+                    !hip_ok(hipMalloc(&b->dPathsSide, (size_t)b->soloSlots * sizeof(GzPath)), "batch side paths")) {
+                        free_batch(b);
+                        return nullptr;
+                }
+        }
         return b;
 }
 
@@ -985,7 +1027,7 @@ extern "C" int gz_batch_info(void *batch, GzBatchInfo *info) {
         info->levels = b->levels;
         info->allBsdfs = b->all ? 1 : 0;
         info->soloScenes = (int32_t)b->solo.size();
-        info->reserved = 0;
+        info->sideScenes = (int32_t)std::count(b->soloLane.begin(), b->soloLane.end(), 1);
         return GZ_OK;
 }
 
@@ -1036,36 +1078,53 @@ extern "C" int gz_batch_render_device(void *batch, int32_t useGraph, void *strea
         if (!hip_ok(hipGetDevice(&cur), "get device") || (cur != b->device && !hip_ok(hipSetDevice(b->device), "set device")))
                 return GZ_E_HIP;
         const hipStream_t st = static_cast<hipStream_t>(stream);
+        // The scenes that render alone, in pieces of whole pixels: the single-scene launch sequence,
+        // lane 1's scenes on the side stream, which starts after the work queued on `stream` before
+        // this call and which `stream` waits for at the end (also after a failure).
+        if (b->side) {
+                if (!hip_ok(hipEventRecord(b->fork, st), "batch fork") ||
+                    !hip_ok(hipStreamWaitEvent(b->side, b->fork, 0), "batch fork"))
+                        return GZ_E_HIP;
+        }
+        int rc = GZ_OK;
         if (!b->hRounds.empty()) {
-                if (useGraph && !b->exec && !capture(b)) return GZ_E_HIP;
-                hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, b->dCursor);
-                if (!hip_ok(hipGetLastError(), "batch begin")) return GZ_E_HIP;
-                for (size_t r = 0; r < b->hRounds.size(); ++r) {
+                if (useGraph && !b->exec && !capture(b)) rc = GZ_E_HIP;
+                if (rc == GZ_OK) {
+                        hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, b->dCursor);
+                        if (!hip_ok(hipGetLastError(), "batch begin")) rc = GZ_E_HIP;
+                }
+                for (size_t r = 0; rc == GZ_OK && r < b->hRounds.size(); ++r) {
                         if (useGraph) {
-                                if (!hip_ok(hipGraphLaunch(b->exec, st), "graph launch")) return GZ_E_HIP;
+                                if (!hip_ok(hipGraphLaunch(b->exec, st), "graph launch")) rc = GZ_E_HIP;
                         } else if (!enqueue_round(b, st)) {
-                                return GZ_E_HIP;
+                                rc = GZ_E_HIP;
                         }
                 }
         }
-        // The scenes that render alone, in pieces of whole pixels: the single-scene launch sequence.
-        for (const int k : b->solo) {
+        for (size_t j = 0; rc == GZ_OK && j < b->solo.size(); ++j) {
+                const int k = b->solo[j], lane = b->soloLane[j];
+                const hipStream_t ls = lane ? b->side : st;
+                GzPath *paths = lane ? b->dPathsSide : b->dPaths;
                 const GzCamera &c = b->hCams[(size_t)k];
                 const BatchFilm &f = b->hFilms[(size_t)k];
                 const int64_t n = crop_paths(&c), piece = b->soloSlots / c.spp * c.spp;
-                for (int64_t first = 0; first < n; first += piece) {
+                for (int64_t first = 0; rc == GZ_OK && first < n; first += piece) {
                         const int64_t count = std::min(piece, n - first);
-                        int rc = gz_gpu_camera_paths_device(&c, first, count, b->dPaths, stream);
+                        rc = gz_gpu_camera_paths_device(&c, first, count, paths, ls);
                         if (rc == GZ_OK)
-                                rc = mis_trace_shared(b->scenes[(size_t)k], b->dPaths, count, &b->params[(size_t)k], st,
-                                                      &b->soloQueues, b->dErr);
+                                rc = mis_trace_shared(b->scenes[(size_t)k], paths, count, &b->params[(size_t)k], ls,
+                                                      &b->soloQueues[lane], b->dErr);
                         if (rc == GZ_OK)
-                                rc = gz_gpu_film_add_device(&c, b->dPaths, count, f.iso, f.maxComponent, f.light, f.weight,
-                                                            f.albedo, f.normal, stream);
-                        if (rc != GZ_OK) return rc;
+                                rc = gz_gpu_film_add_device(&c, paths, count, f.iso, f.maxComponent, f.light, f.weight,
+                                                            f.albedo, f.normal, ls);
                 }
         }
-        return GZ_OK;
+        if (b->side) {
+                if (!hip_ok(hipEventRecord(b->join, b->side), "batch join") ||
+                    !hip_ok(hipStreamWaitEvent(st, b->join, 0), "batch join"))
+                        return rc == GZ_OK ? GZ_E_HIP : rc;
+        }
+        return rc;
 }
 
 extern "C" int gz_batch_check(void *batch, void *stream) {

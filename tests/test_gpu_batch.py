@@ -111,6 +111,7 @@ def test_batch_scenes_that_render_alone(gpu):
         batch = SceneBatch(entries, round_paths=3000, solo_paths=solo_paths)
         info = batch.info()
         assert info.soloScenes == want_solo and 0 < want_solo
+        assert 0 < info.sideScenes < want_solo  # both lanes in use
         assert info.totalPaths == sum(counts)
         assert (info.rounds == 0) == (want_solo == len(sc))
         for mode in (False, True, True):
@@ -123,6 +124,28 @@ def test_batch_scenes_that_render_alone(gpu):
                 assert got.tobytes() == ref.tobytes(), (solo_paths, mode, k)
         batch.check()
         batch.close()
+    # the Cornell frame as two crops adding to ONE film, both alone: the second would go to the
+    # empty lane 1 by path count, but shares the first one's lane (no two lanes add to one film),
+    # and the film equals the whole frame's
+    integ, cam = sc[0]
+    top = cornell_camera(res=40, spp=4, seed=3, crop=(0, 40, 0, 17))
+    bottom = cornell_camera(res=40, spp=4, seed=3, crop=(0, 40, 17, 40))
+    shared = DeviceFilm(cam)
+    others = [BatchEntry(i, c, DeviceFilm(c)) for i, c in sc[2:]]
+    entries = [BatchEntry(integ, top, shared), BatchEntry(integ, bottom, shared)] + others
+    batch = SceneBatch(entries, round_paths=3000, solo_paths=1)
+    info = batch.info()
+    assert info.soloScenes == len(entries) and 0 < info.sideScenes < len(entries)
+    for mode in (False, True):
+        batch.clear_films()
+        batch.render(use_graph=mode)
+        torch.cuda.synchronize()
+        got = _films([entries[0]] + others)
+        want = [refs[0]] + refs[2:]
+        for k, (g, r) in enumerate(zip(got, want)):
+            assert g.tobytes() == r.tobytes(), ("crops", mode, k)
+    batch.check()
+    batch.close()
     with pytest.raises(RuntimeError, match="soloPaths"):
         SceneBatch([BatchEntry(integ, cam, DeviceFilm(cam)) for integ, cam in sc], solo_paths=-1)
 
