@@ -28,9 +28,9 @@ class SceneBatch:
     """Scenes rendered together (gz_batch_*): the scenes with at most `solo_paths` paths are
     packed into rounds of at most `round_paths` path slots, each round one launch sequence (a
     replayed HIP graph); a scene with more paths fills the GPU alone and renders with the
-    single-scene launch sequence, in two lanes (after the rounds on the render's stream, and
-    beside them on the batch's side stream), so that one scene's level tails overlap the other
-    lane's work. solo_paths = 0 puts every scene in the rounds."""
+    single-scene launch sequence, in four lanes (after the rounds on the render's stream, and
+    beside them on the batch's three side streams), so that one scene's level tails overlap the
+    other lanes' work. solo_paths = 0 puts every scene in the rounds."""
 
     def __init__(self, entries: list[BatchEntry], round_paths: int = 1 << 25, solo_paths: int = abi.GZ_BATCH_SOLO_PATHS):
         if not entries:

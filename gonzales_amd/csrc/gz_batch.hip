@@ -20,9 +20,9 @@
 // A scene with more than `soloPaths` paths fills the chip by itself: it stays out of the rounds
 // and renders alone, with the launch sequence of the single-scene path (camera kernel,
 // gz_gpu_mis_trace_device's kernels under the scene's own launch shape, film), in pieces of whole
-// pixels, in two lanes: lane 0 on the caller's stream after the rounds, lane 1 on a side stream
+// pixels, in four lanes: lane 0 on the caller's stream after the rounds, lanes 1-3 on side streams
 // from the start of the render, each with its own path slots and queue storage, so that the level
-// tails of one scene (the chip draining a level's last waves) overlap the other lane's work. A
+// tails of one scene (the chip draining a level's last waves) overlap the other lanes' work. A
 // scene's pieces stay in one lane, and scenes that share a film buffer share a lane. The rounds' kernels pay a
 // per-chunk item lookup and MisArgs read from memory that the single-scene kernels (MisArgs as a
 // kernel argument) do not; that only pays for scenes too small to fill the chip alone.
@@ -566,6 +566,9 @@ __global__ __launch_bounds__(kMisBlock) void k_bw_finish(const int *__restrict__
         }
 }
 
+// Lanes of the scenes that render alone: the render's stream and kSoloLanes - 1 side streams.
+constexpr int kSoloLanes = 4;  // 64-scene batch: 1598 ms with 2, 1612 with 3, 1550 with 4
+
 struct Batch {
         int device = 0;
         int nScenes = 0;
@@ -608,19 +611,21 @@ struct Batch {
         std::vector<uint64_t> misGeneration;
 
         // Scenes that render alone (more than soloPaths paths), in scene order, and their lanes:
-        // lane 0 uses dPaths (pathSlots >= soloSlots) on the render's stream, lane 1 dPathsSide on
-        // the side stream; each lane has one queue storage shared by its scenes.
+        // lane 0 uses dPaths (pathSlots >= soloSlots) on the render's stream, lane l > 0 uses
+        // dPathsSide[l - 1] on side stream side[l - 1] (created for the lanes that have scenes);
+        // each lane has one queue storage shared by its scenes.
         std::vector<int> solo;
         std::vector<int> soloLane;
-        GzPath *dPathsSide = nullptr;
-        hipStream_t side = nullptr;
-        hipEvent_t fork = nullptr, join = nullptr;
+        GzPath *dPathsSide[kSoloLanes - 1] = {};
+        hipStream_t side[kSoloLanes - 1] = {};
+        hipEvent_t join[kSoloLanes - 1] = {};
+        hipEvent_t fork = nullptr;
         std::vector<GzMisParams> params;
         std::vector<GzCamera> hCams;
         std::vector<BatchFilm> hFilms;
         int64_t soloSlots = 0;
         int64_t pathSlots = 0;
-        MisQueues *soloQueues[2] = {nullptr, nullptr};
+        MisQueues *soloQueues[kSoloLanes] = {};
 };
 
 static void release_graph(Batch *b) {
@@ -636,18 +641,20 @@ static void free_batch(Batch *b) {
         if (hipGetDevice(&cur) == hipSuccess && cur != b->device) (void)hipSetDevice(b->device);
         release_graph(b);
         if (b->captureStream) (void)hipStreamDestroy(b->captureStream);
-        if (b->side) (void)hipStreamDestroy(b->side);
+        for (int l = 0; l < kSoloLanes - 1; ++l) {
+                if (b->side[l]) (void)hipStreamDestroy(b->side[l]);
+                if (b->join[l]) (void)hipEventDestroy(b->join[l]);
+                if (b->dPathsSide[l]) (void)hipFree(b->dPathsSide[l]);
+        }
         if (b->fork) (void)hipEventDestroy(b->fork);
-        if (b->join) (void)hipEventDestroy(b->join);
         void *ptrs[] = {b->dErr,      b->dCursor,   b->dRounds,  b->dItems,   b->dArgs,     b->dCams,     b->dFilms,
                         b->dPaths,    b->dQ[0],    b->dQ[1],    b->ctr.n,     b->ctr.head,  b->ctr.base,
                         b->wctr.n,    b->wctr.base, b->ws.hits,  b->ws.pend,   b->ws.srays,
                         b->ws.mrays,  b->ws.occ,   b->ws.mhits, b->ws.q[0],  b->ws.q[1],   b->ws.q[2],
-                        b->ws.q[3],  b->dPathsSide};
+                        b->ws.q[3]};
         for (void *p : ptrs)
                 if (p) (void)hipFree(p);
-        release_mis_queues(b->soloQueues[0]);
-        release_mis_queues(b->soloQueues[1]);
+        for (MisQueues *q : b->soloQueues) release_mis_queues(q);
         delete b;
 }
 
@@ -979,15 +986,17 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
                 }
                 b->launch = GzMisLaunch{3, 256, 1, 64, 3, {0, 0, 0}};
         }
-        // Lanes of the scenes that render alone: each scene to the lane with fewer paths so far
-        // (lane 0 starts with the rounds' paths, which it runs first), except that a scene sharing
-        // a film buffer with an earlier solo scene joins that scene's lane, so that no two lanes
-        // add to one film.
-        int64_t load[2] = {b->totalPaths, 0};
+        // Lanes of the scenes that render alone: each scene to the lane with the fewest paths so
+        // far (lane 0 starts with the rounds' paths, which it runs first; ties to the lower lane),
+        // except that a scene sharing a film buffer with an earlier solo scene joins that scene's
+        // lane, so that no two lanes add to one film.
+        int64_t load[kSoloLanes] = {b->totalPaths};
         for (const int k : b->solo) load[0] -= crop_paths(&b->hCams[(size_t)k]);
         for (size_t j = 0; j < b->solo.size(); ++j) {
                 const BatchFilm &f = b->hFilms[(size_t)b->solo[j]];
-                int lane = load[1] < load[0] ? 1 : 0;
+                int lane = 0;
+                for (int l = 1; l < kSoloLanes; ++l)
+                        if (load[l] < load[lane]) lane = l;
                 for (size_t i = 0; i < j; ++i) {
                         const BatchFilm &g = b->hFilms[(size_t)b->solo[i]];
                         const float *fp[4] = {f.light, f.weight, f.albedo, f.normal};
@@ -1000,11 +1009,12 @@ extern "C" void *gz_batch_create(const GzBatchScene *scenes, int32_t sceneCount,
                 b->soloLane.push_back(lane);
                 load[lane] += crop_paths(&b->hCams[(size_t)b->solo[j]]);
         }
-        if (std::find(b->soloLane.begin(), b->soloLane.end(), 1) != b->soloLane.end()) {
-                if (!hip_ok(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "batch side stream") ||
-                    !hip_ok(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "batch fork event") ||
-                    !hip_ok(hipEventCreateWithFlags(&b->join, hipEventDisableTiming), "batch join event") ||
-                    !hip_ok(hipMalloc(&b->dPathsSide, (size_t)b->soloSlots * sizeof(GzPath)), "batch side paths")) {
+        for (int l = 1; l < kSoloLanes; ++l) {
+                if (std::find(b->soloLane.begin(), b->soloLane.end(), l) == b->soloLane.end()) continue;
+                if ((!b->fork && !hip_ok(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "batch fork event")) ||
+                    !hip_ok(hipStreamCreateWithFlags(&b->side[l - 1], hipStreamNonBlocking), "batch side stream") ||
+                    !hip_ok(hipEventCreateWithFlags(&b->join[l - 1], hipEventDisableTiming), "batch join event") ||
+                    !hip_ok(hipMalloc(&b->dPathsSide[l - 1], (size_t)b->soloSlots * sizeof(GzPath)), "batch side paths")) {
                         free_batch(b);
                         return nullptr;
                 }
@@ -1027,7 +1037,7 @@ extern "C" int gz_batch_info(void *batch, GzBatchInfo *info) {
         info->levels = b->levels;
         info->allBsdfs = b->all ? 1 : 0;
         info->soloScenes = (int32_t)b->solo.size();
-        info->sideScenes = (int32_t)std::count(b->soloLane.begin(), b->soloLane.end(), 1);
+        info->sideScenes = (int32_t)(b->soloLane.size() - std::count(b->soloLane.begin(), b->soloLane.end(), 0));
         return GZ_OK;
 }
 
@@ -1081,10 +1091,10 @@ extern "C" int gz_batch_render_device(void *batch, int32_t useGraph, void *strea
         // The scenes that render alone, in pieces of whole pixels: the single-scene launch sequence,
         // lane 1's scenes on the side stream, which starts after the work queued on `stream` before
         // this call and which `stream` waits for at the end (also after a failure).
-        if (b->side) {
-                if (!hip_ok(hipEventRecord(b->fork, st), "batch fork") ||
-                    !hip_ok(hipStreamWaitEvent(b->side, b->fork, 0), "batch fork"))
-                        return GZ_E_HIP;
+        if (b->fork) {
+                if (!hip_ok(hipEventRecord(b->fork, st), "batch fork")) return GZ_E_HIP;
+                for (hipStream_t sd : b->side)
+                        if (sd && !hip_ok(hipStreamWaitEvent(sd, b->fork, 0), "batch fork")) return GZ_E_HIP;
         }
         int rc = GZ_OK;
         if (!b->hRounds.empty()) {
@@ -1103,8 +1113,8 @@ extern "C" int gz_batch_render_device(void *batch, int32_t useGraph, void *strea
         }
         for (size_t j = 0; rc == GZ_OK && j < b->solo.size(); ++j) {
                 const int k = b->solo[j], lane = b->soloLane[j];
-                const hipStream_t ls = lane ? b->side : st;
-                GzPath *paths = lane ? b->dPathsSide : b->dPaths;
+                const hipStream_t ls = lane ? b->side[lane - 1] : st;
+                GzPath *paths = lane ? b->dPathsSide[lane - 1] : b->dPaths;
                 const GzCamera &c = b->hCams[(size_t)k];
                 const BatchFilm &f = b->hFilms[(size_t)k];
                 const int64_t n = crop_paths(&c), piece = b->soloSlots / c.spp * c.spp;
@@ -1119,10 +1129,11 @@ extern "C" int gz_batch_render_device(void *batch, int32_t useGraph, void *strea
                                                             f.albedo, f.normal, ls);
                 }
         }
-        if (b->side) {
-                if (!hip_ok(hipEventRecord(b->join, b->side), "batch join") ||
-                    !hip_ok(hipStreamWaitEvent(st, b->join, 0), "batch join"))
-                        return rc == GZ_OK ? GZ_E_HIP : rc;
+        for (int l = 0; l < kSoloLanes - 1; ++l) {
+                if (!b->side[l]) continue;
+                if (!hip_ok(hipEventRecord(b->join[l], b->side[l]), "batch join") ||
+                    !hip_ok(hipStreamWaitEvent(st, b->join[l], 0), "batch join"))
+                        rc = rc == GZ_OK ? GZ_E_HIP : rc;
         }
         return rc;
 }

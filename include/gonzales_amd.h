@@ -584,7 +584,7 @@ struct GzBatchInfo {
         int32_t levels;          /* bounce levels of a round: 1 + the largest maxDepth */
         int32_t allBsdfs;        /* 1 if some scene of the rounds has a non-diffuse material */
         int32_t soloScenes;      /* scenes that render alone (more than soloPaths paths) */
-        int32_t sideScenes;      /* of those, the ones on the batch's side stream (lane 1) */
+        int32_t sideScenes;      /* of those, the ones on the batch's side streams (lanes 1-3) */
 };
 
 /* Plans the batch: the paths of all scenes (in scene order, whole pixels) are cut into rounds of
@@ -593,15 +593,15 @@ struct GzBatchInfo {
  * (soloPaths > 0; GZ_BATCH_SOLO_PATHS is the measured break-even on MI355X) fills the chip alone:
  * it stays out of the rounds and each render runs it with the single-scene launch sequence
  * (gz_gpu_camera_paths_device, gz_gpu_mis_trace_device under the scene's own GzMisLaunch,
- * gz_gpu_film_add_device) in pieces of at most roundPaths paths, in two lanes: lane 0 on the
- * render's stream after the rounds, lane 1 on a side stream of the batch's own that starts with
- * the render (it waits for the work queued on the render's stream before the call, and that
- * stream waits for it at the end), so that one scene's bounce-level tails overlap the other
- * lane's work. Each scene goes to the lane with fewer paths so far (lane 0 counting the rounds'
- * paths); a scene whose film shares a buffer with an earlier solo scene's goes to that scene's
- * lane. Lane 1 has path slots of its own (allocated here, 112 B per slot) and each lane one queue
- * storage for its scenes (allocated by the first render); soloPaths = 0 puts every scene in the
- * rounds.
+ * gz_gpu_film_add_device) in pieces of at most roundPaths paths, in four lanes: lane 0 on the
+ * render's stream after the rounds, lanes 1-3 on side streams of the batch's own that start with
+ * the render (they wait for the work queued on the render's stream before the call, and that
+ * stream waits for them at the end), so that one scene's bounce-level tails overlap the other
+ * lanes' work. Each scene goes to the lane with the fewest paths so far (lane 0 counting the
+ * rounds' paths); a scene whose film shares a buffer with an earlier solo scene's goes to that
+ * scene's lane. Each side lane in use has path slots of its own (allocated here, 112 B per slot)
+ * and each lane one queue storage for its scenes (allocated by the first render, about 170 B per
+ * slot); soloPaths = 0 puts every scene in the rounds.
  * The batch snapshots each scene's MIS tables and parameters: a gz_gpu_set_mis_scene on one of
  * its scenes afterwards makes gz_batch_render_device refuse to run (GZ_E_INVALID; create the
  * batch again). Free the batch before its scenes. All scenes must be on one device. NULL on
@@ -620,7 +620,7 @@ int gz_batch_set_launch(void *batch, const struct GzMisLaunch *launch);
  * level, film) driven by device tables and a device round cursor. useGraph = 1 captures that
  * sequence once as a HIP graph (on the first call after creation or gz_batch_set_launch) and
  * replays it once per round; 0 launches it directly. Both give identical films. The scenes that
- * render alone follow the rounds (lane 0) or run beside them on the side stream (lane 1),
+ * render alone follow the rounds (lane 0) or run beside them on the side streams (lanes 1-3),
  * launched directly. Calls on one batch must not overlap (they share its path slots and queues). */
 int gz_batch_render_device(void *batch, int32_t useGraph, void *stream);
 /* gz_gpu_check_traversal for a batch: waits for `stream`, returns GZ_E_OVERFLOW if a traversal

@@ -111,7 +111,7 @@ def test_batch_scenes_that_render_alone(gpu):
         batch = SceneBatch(entries, round_paths=3000, solo_paths=solo_paths)
         info = batch.info()
         assert info.soloScenes == want_solo and 0 < want_solo
-        assert 0 < info.sideScenes < want_solo  # both lanes in use
+        assert 0 < info.sideScenes <= want_solo  # side lanes in use
         assert info.totalPaths == sum(counts)
         assert (info.rounds == 0) == (want_solo == len(sc))
         for mode in (False, True, True):
@@ -124,9 +124,9 @@ def test_batch_scenes_that_render_alone(gpu):
                 assert got.tobytes() == ref.tobytes(), (solo_paths, mode, k)
         batch.check()
         batch.close()
-    # the Cornell frame as two crops adding to ONE film, both alone: the second would go to the
-    # empty lane 1 by path count, but shares the first one's lane (no two lanes add to one film),
-    # and the film equals the whole frame's
+    # the Cornell frame as two crops adding to ONE film, both alone: the second would go to an
+    # empty side lane by path count, but shares the first one's lane (no two lanes add to one
+    # film), and the film equals the whole frame's
     integ, cam = sc[0]
     top = cornell_camera(res=40, spp=4, seed=3, crop=(0, 40, 0, 17))
     bottom = cornell_camera(res=40, spp=4, seed=3, crop=(0, 40, 17, 40))
