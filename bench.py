@@ -375,6 +375,7 @@ def full_path_leg(args, dev, stream, host: dict, run_cpu: bool) -> dict:
     cam.paths_device(rnd.d_paths, n // 2 - tune_n // 2, tune_n)  # centre of the frame
     best, timings = autotune_mis(integ, rnd.d_paths[:tune_n * abi.GZ_PATH.itemsize].clone(), tune_n, reps=1)
     log(f"full path leg launch {best}")
+    log(f"full path leg lanes {rnd.lanes}, bounds {rnd.bounds}")
     torch.cuda.synchronize()
     film.clear()
     a = torch.cuda.Event(enable_timing=True)
@@ -384,7 +385,7 @@ def full_path_leg(args, dev, stream, host: dict, run_cpu: bool) -> dict:
         rnd.render(stream)
     b.record(stream)
     torch.cuda.synchronize()
-    bh.check_traversal()
+    rnd.check(stream)
     ms = a.elapsed_time(b) / args.full_frames
     img = film.resolve().cpu().numpy().reshape(h, w, 3)
     achieved = per_path * n / (ms * 1e-3) / 1e9
@@ -399,6 +400,7 @@ def full_path_leg(args, dev, stream, host: dict, run_cpu: bool) -> dict:
                         "frac": round(achieved / HBM_PEAK_GBS, 4)},
            "image_mean_rgb": [round(float(x), 5) for x in img.reshape(-1, 3).mean(0)],
            "film_zero_weight_pixels": int(film.bad.item()),
+           "lanes": rnd.lanes, "lane_bounds": rnd.bounds,
            "launch": describe(timings)[0], "launch_ms": describe(timings)}
     log(f"full path leg: {res['path_samples_per_s'] / 1e6:.2f} M path samples/s, {ms:.1f} ms/frame, "
         f"{res['bounces_per_path']} bounces/path")

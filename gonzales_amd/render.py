@@ -2,8 +2,9 @@
 (Sources/libgonzales/Core/TileRenderer.swift, Core/Tile.swift:19-219) for the MIS integrator.
 
 A frame is cut into batches of whole pixels (pixel-major path numbering, see GzCamera); each
-batch is: camera paths -> all MIS bounces -> film sums, every step a kernel launch on one stream,
-with the path records and the film resident in HBM. Nothing returns to the host until resolve.
+batch is: camera paths -> all MIS bounces -> film sums, every step a kernel launch on its lane's
+stream, with the path records and the film resident in HBM. Nothing returns to the host until
+resolve.
 """
 from __future__ import annotations
 
@@ -16,33 +17,78 @@ from .integrator import VolumePathIntegrator
 
 @dataclass
 class FrameRenderer:
-    """One frame through the MIS integrator in launch sequences of `batch_paths` paths (whole
-    pixels). Each sequence holds about 285 B of device state per path (112 B here, the rest in the
-    scene handle's queue storage); larger sequences are faster, since every sequence pays each
-    bounce level's tail: the 5M-triangle 1080p x 64 spp frame takes 377 ms in 8M-path sequences,
-    306 ms in 32M-path ones (the default, 9 GB) and 287 ms as one sequence of 132.7M paths (38 GB)."""
+    """One frame through the MIS integrator in launch sequences of whole pixels. Each sequence
+    holds about 285 B of device state per path (112 B here, the rest in the scene handle's queue
+    storage, one per stream); larger sequences are faster, since every sequence pays each bounce
+    level's tail (the chip draining the level's last waves): the 5M-triangle 1080p x 64 spp frame
+    takes 377 ms in 8M-path sequences, 306 ms in 32M-path ones and 287 ms as one sequence of
+    132.7M paths (38 GB), one lane each.
+
+    The frame's pixels are cut into `lanes` contiguous ranges, each rendered on its own stream
+    (lane 0 on the caller's stream, the others on streams of the renderer's own that start after
+    the work queued on the caller's stream and that the caller's stream waits for at the end), in
+    sequences of at most batch_paths / lanes paths with a path buffer per lane: one range's level
+    tails overlap the other's work (the 5M-triangle frame: 241 ms as one 132.7M-path sequence in
+    one lane, 230 ms as two 66M-path lanes). lanes = 0 (the default) takes two lanes for frames of
+    at least 16M paths and one below: small frames do not fill the chip with one lane's kernels,
+    and their persistent kernels only queue behind each other (21 scenes of at most 1M paths: 125
+    ms in one lane, 161 ms in two). The lanes add to disjoint pixels of the film, and every path's
+    result is independent of its sequence, so the film is bit-identical for any lane count and
+    batch size. The lanes take equal pixel counts: cutting them so that they end together (by
+    measured lane times) made the 5M-triangle frame slower (235 against 228-230 ms), as the two
+    lanes' level tails then fall together instead of overlapping the other lane's work."""
     camera: PerspectiveCamera
     integrator: VolumePathIntegrator
     film: DeviceFilm
     batch_paths: int = 1 << 25
+    lanes: int = 0
 
     def __post_init__(self):
         import torch
         spp = self.camera.spp
-        per = max(spp, (min(self.batch_paths, self.camera.path_count) // spp) * spp)
-        self.batch = per
-        self.d_paths = torch.empty(per * abi.GZ_PATH.itemsize, dtype=torch.uint8, device=self.film.light.device)
+        n = self.camera.path_count
+        pixels = n // spp
+        lanes = int(self.lanes) if self.lanes > 0 else (2 if n >= 1 << 24 else 1)
+        lanes = max(1, min(lanes, pixels))
+        self.lanes = lanes
+        self.side = [torch.cuda.Stream(device=self.film.light.device) for _ in range(lanes - 1)]
+        # lane l renders paths [bounds[l], bounds[l + 1]): whole pixels
+        self.bounds = [(pixels * l // lanes) * spp for l in range(lanes + 1)]
+        widest = max(b - a for a, b in zip(self.bounds, self.bounds[1:])) if n else spp
+        self.batch = max(spp, (min(self.batch_paths // lanes, widest) // spp) * spp)
+        self.lane_paths = [torch.empty(self.batch * abi.GZ_PATH.itemsize, dtype=torch.uint8,
+                                       device=self.film.light.device) for _ in range(lanes)]
+        self.d_paths = self.lane_paths[0]
 
     def render(self, stream=None) -> None:
         """Adds one frame (every path sample of the crop) to the film."""
-        n = self.camera.path_count
-        first = 0
-        while first < n:
-            count = min(self.batch, n - first)
-            self.camera.paths_device(self.d_paths, first, count, stream)
-            self.integrator.trace(self.d_paths, count, stream)
-            self.film.add(self.d_paths, count, stream)
-            first += count
+        import torch
+        main = stream if stream is not None else torch.cuda.current_stream()
+        if self.side:
+            fork = torch.cuda.Event()
+            fork.record(main)
+            for s in self.side:
+                s.wait_event(fork)
+        for lane in range(self.lanes):
+            s = main if lane == 0 else self.side[lane - 1]
+            buf = self.lane_paths[lane]
+            first, end = self.bounds[lane], self.bounds[lane + 1]
+            while first < end:
+                count = min(self.batch, end - first)
+                self.camera.paths_device(buf, first, count, s)
+                self.integrator.trace(buf, count, s)
+                self.film.add(buf, count, s)
+                first += count
+        for s in self.side:
+            join = torch.cuda.Event()
+            join.record(s)
+            main.wait_event(join)
+
+    def check(self, stream=None) -> None:
+        """gz_gpu_check_traversal for every lane's stream (each stream has its own flag)."""
+        self.integrator.bh.check_traversal(stream)
+        for s in self.side:
+            self.integrator.bh.check_traversal(s)
 
 
 class TileRenderer:

@@ -150,6 +150,31 @@ def test_batch_scenes_that_render_alone(gpu):
         SceneBatch([BatchEntry(integ, cam, DeviceFilm(cam)) for integ, cam in sc], solo_paths=-1)
 
 
+def test_frame_renderer_lanes_bit_identical(gpu):
+    """FrameRenderer's lanes (pixel ranges on their own streams) and sequence sizes change no bit
+    of the film: 1, 2 and 3 lanes, whole-frame and small sequences, one handle shared."""
+    import torch
+    sc = _scenes()
+    for k, (integ, cam) in enumerate(sc):
+        films = []
+        for lanes, batch in ((1, 1 << 20), (2, 1 << 20), (3, 1 << 11), (2, 192)):
+            f = DeviceFilm(cam)
+            rnd = FrameRenderer(cam, integ, f, batch_paths=batch, lanes=lanes)
+            assert rnd.lanes == lanes and len(rnd.side) == lanes - 1
+            b = rnd.bounds
+            assert b[0] == 0 and b[-1] == cam.path_count and all(y - x >= cam.spp for x, y in zip(b, b[1:]))
+            assert all(x % cam.spp == 0 for x in b)
+            rnd.render()
+            rnd.render()  # twice: the second frame adds onto the first
+            torch.cuda.synchronize()
+            rnd.check()
+            films.append(np.concatenate([t.cpu().numpy() for t in (f.light, f.weight, f.albedo, f.normal)]))
+        for other in films[1:]:
+            assert other.tobytes() == films[0].tobytes(), k
+    integ, cam = sc[0]
+    assert FrameRenderer(cam, integ, DeviceFilm(cam)).lanes == 1  # automatic: one lane below 16M paths
+
+
 def test_batch_rejects_bad_input(gpu):
     sc = _scenes()
     entries = [BatchEntry(integ, cam, DeviceFilm(cam)) for integ, cam in sc]
