@@ -126,7 +126,9 @@ def parse():
     ap.add_argument("--full-subdiv", type=int, default=9)
     ap.add_argument("--full-frames", type=int, default=1)
     ap.add_argument("--full-batch-paths", type=int, default=1 << 28,
-                    help="paths per launch sequence of the frame (default: the whole frame)")
+                    help="paths per launch sequence of the frame, over all lanes (default: the whole frame)")
+    ap.add_argument("--full-lanes", type=int, default=0,
+                    help="FrameRenderer lanes for the frame (0: automatic, two for the 5M-triangle frame)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch leg")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=1)
@@ -351,7 +353,7 @@ def full_path_leg(args, dev, stream, host: dict, run_cpu: bool) -> dict:
     sky = (0.4, 0.45, 0.5)
     integ = VolumePathIntegrator(bh, opt, sky=sky, materials=gz)
     film = DeviceFilm(cam, dev)
-    rnd = FrameRenderer(cam, integ, film, batch_paths=args.full_batch_paths)
+    rnd = FrameRenderer(cam, integ, film, batch_paths=args.full_batch_paths, lanes=args.full_lanes)
     n = cam.path_count
     # counting pass over a whole frame (untimed; it also warms the kernels up)
     m = rnd.batch
