@@ -20,16 +20,17 @@ class FrameRenderer:
     """One frame through the MIS integrator in launch sequences of whole pixels. Each sequence
     holds about 285 B of device state per path (112 B here, the rest in the scene handle's queue
     storage, one per stream); larger sequences are faster, since every sequence pays each bounce
-    level's tail (the chip draining the level's last waves): the 5M-triangle 1080p x 64 spp frame
-    takes 377 ms in 8M-path sequences, 306 ms in 32M-path ones and 287 ms as one sequence of
-    132.7M paths (38 GB), one lane each.
+    level's tail (the chip draining the level's last waves): in one lane the 5M-triangle 1080p x
+    64 spp frame takes 259 ms in 32M-path sequences and 241 ms as one sequence of 132.7M paths
+    (38 GB).
 
     The frame's pixels are cut into `lanes` contiguous ranges, each rendered on its own stream
     (lane 0 on the caller's stream, the others on streams of the renderer's own that start after
     the work queued on the caller's stream and that the caller's stream waits for at the end), in
     sequences of at most batch_paths / lanes paths with a path buffer per lane: one range's level
-    tails overlap the other's work (the 5M-triangle frame: 241 ms as one 132.7M-path sequence in
-    one lane, 230 ms as two 66M-path lanes). lanes = 0 (the default) takes two lanes for frames of
+    tails overlap the other's work. The 5M-triangle frame with the same device state: 229-230 ms
+    as two 66M-path lanes against 241 ms as one sequence, 246 ms as two lanes of 16M-path
+    sequences against 259 ms as one lane of 32M-path sequences. lanes = 0 (the default) takes two lanes for frames of
     at least 16M paths and one below: small frames do not fill the chip with one lane's kernels,
     and their persistent kernels only queue behind each other (21 scenes of at most 1M paths: 125
     ms in one lane, 161 ms in two). The lanes add to disjoint pixels of the film, and every path's
