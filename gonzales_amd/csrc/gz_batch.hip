@@ -1088,9 +1088,10 @@ extern "C" int gz_batch_render_device(void *batch, int32_t useGraph, void *strea
         if (!hip_ok(hipGetDevice(&cur), "get device") || (cur != b->device && !hip_ok(hipSetDevice(b->device), "set device")))
                 return GZ_E_HIP;
         const hipStream_t st = static_cast<hipStream_t>(stream);
-        // The scenes that render alone, in pieces of whole pixels: the single-scene launch sequence,
-        // lane 1's scenes on the side stream, which starts after the work queued on `stream` before
-        // this call and which `stream` waits for at the end (also after a failure).
+        // The side streams (lanes 1-3) start after the work queued on `stream` before this call, and
+        // `stream` waits for them at the end, also after a failure. Then the rounds on `stream`,
+        // and the scenes that render alone, each with the single-scene launch sequence on its
+        // lane's stream, in pieces of whole pixels.
         if (b->fork) {
                 if (!hip_ok(hipEventRecord(b->fork, st), "batch fork")) return GZ_E_HIP;
                 for (hipStream_t sd : b->side)
