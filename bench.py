@@ -127,6 +127,8 @@ def parse():
     ap.add_argument("--full-frames", type=int, default=1)
     ap.add_argument("--full-batch-paths", type=int, default=1 << 28,
                     help="paths per launch sequence of the frame, over all lanes (default: the whole frame)")
+    ap.add_argument("--path-lanes", type=int, default=0,
+                    help="FrameRenderer lanes for the Cornell frame (0: automatic, two at 4.2M paths)")
     ap.add_argument("--full-lanes", type=int, default=0,
                     help="FrameRenderer lanes for the frame (0: automatic, two for the 5M-triangle frame)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch leg")
@@ -286,10 +288,12 @@ def path_leg(args, dev, stream, host: dict, run_cpu: bool) -> dict:
     best, timings = autotune_mis(integ, d_paths.clone(), n, reps=2)
     log(f"path leg launch {best}")
 
+    from gonzales_amd.render import FrameRenderer
+    rnd = FrameRenderer(cam, integ, film, batch_paths=n, lanes=args.path_lanes)
+    log(f"path leg lanes {rnd.lanes}, bounds {rnd.bounds}")
+
     def step():
-        cam.paths_device(d_paths)
-        integ.trace(d_paths)
-        film.add(d_paths, n)
+        rnd.render(stream)
 
     film.clear()
     for _ in range(2):
@@ -304,7 +308,7 @@ def path_leg(args, dev, stream, host: dict, run_cpu: bool) -> dict:
         step()
     b.record(stream)
     torch.cuda.synchronize()
-    bh.check_traversal()
+    rnd.check(stream)
     ms = a.elapsed_time(b) / steps
     img = film.resolve().cpu().numpy().reshape(res, res, 3)
     res_d = {"workload": f"Cornell box (34 triangles, 2 area-light triangles), {res}x{res}, {spp} spp, max depth 5, "
@@ -316,7 +320,7 @@ def path_leg(args, dev, stream, host: dict, run_cpu: bool) -> dict:
              "roofline": {"bound": "hbm", "achieved": round(algo / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
              "image_mean_rgb": [round(float(x), 5) for x in img.reshape(-1, 3).mean(0)],
-             "film_zero_weight_pixels": int(film.bad.item()),
+             "film_zero_weight_pixels": int(film.bad.item()), "lanes": rnd.lanes,
              "launch": describe(timings)[0], "launch_ms": describe(timings)}
     log(f"path leg: {res_d['path_samples_per_s'] / 1e6:.2f} M path samples/s, {ms:.2f} ms/frame, "
         f"{res_d['bounces_per_path']} bounces/path, {res_d['bytes_per_path']} B/path")

@@ -30,10 +30,11 @@ class FrameRenderer:
     sequences of at most batch_paths / lanes paths with a path buffer per lane: one range's level
     tails overlap the other's work. The 5M-triangle frame with the same device state: 229-230 ms
     as two 66M-path lanes against 241 ms as one sequence, 246 ms as two lanes of 16M-path
-    sequences against 259 ms as one lane of 32M-path sequences. lanes = 0 (the default) takes two lanes for frames of
-    at least 16M paths and one below: small frames do not fill the chip with one lane's kernels,
-    and their persistent kernels only queue behind each other (21 scenes of at most 1M paths: 125
-    ms in one lane, 161 ms in two). The lanes add to disjoint pixels of the film, and every path's
+    sequences against 259 ms as one lane of 32M-path sequences; the Cornell box at 512^2 x 16 spp
+    (4.2M paths, the queued megakernel) 3.64 against 3.84 ms. lanes = 0 (the default) takes two
+    lanes for frames of at least 4M paths and one below: smaller frames do not fill the chip with
+    one lane's kernels, and their persistent kernels only queue behind each other (the Cornell box
+    at 256^2 x 16 spp: 2.41 ms in two lanes against 1.52 in one). The lanes add to disjoint pixels of the film, and every path's
     result is independent of its sequence, so the film is bit-identical for any lane count and
     batch size. The lanes take equal pixel counts: cutting them so that they end together (by
     measured lane times) made the 5M-triangle frame slower (235 against 228-230 ms), as the two
@@ -49,7 +50,7 @@ class FrameRenderer:
         spp = self.camera.spp
         n = self.camera.path_count
         pixels = n // spp
-        lanes = int(self.lanes) if self.lanes > 0 else (2 if n >= 1 << 24 else 1)
+        lanes = int(self.lanes) if self.lanes > 0 else (2 if n >= 1 << 22 else 1)
         lanes = max(1, min(lanes, pixels))
         self.lanes = lanes
         self.side = [torch.cuda.Stream(device=self.film.light.device) for _ in range(lanes - 1)]

@@ -172,7 +172,7 @@ def test_frame_renderer_lanes_bit_identical(gpu):
         for other in films[1:]:
             assert other.tobytes() == films[0].tobytes(), k
     integ, cam = sc[0]
-    assert FrameRenderer(cam, integ, DeviceFilm(cam)).lanes == 1  # automatic: one lane below 16M paths
+    assert FrameRenderer(cam, integ, DeviceFilm(cam)).lanes == 1  # automatic: one lane below 4M paths
 
 
 def test_batch_rejects_bad_input(gpu):
