@@ -15,6 +15,19 @@ from .camera import DeviceFilm, PerspectiveCamera
 from .integrator import VolumePathIntegrator
 
 
+def lane_plan(path_count: int, spp: int, lanes: int = 0, batch_paths: int = 1 << 25) -> tuple[list[int], int]:
+    """FrameRenderer's cut of a frame's paths (pixel-major, spp per pixel) into lanes: the lane
+    bounds (lane l renders paths [bounds[l], bounds[l + 1]), whole pixels, equal pixel counts up to
+    one) and the paths per launch sequence. lanes = 0: two lanes from 4M paths (2^22) on, else
+    one; never more lanes than pixels."""
+    pixels = path_count // spp
+    n = int(lanes) if lanes > 0 else (2 if path_count >= 1 << 22 else 1)
+    n = max(1, min(n, pixels))
+    bounds = [(pixels * l // n) * spp for l in range(n + 1)]
+    widest = max(b - a for a, b in zip(bounds, bounds[1:])) if path_count else spp
+    return bounds, max(spp, (min(batch_paths // n, widest) // spp) * spp)
+
+
 @dataclass
 class FrameRenderer:
     """One frame through the MIS integrator in launch sequences of whole pixels. Each sequence
@@ -47,17 +60,9 @@ class FrameRenderer:
 
     def __post_init__(self):
         import torch
-        spp = self.camera.spp
-        n = self.camera.path_count
-        pixels = n // spp
-        lanes = int(self.lanes) if self.lanes > 0 else (2 if n >= 1 << 22 else 1)
-        lanes = max(1, min(lanes, pixels))
-        self.lanes = lanes
+        self.bounds, self.batch = lane_plan(self.camera.path_count, self.camera.spp, self.lanes, self.batch_paths)
+        lanes = self.lanes = len(self.bounds) - 1
         self.side = [torch.cuda.Stream(device=self.film.light.device) for _ in range(lanes - 1)]
-        # lane l renders paths [bounds[l], bounds[l + 1]): whole pixels
-        self.bounds = [(pixels * l // lanes) * spp for l in range(lanes + 1)]
-        widest = max(b - a for a, b in zip(self.bounds, self.bounds[1:])) if n else spp
-        self.batch = max(spp, (min(self.batch_paths // lanes, widest) // spp) * spp)
         self.lane_paths = [torch.empty(self.batch * abi.GZ_PATH.itemsize, dtype=torch.uint8,
                                        device=self.film.light.device) for _ in range(lanes)]
         self.d_paths = self.lane_paths[0]
